@@ -1,0 +1,125 @@
+"""ctypes binding of oracle/lib/liboracle.so (see scp_oracle.h).
+
+TEST INFRASTRUCTURE ONLY (the checker); never imported by scp_amd.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "lib", "liboracle.so")
+REF_BIN = os.path.join(_HERE, "_ref", "scp_sim")
+
+DIAG_NAMES = ["E_phi_kin", "E_theta_kin", "E_grad", "E_mass", "E_pot", "E_tgrad", "E_tmass",
+              "E_coupling", "E_total", "phi_max", "P_max", "P_int", "theta_rms",
+              "Q_phi", "Q_theta", "Q_total", "s_max", "r_core", "omega_core", "Q_core",
+              "thp1u", "thp1v", "thp2u", "thp2v", "gauss_max", "gauss_l2", "E_em", "Q_flux",
+              "Q_p0", "Q_p1", "Q_p2"]
+
+
+class OrParams(C.Structure):
+    _fields_ = [("N", C.c_int), ("mode", C.c_int), ("bc_type", C.c_int), ("rmode", C.c_int),
+                ("L", C.c_double), ("dx", C.c_double), ("dt", C.c_double),
+                ("m2", C.c_double), ("mtheta2", C.c_double), ("eta", C.c_double),
+                ("mu", C.c_double), ("kappa", C.c_double), ("g", C.c_double),
+                ("kappa_h", C.c_double), ("alpha_cs", C.c_double), ("beta_h", C.c_double),
+                ("inv_alpha", C.c_double), ("inv_beta", C.c_double), ("kappa_gamma", C.c_double),
+                ("damp_width", C.c_double), ("damp_rate", C.c_double), ("G_offset", C.c_double),
+                ("qdiag_radius", C.c_double), ("qdiag_probe1", C.c_double),
+                ("qdiag_probe2", C.c_double)]
+
+
+def make_params(N, mode, L, dt_factor, **kw):
+    """Parameters with the reference defaults (scp_config.h:165-215)."""
+    dx = 2.0 * L / (N - 1)
+    p = OrParams(N=N, mode=mode, bc_type=kw.pop("bc_type", 2), rmode=kw.pop("rmode", 0),
+                 L=L, dx=dx, dt=dt_factor * dx, m2=2.25, mtheta2=0.0, eta=0.5, mu=-41.345,
+                 kappa=50.0, g=0.05, kappa_h=0.0, alpha_cs=0.0, beta_h=0.0, inv_alpha=2.25,
+                 inv_beta=5.0, kappa_gamma=2.0, damp_width=3.0, damp_rate=0.01, G_offset=0.0,
+                 qdiag_radius=8.0, qdiag_probe1=8.0, qdiag_probe2=16.0)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def _lib():
+    lib = C.CDLL(LIB)
+    P = C.POINTER(OrParams)
+    D = C.POINTER(C.c_double)
+    lib.or_forces.argtypes = [P, D, D]
+    lib.or_step.argtypes = [P, D, D, C.c_int]
+    lib.or_diag.argtypes = [P, D, D]
+    lib.or_net_charge.restype = C.c_double
+    lib.or_net_charge.argtypes = [P, D, D]
+    lib.or_gauss_project.argtypes = [P, D, D]
+    lib.or_gauge_transform.argtypes = [P, D]
+    return lib
+
+
+_L = None
+
+
+def lib():
+    global _L
+    if _L is None:
+        _L = _lib()
+    return _L
+
+
+def _ptr(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def ncols(mode):
+    return {0: 12, 1: 24, 2: 30}[mode]
+
+
+def nacc(mode):
+    return {0: 6, 1: 12, 2: 15}[mode]
+
+
+def forces(p, state):
+    n3 = p.N ** 3
+    acc = np.zeros((nacc(p.mode), n3))
+    st = np.ascontiguousarray(state, dtype=np.float64).reshape(ncols(p.mode), n3)
+    lib().or_forces(C.byref(p), _ptr(st), _ptr(acc))
+    return acc
+
+
+def step(p, state, nsteps, acc=None):
+    """Run nsteps of the reference KDK step; returns (state, acc). The initial
+    acceleration is F(state) unless given (the reference's initial force call)."""
+    st = np.array(state, dtype=np.float64).reshape(ncols(p.mode), -1).copy()
+    a = forces(p, st) if acc is None else np.array(acc, dtype=np.float64)
+    lib().or_step(C.byref(p), _ptr(st), _ptr(a), nsteps)
+    return st, a
+
+
+def diag(p, state):
+    st = np.ascontiguousarray(state, dtype=np.float64).reshape(ncols(p.mode), -1)
+    out = np.zeros(len(DIAG_NAMES))
+    lib().or_diag(C.byref(p), _ptr(st), _ptr(out))
+    return dict(zip(DIAG_NAMES, out))
+
+
+def gauss_project(p, state):
+    """Returns (projected state, iterations); p.G_offset is updated in place."""
+    st = np.array(state, dtype=np.float64).reshape(ncols(p.mode), -1).copy()
+    g0 = C.c_double()
+    it = lib().or_gauss_project(C.byref(p), _ptr(st), C.byref(g0))
+    return st, it
+
+
+def net_charge(p, state):
+    st = np.ascontiguousarray(state, dtype=np.float64).reshape(ncols(p.mode), -1)
+    qa = C.c_double()
+    qn = lib().or_net_charge(C.byref(p), _ptr(st), C.byref(qa))
+    return qn, qa.value
+
+
+def gauge_transform(p, state):
+    st = np.array(state, dtype=np.float64).reshape(ncols(p.mode), -1).copy()
+    lib().or_gauge_transform(C.byref(p), _ptr(st))
+    return st
