@@ -1,0 +1,116 @@
+/* scp_amd.h — C-ABI of the MI355X sfa/sim leapfrog step (libscp_amd.so).
+ *
+ * This is the boundary a host program binds (the drop-in scp_sim binary in
+ * scp_amd/csrc/host, Python via ctypes, Go via cgo — see INTEGRATION.md). Plain
+ * C types only: pointers, sizes, int status codes (0 = ok, <0 = error, message
+ * via scp_last_error). No process exit inside the library.
+ *
+ * Each entry point replaces one reference interface:
+ *   scp_grid_create    grid_alloc (sfa/sim/scp_sim.c:199-367) + gpu_alloc /
+ *                      gpu_set_constants (sfa/sim/scp_sim.cu:2659-2707, 2915-2948)
+ *   scp_grid_upload    gpu_upload (scp_sim.cu:2768-2820) + the initial force
+ *                      call (scp_sim.c:2833-2838)
+ *   scp_grid_step      n x verlet_step (scp_sim.c:1583-1716) / gpu_verlet_step
+ *                      (scp_sim.cu:2971-3208), boundary stage included
+ *   scp_grid_download  gpu_download (scp_sim.cu:2822-2864); the 12/24/30 snapshot
+ *                      columns of sfa_snap (scp_sim.c:2646-2700)
+ *   scp_grid_diag      compute_energy* + P_integrated* + theta_rms* +
+ *                      compute_charges + compute_gauss (scp_sim.c:1722-2243) /
+ *                      run_gpu_diagnostics (scp_sim.cu:3697-4041)
+ *   scp_grid_forces    compute_forces / _complex / _complex_gauge
+ *                      (scp_sim.c:743-994, 1053-1263): accelerations of the
+ *                      current state (used by the gauge-transform self-test,
+ *                      scp_sim.c:2461-2556)
+ *   scp_grid_gauss_project  init_gauss_project (scp_sim.c:2284-2400) on the GPU
+ *   scp_grid_rescale_phi    rescale_phi_kernel (scp_sim.cu:2619-2624)
+ *
+ * Column order of every host state buffer = the reference snapshot order
+ * (scp_sim.c:2862-2897): phi_x..z, theta_x..z, phi_vx..vz, theta_vx..vz,
+ * [phiim, thetaim, phiim_v, thetaim_v], [th_x..z, E_x..z]; site index
+ * i*N*N + j*N + k (z fastest).
+ */
+#ifndef SCP_AMD_H
+#define SCP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCP_ABI_VERSION 1
+
+enum { SCP_MODE_REAL = 0, SCP_MODE_COMPLEX = 1, SCP_MODE_GAUGED = 2 };
+enum { SCP_PREC_F64 = 0, SCP_PREC_F32C = 1 };
+
+typedef struct {
+    int32_t N;            /* sites per edge (periodic cube) */
+    int32_t mode;         /* SCP_MODE_* ; gauged with g == 0 runs the complex step (scp_sim.c:1586) */
+    int32_t precision;    /* SCP_PREC_F64: FP64 state and stencil (parity path);
+                             SCP_PREC_F32C: FP32 state + FP32 stencil with compensated
+                             (two-sum) position/velocity accumulation */
+    int32_t bc_type;      /* 0 absorbing sphere, 2 periodic (1 unsupported: returns error) */
+    int32_t rmode;        /* real-mode mass coupling: 0 const, 1 inverse, 3 density-kappa */
+    int32_t reserved;
+    double L, dx, dt;     /* dx = 2L/(N-1), dt = dt_factor*dx (scp_sim.c:204-205) */
+    double m2, mtheta2, eta, mu, kappa, g;
+    double kappa_h, alpha_cs, beta_h, inv_alpha, inv_beta, kappa_gamma;
+    double damp_width, damp_rate;
+    double G_offset;      /* frozen jellium Gauss offset (set by gauss_project or the host) */
+    double qdiag_radius, qdiag_probe1, qdiag_probe2;
+} scp_params;
+
+/* Diagnostics vector filled by scp_grid_diag: every value of one diag.tsv row
+ * (scp_sim.c:3040-3065), FP64, in this order. */
+enum {
+    SCP_D_E_PHI_KIN = 0, SCP_D_E_THETA_KIN, SCP_D_E_GRAD, SCP_D_E_MASS, SCP_D_E_POT,
+    SCP_D_E_TGRAD, SCP_D_E_TMASS, SCP_D_E_COUPLING, SCP_D_E_TOTAL, SCP_D_PHI_MAX,
+    SCP_D_P_MAX, SCP_D_P_INT, SCP_D_THETA_RMS,
+    SCP_D_Q_PHI, SCP_D_Q_THETA, SCP_D_Q_TOTAL, SCP_D_S_MAX, SCP_D_R_CORE, SCP_D_OMEGA_CORE,
+    SCP_D_Q_CORE, SCP_D_THP1U, SCP_D_THP1V, SCP_D_THP2U, SCP_D_THP2V,
+    SCP_D_GAUSS_MAX, SCP_D_GAUSS_L2, SCP_D_E_EM, SCP_D_Q_FLUX,
+    SCP_D_Q_P0, SCP_D_Q_P1, SCP_D_Q_P2,
+    SCP_D_COUNT
+};
+
+typedef struct scp_grid scp_grid;
+
+int  scp_abi_version(void);
+/* Copies the last error message of the calling thread; returns its length. */
+int  scp_last_error(char *buf, int buflen);
+/* Device HBM bytes a grid with these parameters allocates (admission control). */
+int64_t scp_grid_bytes(const scp_params *p);
+
+int  scp_grid_create(const scp_params *p, scp_grid **out);
+int  scp_grid_destroy(scp_grid *g);
+/* ncols must be 12/24/30 for real/complex/gauged. Uploads the state, runs the
+ * initial force evaluation and the first half-kick + drift. */
+int  scp_grid_upload(scp_grid *g, const double *const *cols, int ncols);
+/* Current positions and velocities (state after the last completed step). */
+int  scp_grid_download(scp_grid *g, double *const *cols, int ncols);
+/* Advance nsteps leapfrog steps (nsteps >= 1). */
+int  scp_grid_step(scp_grid *g, int nsteps);
+int  scp_grid_diag(scp_grid *g, double *out, int nout);
+/* Accelerations of the current positions: 6/12/15 columns in the order
+ * phi_acc, theta_acc, [phi_im_acc, theta_im_acc], [E_acc]. */
+int  scp_grid_forces(scp_grid *g, double *const *acc, int nacc);
+/* Gauss projection of the current E (gauged mode): CG solve, E correction,
+ * G_offset freeze. Writes iterations and the post-projection max|G - G_offset|. */
+int  scp_grid_gauss_project(scp_grid *g, int *iters, double *gauss_max);
+int  scp_grid_set_g_offset(scp_grid *g, double g_offset);
+double scp_grid_g_offset(const scp_grid *g);
+/* phi, phi_vel (and imaginary parts) *= f — the fixed-charge projection. */
+int  scp_grid_rescale_phi(scp_grid *g, double f);
+/* Steps completed since upload. */
+int64_t scp_grid_steps_done(const scp_grid *g);
+
+/* Timing of the step kernel on the grid's own stream (HIP events around
+ * nsteps launches; no host synchronisation inside the timed region). */
+int  scp_grid_time_steps(scp_grid *g, int nsteps, float *ms_total);
+/* Name of the step kernel as rocprof reports it (for profile matching). */
+const char *scp_grid_step_kernel_name(const scp_grid *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -5,6 +5,7 @@
 #include <string.h>
 
 #define TWO_PI 6.283185307179586476925286766559
+#define CUTOFF_SIGMAS 10.0    /* exp(-50) ~ 2e-22: contributions beyond are dropped */
 
 static inline uint64_t mix64(uint64_t z) {          /* splitmix64 finaliser */
     z += 0x9E3779B97F4A7C15ull;
@@ -38,36 +39,62 @@ int scp_seed_balls(int N, double L, int mode, const scp_ball *balls, int nballs,
     const long n3 = (long)N * N * N, NN = (long)N * N;
     const int ncol = mode == 2 ? 30 : 24;
     const double dx = 2.0 * L / (N - 1), box = N * dx;
-    memset(state, 0, sizeof(double) * (size_t)ncol * (size_t)n3);
+    #pragma omp parallel for schedule(static)
+    for (long c = 0; c < (long)ncol * n3; c++) state[c] = 0.0;
     double *u[3], *tu[3], *ud[3], *v[3], *tv[3], *vd[3];
     for (int a = 0; a < 3; a++) {
         u[a] = state + (0 + a) * n3;   tu[a] = state + (3 + a) * n3;  ud[a] = state + (6 + a) * n3;
         v[a] = state + (12 + a) * n3;  tv[a] = state + (15 + a) * n3; vd[a] = state + (18 + a) * n3;
     }
-    #pragma omp parallel for schedule(static)
-    for (long c = 0; c < n3; c++) {
-        int i = (int)(c / NN), j = (int)((c / N) % N), k = (int)(c % N);
-        double x = -L + i * dx, y = -L + j * dx, z = -L + k * dx;
-        for (int b = 0; b < nballs; b++) {
-            const scp_ball *B = &balls[b];
-            double ex = min_image(x - B->x0, box), ey = min_image(y - B->y0, box), ez = min_image(z - B->z0, box);
-            double r2 = ex * ex + ey * ey + ez * ez;
-            double env = B->amp * exp(-r2 / (2.0 * B->sigma * B->sigma));
-            for (int a = 0; a < 3; a++) {
-                double ph = B->phase + B->winding * TWO_PI * a / 3.0;
-                double re = env * cos(ph), im = env * sin(ph);
-                u[a][c] += re; v[a][c] += im;
-                ud[a][c] += -B->omega * im;     /* d/dt (re + i im) = i omega (re + i im) */
-                vd[a][c] += B->omega * re;
+    /* balls one at a time over their bounding boxes (periodic), so overlapping
+     * balls never race; the site loop inside a ball is parallel */
+    for (int b = 0; b < nballs; b++) {
+        const scp_ball *B = &balls[b];
+        int half = (int)ceil(CUTOFF_SIGMAS * B->sigma / dx);
+        const int whole = 2 * half + 1 >= N;          /* box covers the grid: visit every site once */
+        const int span = whole ? N : 2 * half + 1;
+        if (whole) half = 0;
+        const int ci = whole ? 0 : (int)lround((B->x0 + L) / dx);
+        const int cj = whole ? 0 : (int)lround((B->y0 + L) / dx);
+        const int ck = whole ? 0 : (int)lround((B->z0 + L) / dx);
+        double cre[3], cim[3];
+        for (int a = 0; a < 3; a++) {
+            double ph = B->phase + B->winding * TWO_PI * a / 3.0;
+            cre[a] = cos(ph); cim[a] = sin(ph);
+        }
+        #pragma omp parallel for schedule(static)
+        for (int di = 0; di < span; di++) {
+            int i = ((ci - half + di) % N + N) % N;
+            double x = -L + i * dx, ex = min_image(x - B->x0, box);
+            for (int dj = 0; dj < span; dj++) {
+                int j = ((cj - half + dj) % N + N) % N;
+                double y = -L + j * dx, ey = min_image(y - B->y0, box);
+                for (int dk = 0; dk < span; dk++) {
+                    int k = ((ck - half + dk) % N + N) % N;
+                    double z = -L + k * dx, ez = min_image(z - B->z0, box);
+                    long c = (long)i * NN + (long)j * N + k;
+                    double r2 = ex * ex + ey * ey + ez * ez;
+                    double env = B->amp * exp(-r2 / (2.0 * B->sigma * B->sigma));
+                    for (int a = 0; a < 3; a++) {
+                        double re = env * cre[a], im = env * cim[a];
+                        u[a][c] += re; v[a][c] += im;
+                        ud[a][c] += -B->omega * im;     /* d/dt (re + i im) = i omega (re + i im) */
+                        vd[a][c] += B->omega * re;
+                    }
+                }
             }
         }
+    }
+    #pragma omp parallel for schedule(static)
+    for (long c = 0; c < n3; c++) {
         for (int a = 0; a < 3; a++) {
             tu[a][c] = theta_sigma * scp_rng_normal(seed, (uint64_t)a, (uint64_t)c);
             tv[a][c] = theta_sigma * scp_rng_normal(seed, (uint64_t)(3 + a), (uint64_t)c);
         }
         if (mode == 2)
             for (int d = 0; d < 3; d++)
-                state[(24 + d) * n3 + c] = link_sigma * scp_rng_normal(seed, (uint64_t)(6 + d), (uint64_t)c);
+                state[(24 + d) * n3 + c] = link_sigma == 0.0 ? 0.0
+                    : link_sigma * scp_rng_normal(seed, (uint64_t)(6 + d), (uint64_t)c);
     }
     return 0;
 }

@@ -1,0 +1,545 @@
+// scp_abi.hip — the C-ABI of include/scp_amd.h over the HIP kernels.
+//
+// Device state of a grid (one allocation per group, fields 64-element aligned):
+//   P[2]  positions, ping-pong (the stencil source and the drift target)
+//   W     half-kicked velocities / E, updated in place
+//   xo,vo positions and velocities of the last completed step (FP64), written
+//         by the last launch of every scp_grid_step call and by upload
+// See scp_step.hip for the step schedule.
+#include "scp_amd.h"
+#include "scp_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+using namespace scp;
+
+struct scp_grid {
+    scp_params p;
+    KParams kp;
+    int mode;        // storage mode (0/1/2), number of columns follows it
+    int step_mode;   // kernel mode: 2 only when gauged with g != 0
+    int NP;
+    hipStream_t stream;
+    int device;
+    double *P[2];
+    double *W;
+    double *xo, *vo;
+    double *aux;     // real-mode Cosserat/hardening fields (6), when needed
+    double *acc;     // forces output (lazily allocated)
+    double *partials;
+    double *dout;
+    unsigned long long *argmin;
+    int cur;
+    int nblocks_diag;
+    int64_t steps;
+    hipEvent_t ev0, ev1;
+    char kname[96];
+};
+
+static thread_local char g_err[512];
+
+static int set_err(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(x)                                                                    \
+    do {                                                                             \
+        hipError_t e_ = (x);                                                         \
+        if (e_ != hipSuccess)                                                        \
+            return set_err(-2, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_),  \
+                           __FILE__, __LINE__);                                      \
+    } while (0)
+
+static int kerr(int rc, const char *what) {
+    if (rc != 0) return set_err(-2, "%s launch failed: %s", what, hipGetErrorString((hipError_t)rc));
+    return 0;
+}
+
+extern "C" int scp_abi_version(void) { return SCP_ABI_VERSION; }
+
+extern "C" int scp_last_error(char *buf, int n) {
+    int len = (int)strlen(g_err);
+    if (buf && n > 0) {
+        strncpy(buf, g_err, (size_t)n - 1);
+        buf[n - 1] = 0;
+    }
+    return len;
+}
+
+static int ncols_of(int mode) { return mode == 2 ? 30 : (mode == 1 ? 24 : 12); }
+
+// device field f (position, or velocity when vel) -> SFA snapshot column
+static int col_of(int mode, int f, bool vel) {
+    if (mode == 0) return vel ? 6 + f : f;        // u0..2 -> 0..2, tu0..2 -> 3..5
+    static const int POS[15] = {0, 1, 2, 12, 13, 14, 3, 4, 5, 15, 16, 17, 24, 25, 26};
+    static const int VEL[15] = {6, 7, 8, 18, 19, 20, 9, 10, 11, 21, 22, 23, 27, 28, 29};
+    return vel ? VEL[f] : POS[f];
+}
+
+// device field f -> reference acceleration column (phi_acc, theta_acc, phi_im_acc, theta_im_acc, E_acc)
+static int acc_col_of(int mode, int f) {
+    if (mode == 0) return f;
+    static const int A[15] = {0, 1, 2, 6, 7, 8, 3, 4, 5, 9, 10, 11, 12, 13, 14};
+    return A[f];
+}
+
+static uint32_t field_stride(int64_t n3) { return (uint32_t)((n3 + 63) / 64 * 64); }
+
+static int check_params(const scp_params *p) {
+    if (!p) return set_err(-1, "null params");
+    if (p->N < 4 || p->N > 1290) return set_err(-1, "N=%d out of range [4, 1290]", p->N);
+    if (p->mode < 0 || p->mode > 2) return set_err(-1, "mode=%d invalid", p->mode);
+    if (p->precision != SCP_PREC_F64 && p->precision != SCP_PREC_F32C)
+        return set_err(-1, "precision=%d invalid", p->precision);
+    if (p->precision == SCP_PREC_F32C) return set_err(-3, "FP32C step is not built into this library");
+    if (p->bc_type == 1) return set_err(-3, "bc_type=1 (gradient pinned) is not implemented");
+    if (p->bc_type != 0 && p->bc_type != 2) return set_err(-1, "bc_type=%d invalid", p->bc_type);
+    if (!(p->dx > 0) || !(p->dt > 0)) return set_err(-1, "dx/dt must be positive");
+    if (p->mode == 2 && p->g < 0) return set_err(-1, "g must be >= 0");
+    return 0;
+}
+
+extern "C" int64_t scp_grid_bytes(const scp_params *p) {
+    if (check_params(p)) return -1;
+    int64_t n3 = (int64_t)p->N * p->N * p->N;
+    int64_t fs = field_stride(n3);
+    int NP = np_of_mode(p->mode);
+    int64_t b = 5LL * NP * fs * 8;                       // P[2], W, xo, vo
+    if (p->mode == 0 && (p->alpha_cs != 0 || p->beta_h != 0)) b += 6LL * fs * 8;
+    b += 1024LL * 32 * 8 + 4096;
+    return b;
+}
+
+static KParams make_kparams(const scp_params &p, int step_mode) {
+    KParams k;
+    memset(&k, 0, sizeof k);
+    k.N = p.N;
+    k.rmode = p.rmode;
+    int64_t n3 = (int64_t)p.N * p.N * p.N;
+    k.n3 = (uint32_t)n3;
+    k.fs = field_stride(n3);
+    const double dx = p.dx, dt = p.dt;
+    k.hdt = 0.5 * dt;                        // scp_sim.c:1584
+    k.dt = dt;
+    k.gad = -p.g * dx * dt;                  // scp_sim.c:1642
+    k.h1 = 1.0 / (2.0 * dx);                 // scp_sim.c:935
+    k.h2 = 1.0 / (dx * dx);                  // scp_sim.c:934
+    k.inva = 1.0 / dx;                       // scp_sim.c:1059
+    k.GG = p.g;
+    k.STP = step_mode == 2 ? 1.0 / (p.g * dx * dx * dx) : 0.0;   // scp_sim.c:1063
+    k.MU = p.mu; k.KAPPA = p.kappa; k.MASS2 = p.m2; k.MT2 = p.mtheta2; k.ETA = p.eta;
+    k.KH = p.kappa_h; k.ACS = p.alpha_cs; k.BH = p.beta_h;
+    k.IA = p.inv_alpha; k.IB = p.inv_beta; k.KG = p.kappa_gamma;
+    k.L = p.L; k.dx = dx;
+    k.Rd = p.L - p.damp_width;               // scp_sim.c:1444
+    k.DW = p.damp_width; k.DR = p.damp_rate;
+    k.damp = (p.bc_type == 0 && p.damp_width > 0 && p.damp_rate > 0) ? 1 : 0;
+    return k;
+}
+
+static void free_grid(scp_grid *g) {
+    if (!g) return;
+    hipFree(g->P[0]); hipFree(g->P[1]); hipFree(g->W); hipFree(g->xo); hipFree(g->vo);
+    hipFree(g->aux); hipFree(g->acc); hipFree(g->partials); hipFree(g->dout); hipFree(g->argmin);
+    if (g->ev0) hipEventDestroy(g->ev0);
+    if (g->ev1) hipEventDestroy(g->ev1);
+    if (g->stream) hipStreamDestroy(g->stream);
+    delete g;
+}
+
+extern "C" int scp_grid_create(const scp_params *p, scp_grid **out) {
+    if (!out) return set_err(-1, "null out");
+    *out = nullptr;
+    int rc = check_params(p);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(-4, "no HIP device");
+    scp_grid *g = new scp_grid;
+    memset(g, 0, sizeof *g);
+    g->p = *p;
+    g->mode = p->mode;
+    g->step_mode = (p->mode == 2 && p->g == 0.0) ? 1 : p->mode;
+    g->NP = np_of_mode(p->mode);
+    g->kp = make_kparams(*p, g->step_mode);
+    hipGetDevice(&g->device);
+    const size_t fsb = (size_t)g->kp.fs * sizeof(double);
+    const size_t grp = fsb * g->NP;
+    hipError_t e = hipSuccess;
+    e = e ? e : hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+    e = e ? e : hipMalloc(&g->P[0], grp);
+    e = e ? e : hipMalloc(&g->P[1], grp);
+    e = e ? e : hipMalloc(&g->W, grp);
+    e = e ? e : hipMalloc(&g->xo, grp);
+    e = e ? e : hipMalloc(&g->vo, grp);
+    if (p->mode == 0 && (p->alpha_cs != 0 || p->beta_h != 0)) {
+        e = e ? e : hipMalloc(&g->aux, 6 * fsb);
+        e = e ? e : hipMemsetAsync(g->aux, 0, 6 * fsb, g->stream);
+    }
+    g->nblocks_diag = (int)std::min<int64_t>(1024, ((int64_t)g->kp.n3 + DIAG_BLOCK - 1) / DIAG_BLOCK);
+    e = e ? e : hipMalloc(&g->partials, (size_t)g->nblocks_diag * 32 * sizeof(double));
+    e = e ? e : hipMalloc(&g->dout, 64 * sizeof(double));
+    e = e ? e : hipMalloc(&g->argmin, sizeof(unsigned long long));
+    e = e ? e : hipEventCreate(&g->ev0);
+    e = e ? e : hipEventCreate(&g->ev1);
+    if (e != hipSuccess) {
+        set_err(-2, "allocation failed: %s", hipGetErrorString(e));
+        free_grid(g);
+        return -2;
+    }
+    snprintf(g->kname, sizeof g->kname, "k_step64<%d, 1, false>", g->step_mode);
+    *out = g;
+    return 0;
+}
+
+extern "C" int scp_grid_destroy(scp_grid *g) {
+    if (!g) return 0;
+    hipStreamSynchronize(g->stream);
+    free_grid(g);
+    return 0;
+}
+
+static Bufs<double> bufs_for(scp_grid *g, int kind) {
+    Bufs<double> b;
+    memset(&b, 0, sizeof b);
+    b.xo = g->xo;
+    b.vo = g->vo;
+    b.w = g->W;
+    b.aux = g->aux;
+    b.acc = g->acc;
+    if (kind == K_INIT) { b.pin = g->xo; b.pout = g->P[0]; }
+    else if (kind == K_STEP) { b.pin = g->P[g->cur]; b.pout = g->P[g->cur ^ 1]; }
+    else { b.pin = g->xo; }
+    return b;
+}
+
+static int run_aux(scp_grid *g, const double *src) {
+    if (!g->aux) return 0;
+    return kerr(launch_real_aux(g->kp, src, g->aux, g->stream), "real aux");
+}
+
+// initial force + first half kick + first drift, from xo/vo
+static int run_init(scp_grid *g) {
+    int rc = run_aux(g, g->xo);
+    if (rc) return rc;
+    rc = kerr(launch_step_f64(g->step_mode, K_INIT, false, g->kp, bufs_for(g, K_INIT), g->stream), "init");
+    if (rc) return rc;
+    g->cur = 0;
+    return 0;
+}
+
+extern "C" int scp_grid_upload(scp_grid *g, const double *const *cols, int ncols) {
+    if (!g || !cols) return set_err(-1, "null argument");
+    if (ncols != ncols_of(g->mode)) return set_err(-1, "ncols=%d, mode %d needs %d", ncols, g->mode, ncols_of(g->mode));
+    const size_t nb = (size_t)g->kp.n3 * sizeof(double);
+    for (int f = 0; f < g->NP; f++) {
+        const double *xp = cols[col_of(g->mode, f, false)], *vp = cols[col_of(g->mode, f, true)];
+        if (!xp || !vp) return set_err(-1, "null column");
+        HIPCHK(hipMemcpyAsync(g->xo + (size_t)f * g->kp.fs, xp, nb, hipMemcpyHostToDevice, g->stream));
+        HIPCHK(hipMemcpyAsync(g->vo + (size_t)f * g->kp.fs, vp, nb, hipMemcpyHostToDevice, g->stream));
+    }
+    int rc = run_init(g);
+    if (rc) return rc;
+    g->steps = 0;
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+extern "C" int scp_grid_download(scp_grid *g, double *const *cols, int ncols) {
+    if (!g || !cols) return set_err(-1, "null argument");
+    if (ncols != ncols_of(g->mode)) return set_err(-1, "ncols=%d, mode %d needs %d", ncols, g->mode, ncols_of(g->mode));
+    const size_t nb = (size_t)g->kp.n3 * sizeof(double);
+    for (int f = 0; f < g->NP; f++) {
+        double *xp = cols[col_of(g->mode, f, false)], *vp = cols[col_of(g->mode, f, true)];
+        if (!xp || !vp) return set_err(-1, "null column");
+        HIPCHK(hipMemcpyAsync(xp, g->xo + (size_t)f * g->kp.fs, nb, hipMemcpyDeviceToHost, g->stream));
+        HIPCHK(hipMemcpyAsync(vp, g->vo + (size_t)f * g->kp.fs, nb, hipMemcpyDeviceToHost, g->stream));
+    }
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+static int enqueue_steps(scp_grid *g, int nsteps) {
+    for (int s = 0; s < nsteps; s++) {
+        int rc = run_aux(g, g->P[g->cur]);
+        if (rc) return rc;
+        const bool emit = (s == nsteps - 1);
+        rc = kerr(launch_step_f64(g->step_mode, K_STEP, emit, g->kp, bufs_for(g, K_STEP), g->stream), "step");
+        if (rc) return rc;
+        g->cur ^= 1;
+    }
+    g->steps += nsteps;
+    return 0;
+}
+
+extern "C" int scp_grid_step(scp_grid *g, int nsteps) {
+    if (!g) return set_err(-1, "null grid");
+    if (nsteps < 1) return set_err(-1, "nsteps=%d must be >= 1", nsteps);
+    int rc = enqueue_steps(g, nsteps);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+extern "C" int scp_grid_time_steps(scp_grid *g, int nsteps, float *ms) {
+    if (!g || !ms) return set_err(-1, "null argument");
+    if (nsteps < 1) return set_err(-1, "nsteps must be >= 1");
+    HIPCHK(hipEventRecord(g->ev0, g->stream));
+    int rc = enqueue_steps(g, nsteps);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(g->ev1, g->stream));
+    HIPCHK(hipEventSynchronize(g->ev1));
+    HIPCHK(hipEventElapsedTime(ms, g->ev0, g->ev1));
+    return 0;
+}
+
+extern "C" const char *scp_grid_step_kernel_name(const scp_grid *g) { return g ? g->kname : ""; }
+
+extern "C" int64_t scp_grid_steps_done(const scp_grid *g) { return g ? g->steps : -1; }
+
+extern "C" int scp_grid_set_g_offset(scp_grid *g, double off) {
+    if (!g) return set_err(-1, "null grid");
+    g->p.G_offset = off;
+    return 0;
+}
+
+extern "C" double scp_grid_g_offset(const scp_grid *g) { return g ? g->p.G_offset : 0.0; }
+
+static int fetch(scp_grid *g, double *host, const double *dev, size_t n) {
+    HIPCHK(hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+extern "C" int scp_grid_diag(scp_grid *g, double *out, int nout) {
+    if (!g || !out) return set_err(-1, "null argument");
+    if (nout < SCP_D_COUNT) return set_err(-1, "nout=%d < %d", nout, SCP_D_COUNT);
+    for (int q = 0; q < nout; q++) out[q] = 0.0;
+    const scp_params &p = g->p;
+    const KParams &kp = g->kp;
+    const double dx = p.dx, dV = dx * dx * dx;
+    const int64_t n3 = kp.n3;
+    DiagParams dp;
+    memset(&dp, 0, sizeof dp);
+    dp.R2core = p.qdiag_radius * p.qdiag_radius;
+    const double Rint = (p.bc_type == 0) ? (p.L - p.damp_width - dx) : 1e300;   // C:2194
+    dp.Rint2 = Rint * Rint;
+    dp.Rcube = p.qdiag_radius;
+    dp.G_offset = p.G_offset;
+    dp.gauged = (g->mode == 2 && p.g != 0.0) ? 1 : 0;
+    dp.has_gauge = g->mode == 2;
+    dp.BPRE = dp.gauged ? 1.0 / (p.g * p.g * dx * dx * dx * dx) : 0.0;
+    dp.dV = dV;
+    const int nb = g->nblocks_diag;
+    int rc = kerr(launch_diag_pass1(g->mode, kp, dp, g->xo, g->vo, g->partials, nb, g->stream), "diag1");
+    if (rc) return rc;
+    rc = kerr(launch_reduce(g->partials, nb, DS_NSUM, DM_NMAX, g->dout, g->stream), "reduce");
+    if (rc) return rc;
+    double r[DS_NSUM + DM_NMAX];
+    if ((rc = fetch(g, r, g->dout, DS_NSUM + DM_NMAX))) return rc;
+    const double *S = r, *M = r + DS_NSUM;
+    out[SCP_D_E_PHI_KIN] = S[DS_EPK]; out[SCP_D_E_THETA_KIN] = S[DS_ETK];
+    out[SCP_D_E_GRAD] = S[DS_EG]; out[SCP_D_E_MASS] = S[DS_EM]; out[SCP_D_E_POT] = S[DS_EP];
+    out[SCP_D_E_TGRAD] = S[DS_ETG]; out[SCP_D_E_TMASS] = S[DS_ETM]; out[SCP_D_E_COUPLING] = S[DS_EC];
+    double et = S[DS_EPK] + S[DS_ETK] + S[DS_EG] + S[DS_EM] + S[DS_EP] + S[DS_ETG] + S[DS_ETM] + S[DS_EC];
+    if (dp.gauged) et = et + S[DS_EEM_E];
+    out[SCP_D_E_TOTAL] = et;
+    out[SCP_D_PHI_MAX] = M[DM_PHI];
+    out[SCP_D_P_MAX] = M[DM_P];
+    out[SCP_D_P_INT] = S[DS_PINT];
+    out[SCP_D_THETA_RMS] = std::sqrt(S[DS_TH2] / ((g->mode == 0 ? 3.0 : 6.0) * (double)n3));
+    if (g->mode == 0) return 0;
+
+    // charges (C:2110-2168)
+    out[SCP_D_Q_PHI] = S[DS_QP] * dV;
+    out[SCP_D_Q_THETA] = S[DS_QT] * dV;
+    out[SCP_D_Q_TOTAL] = out[SCP_D_Q_PHI] + out[SCP_D_Q_THETA];
+    out[SCP_D_S_MAX] = M[DM_S];
+    out[SCP_D_Q_CORE] = S[DS_QC] * dV;
+    out[SCP_D_Q_P0] = S[DS_QP0] * dV; out[SCP_D_Q_P1] = S[DS_QP1] * dV; out[SCP_D_Q_P2] = S[DS_QP2] * dV;
+    const int N = p.N;
+    const int64_t NN = (int64_t)N * N;
+    {   // theta probes (C:2142-2150)
+        auto clampi = [N](long v) { return (int)(v < 0 ? 0 : (v > N - 1 ? N - 1 : v)); };
+        int jc = clampi(lround(p.L / dx));
+        int i1 = clampi(lround((p.qdiag_probe1 + p.L) / dx));
+        int i2 = clampi(lround((p.qdiag_probe2 + p.L) / dx));
+        int64_t p1 = i1 * NN + (int64_t)jc * N + jc, p2 = i2 * NN + (int64_t)jc * N + jc;
+        double v[4];
+        if ((rc = fetch(g, &v[0], g->xo + (size_t)(F_TU + 1) * kp.fs + p1, 1))) return rc;
+        if ((rc = fetch(g, &v[1], g->xo + (size_t)(F_TV + 1) * kp.fs + p1, 1))) return rc;
+        if ((rc = fetch(g, &v[2], g->xo + (size_t)(F_TU + 1) * kp.fs + p2, 1))) return rc;
+        if ((rc = fetch(g, &v[3], g->xo + (size_t)(F_TV + 1) * kp.fs + p2, 1))) return rc;
+        out[SCP_D_THP1U] = v[0]; out[SCP_D_THP1V] = v[1]; out[SCP_D_THP2U] = v[2]; out[SCP_D_THP2V] = v[3];
+    }
+    double M0 = S[DS_M0] * dV;
+    double cx = 0, cy = 0, cz = 0;
+    bool rcore = M0 >= 1e-30;
+    if (rcore) { cx = S[DS_CX] * dV / M0; cy = S[DS_CY] * dV / M0; cz = S[DS_CZ] * dV / M0; }
+    const double thr = M[DM_S] * (1.0 - 1e-12);
+    unsigned long long init = ~0ull;
+    HIPCHK(hipMemcpyAsync(g->argmin, &init, sizeof init, hipMemcpyHostToDevice, g->stream));
+    rc = kerr(launch_diag_pass2(kp, g->xo, thr, cx, cy, cz, g->partials, g->argmin, nb, g->stream), "diag2");
+    if (rc) return rc;
+    rc = kerr(launch_reduce(g->partials, nb, 1, 0, g->dout, g->stream), "reduce2");
+    if (rc) return rc;
+    double rsum;
+    if ((rc = fetch(g, &rsum, g->dout, 1))) return rc;
+    unsigned long long am;
+    HIPCHK(hipMemcpyAsync(&am, g->argmin, sizeof am, hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    out[SCP_D_R_CORE] = rcore ? std::sqrt(rsum * dV / M0) : 0.0;
+    out[SCP_D_OMEGA_CORE] = 0.0;
+    if (am != ~0ull) {
+        double u0, v0, ud0, vd0;
+        if ((rc = fetch(g, &u0, g->xo + (size_t)F_U * kp.fs + am, 1))) return rc;
+        if ((rc = fetch(g, &v0, g->xo + (size_t)F_V * kp.fs + am, 1))) return rc;
+        if ((rc = fetch(g, &ud0, g->vo + (size_t)F_U * kp.fs + am, 1))) return rc;
+        if ((rc = fetch(g, &vd0, g->vo + (size_t)F_V * kp.fs + am, 1))) return rc;
+        double den = u0 * u0 + v0 * v0;
+        if (den >= 1e-20) out[SCP_D_OMEGA_CORE] = (u0 * vd0 - v0 * ud0) / den;
+    }
+    if (g->mode == 2) {   // compute_gauss (C:2239-2242)
+        out[SCP_D_GAUSS_MAX] = M[DM_GAUSS];
+        out[SCP_D_GAUSS_L2] = S[DS_NOMEGA] > 0 ? std::sqrt(S[DS_GL2] / S[DS_NOMEGA]) : 0.0;
+        out[SCP_D_E_EM] = S[DS_EEM_G];
+        out[SCP_D_Q_FLUX] = (p.g != 0) ? (S[DS_FSUM] - p.G_offset * S[DS_NC] * dV) / p.g : 0.0;
+    }
+    return 0;
+}
+
+extern "C" int scp_grid_forces(scp_grid *g, double *const *acc, int nacc) {
+    if (!g || !acc) return set_err(-1, "null argument");
+    const int want = g->mode == 2 ? 15 : (g->mode == 1 ? 12 : 6);
+    if (nacc != want) return set_err(-1, "nacc=%d, mode %d needs %d", nacc, g->mode, want);
+    if (!g->acc) HIPCHK(hipMalloc(&g->acc, (size_t)g->NP * g->kp.fs * sizeof(double)));
+    HIPCHK(hipMemsetAsync(g->acc, 0, (size_t)g->NP * g->kp.fs * sizeof(double), g->stream));
+    int rc = run_aux(g, g->xo);
+    if (rc) return rc;
+    rc = kerr(launch_step_f64(g->step_mode, K_FORCES, false, g->kp, bufs_for(g, K_FORCES), g->stream), "forces");
+    if (rc) return rc;
+    const size_t nb = (size_t)g->kp.n3 * sizeof(double);
+    for (int f = 0; f < g->NP; f++)
+        HIPCHK(hipMemcpyAsync(acc[acc_col_of(g->mode, f)], g->acc + (size_t)f * g->kp.fs, nb,
+                              hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+extern "C" int scp_grid_rescale_phi(scp_grid *g, double f) {
+    if (!g) return set_err(-1, "null grid");
+    const int nphi = g->mode == 0 ? 3 : 6;    // u (and v)
+    for (int q = 0; q < nphi; q++) {
+        int fld = q < 3 ? F_U + q : F_V + (q - 3);
+        int rc = kerr(launch_scale(g->xo + (size_t)fld * g->kp.fs, f, g->kp.n3, g->stream), "scale");
+        if (rc) return rc;
+        rc = kerr(launch_scale(g->vo + (size_t)fld * g->kp.fs, f, g->kp.n3, g->stream), "scale");
+        if (rc) return rc;
+    }
+    int rc = run_init(g);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+// Gauss projection on the device, same algorithm and stopping rule as the
+// reference (CG on -lap chi = G - Gbar, chi kept zero-mean, tol 1e-13*max(1,|b|max),
+// at most 20000 iterations), in FP64.
+extern "C" int scp_grid_gauss_project(scp_grid *g, int *iters, double *gauss_max) {
+    if (!g) return set_err(-1, "null grid");
+    if (g->mode != 2) return set_err(-1, "gauss projection needs the gauged mode");
+    const KParams &kp = g->kp;
+    const uint32_t n = kp.n3;
+    const double N3 = (double)n;
+    const int nb = g->nblocks_diag;
+    double *b = nullptr, *x = nullptr, *r = nullptr, *q = nullptr, *Aq = nullptr;
+    const size_t bytes = (size_t)n * sizeof(double);
+    int rc = 0;
+    auto cleanup = [&]() { hipFree(b); hipFree(x); hipFree(r); hipFree(q); hipFree(Aq); };
+    if (hipMalloc(&b, bytes) || hipMalloc(&x, bytes) || hipMalloc(&r, bytes) || hipMalloc(&q, bytes) ||
+        hipMalloc(&Aq, bytes)) {
+        cleanup();
+        return set_err(-2, "gauss projection alloc failed");
+    }
+    double hv[3];
+    // b = G; gbar; b -= gbar; gmax0
+    rc = kerr(launch_gauss_residual(kp, g->xo, g->vo, b, g->stream), "gauss residual");
+    // sum of b: dot with a ones vector stored in x temporarily
+    if (!rc) {
+        std::vector<double> ones(n, 1.0);
+        if (hipMemcpyAsync(x, ones.data(), bytes, hipMemcpyHostToDevice, g->stream)) rc = set_err(-2, "copy");
+        if (!rc) rc = kerr(launch_dot_partials(b, x, n, g->partials, nb, g->stream), "dot");
+        if (!rc) rc = kerr(launch_reduce(g->partials, nb, 1, 0, g->dout, g->stream), "reduce");
+        if (!rc) rc = fetch(g, hv, g->dout, 1);
+        hipStreamSynchronize(g->stream);
+    }
+    double gbar = rc ? 0 : hv[0] / N3, gmax0 = 0;
+    if (!rc) rc = kerr(launch_sub_mean_max(b, gbar, n, g->partials, nb, g->stream), "submean");
+    if (!rc) rc = kerr(launch_reduce(g->partials, nb, 0, 1, g->dout, g->stream), "reduce");
+    if (!rc) rc = fetch(g, &gmax0, g->dout, 1);
+    const double tol = 1e-13 * std::fmax(1.0, gmax0);
+    if (!rc && (hipMemsetAsync(x, 0, bytes, g->stream) || hipMemcpyAsync(r, b, bytes, hipMemcpyDeviceToDevice, g->stream) ||
+                hipMemcpyAsync(q, b, bytes, hipMemcpyDeviceToDevice, g->stream)))
+        rc = set_err(-2, "copy");
+    double rz = 0;
+    if (!rc) rc = kerr(launch_dot_partials(r, r, n, g->partials, nb, g->stream), "dot");
+    if (!rc) rc = kerr(launch_reduce(g->partials, nb, 1, 0, g->dout, g->stream), "reduce");
+    if (!rc) rc = fetch(g, &rz, g->dout, 1);
+    int it = 0;
+    double rmax = gmax0;
+    while (!rc && rmax > tol && it < 20000) {
+        double qAq = 0;
+        rc = kerr(launch_neg_lap(kp, q, Aq, g->stream), "neglap");
+        if (!rc) rc = kerr(launch_dot_partials(q, Aq, n, g->partials, nb, g->stream), "dot");
+        if (!rc) rc = kerr(launch_reduce(g->partials, nb, 1, 0, g->dout, g->stream), "reduce");
+        if (!rc) rc = fetch(g, &qAq, g->dout, 1);
+        if (rc) break;
+        double alpha = rz / qAq;
+        rc = kerr(launch_axpy_cg(alpha, q, Aq, x, r, n, g->partials, nb, g->stream), "axpy");
+        if (!rc) rc = kerr(launch_reduce(g->partials, nb, 2, 1, g->dout, g->stream), "reduce");
+        if (!rc) rc = fetch(g, hv, g->dout, 3);
+        if (rc) break;
+        double rz2 = hv[0], xbar = hv[1] / N3, rm = hv[2];
+        double beta = rz2 / rz;
+        rc = kerr(launch_cg_update(xbar, beta, r, x, q, n, g->stream), "cgupdate");
+        rz = rz2;
+        rmax = rm;
+        it++;
+    }
+    if (!rc && rmax > tol)
+        rc = set_err(-5, "Gauss projection CG failed to converge (%d iters, max|res|=%.3e > tol=%.3e)", it, rmax, tol);
+    if (!rc) rc = kerr(launch_add_grad(kp, x, g->vo + (size_t)F_TH * kp.fs, g->stream), "addgrad");
+    // G_offset = mean of the corrected residual; report max |G - G_offset|
+    if (!rc) rc = kerr(launch_gauss_residual(kp, g->xo, g->vo, b, g->stream), "gauss residual");
+    if (!rc) {
+        std::vector<double> ones(n, 1.0);
+        if (hipMemcpyAsync(x, ones.data(), bytes, hipMemcpyHostToDevice, g->stream)) rc = set_err(-2, "copy");
+        if (!rc) rc = kerr(launch_dot_partials(b, x, n, g->partials, nb, g->stream), "dot");
+        if (!rc) rc = kerr(launch_reduce(g->partials, nb, 1, 0, g->dout, g->stream), "reduce");
+        if (!rc) rc = fetch(g, hv, g->dout, 1);
+        hipStreamSynchronize(g->stream);
+    }
+    double gres = 0;
+    if (!rc) {
+        g->p.G_offset = hv[0] / N3;
+        rc = kerr(launch_sub_mean_max(b, g->p.G_offset, n, g->partials, nb, g->stream), "submean");
+        if (!rc) rc = kerr(launch_reduce(g->partials, nb, 0, 1, g->dout, g->stream), "reduce");
+        if (!rc) rc = fetch(g, &gres, g->dout, 1);
+    }
+    if (!rc) rc = run_init(g);
+    hipStreamSynchronize(g->stream);
+    cleanup();
+    if (rc) return rc;
+    if (iters) *iters = it;
+    if (gauss_max) *gauss_max = gres;
+    return 0;
+}

@@ -1,0 +1,104 @@
+// scp_kernels.h — internal interface between the HIP kernels (scp_step.hip)
+// and the C-ABI layer (scp_abi.hip). Not installed; include/scp_amd.h is the
+// public boundary.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace scp {
+
+// Stencil ("position") fields on the device, per mode:
+//   real    NP=6 : u0..2, tu0..2
+//   complex NP=12: u0..2, v0..2, tu0..2, tv0..2
+//   gauged  NP=15: u0..2, v0..2, tu0..2, tv0..2, th0..2
+// Velocity-like fields use the same numbering (ud, vd, tud, tvd, E).
+enum { F_U = 0, F_V = 3, F_TU = 6, F_TV = 9, F_TH = 12 };
+
+inline int np_of_mode(int mode) { return mode == 2 ? 15 : (mode == 1 ? 12 : 6); }
+
+// Physics constants, derived on the host with the reference's own expressions
+// (scp_sim.c:204-205, 746-747, 1057-1063, 1642) so they round identically.
+struct KParams {
+    int32_t N, rmode;
+    uint32_t n3, fs;           // sites, field stride (elements, 64-aligned)
+    double hdt, dt, gad;       // 0.5*dt, dt, -g*dx*dt
+    double h1, h2, inva;       // 1/(2dx), 1/dx^2, 1/dx
+    double STP, GG;            // 1/(g dx^3), g
+    double MU, KAPPA, MASS2, MT2, ETA;
+    double KH, ACS, BH, IA, IB, KG;   // real-mode optional terms
+    double L, dx, Rd, DW, DR;  // sponge: R_damp = L - damp_width
+    int32_t damp;              // 1 when the sponge is active (bc 0 with width, rate > 0)
+    int32_t pad;
+};
+
+// Device buffers. Each group is one allocation of nfields * fs elements.
+template <typename T>
+struct Bufs {
+    const T *pin;   // positions at the stencil time level (read with neighbours)
+    T *pout;        // positions one drift later
+    T *w;           // half-kicked velocities (read and written at the own site)
+    T *wlo;         // compensation words of w (FP32C only)
+    const T *pinlo; // compensation words of pin (FP32C only)
+    T *poutlo;
+    double *xo;     // emitted positions (FP64), also the INIT source
+    double *vo;     // emitted velocities (FP64), also the INIT source
+    double *acc;    // forces-only output (FP64)
+    const double *aux;  // real mode: Cosserat mismatch M[3] and hardening Q[3] (6 fields)
+};
+
+enum StepKind { K_INIT = 0, K_STEP = 1, K_FORCES = 2 };
+
+// Launchers (scp_step.hip). Return hipError_t as int.
+int launch_step_f64(int mode, int kind, bool emit, const KParams &kp, const Bufs<double> &b,
+                    hipStream_t s);
+int launch_step_f32c(int mode, int kind, bool emit, const KParams &kp, const Bufs<float> &b,
+                     hipStream_t s);
+int launch_real_aux(const KParams &kp, const double *pin, double *aux, hipStream_t s);
+int launch_real_aux_f32(const KParams &kp, const float *pin, double *aux, hipStream_t s);
+
+// Diagnostics: pass 1 writes per-block partials; the finish kernel reduces
+// them in a fixed order into out (deterministic).
+enum {
+    DS_EPK = 0, DS_ETK, DS_EG, DS_EM, DS_EP, DS_ETG, DS_ETM, DS_EC, DS_EEM_E, DS_PINT, DS_TH2,
+    DS_QP, DS_QT, DS_QC, DS_QP0, DS_QP1, DS_QP2, DS_M0, DS_CX, DS_CY, DS_CZ,
+    DS_GL2, DS_NOMEGA, DS_EEM_G, DS_FSUM, DS_NC,
+    DS_NSUM
+};
+enum { DM_PHI = 0, DM_P, DM_S, DM_GAUSS, DM_NMAX };
+struct DiagParams {
+    double R2core;      // qdiag_radius^2
+    double Rint2;       // Gauss interior radius^2
+    double Rcube;       // Q_flux cube half-width
+    double G_offset;
+    double BPRE;        // 1/(g^2 dx^4)
+    double dV;
+    int32_t gauged;     // g != 0
+    int32_t has_gauge;  // gauge sector present (complex_gauge)
+};
+int launch_diag_pass1(int mode, const KParams &kp, const DiagParams &dp, const double *xo,
+                      const double *vo, double *partials, int nblocks, hipStream_t s);
+int launch_diag_pass2(const KParams &kp, const double *xo, double thr, double cx, double cy,
+                      double cz, double *partials, unsigned long long *argmin, int nblocks,
+                      hipStream_t s);
+int launch_reduce(const double *partials, int nblocks, int nsum, int nmax, double *out,
+                  hipStream_t s);
+
+// Gauss-law helpers (FP64, device CG).
+int launch_gauss_residual(const KParams &kp, const double *xo, const double *vo, double *b,
+                          hipStream_t s);
+int launch_neg_lap(const KParams &kp, const double *p, double *ap, hipStream_t s);
+int launch_add_grad(const KParams &kp, const double *chi, double *E, hipStream_t s);
+int launch_dot_partials(const double *a, const double *b, uint32_t n, double *partials,
+                        int nblocks, hipStream_t s);
+int launch_axpy_cg(double alpha, const double *p, const double *ap, double *x, double *r,
+                   uint32_t n, double *partials, int nblocks, hipStream_t s);
+int launch_cg_update(double xbar, double beta, const double *r, double *x, double *p,
+                     uint32_t n, hipStream_t s);
+int launch_sub_mean_max(double *b, double mean, uint32_t n, double *partials, int nblocks,
+                        hipStream_t s);
+int launch_scale(double *x, double f, uint32_t n, hipStream_t s);
+
+constexpr int DIAG_BLOCK = 256;
+
+}  // namespace scp

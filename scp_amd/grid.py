@@ -1,0 +1,210 @@
+"""Host-side mirror of the reference step interface over libscp_amd.so.
+
+The reference drives its step through `verlet_step(Grid*, const Config*)`
+(sfa/sim/scp_sim.c:1583) and the GPU path through gpu_alloc / gpu_upload /
+gpu_verlet_step / run_gpu_diagnostics (sfa/sim/scp_sim.cu:2659-4041). `Grid`
+offers the same life cycle over the C-ABI of include/scp_amd.h:
+
+    g = Grid(params)                # grid_alloc + gpu_alloc
+    g.upload(state)                 # gpu_upload + initial compute_forces_*
+    g.step(n)                       # n x verlet_step (boundary stage included)
+    state = g.download()            # snapshot columns (sfa_snap)
+    d = g.diag()                    # one diag.tsv row (compute_energy*, charges, gauss)
+
+Errors raise ScpError with the library's message (the reference exits the
+process; a library must not). There is no CPU fallback: without the HIP
+library or a GPU, construction raises.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native
+
+MODE_REAL, MODE_COMPLEX, MODE_GAUGED = 0, 1, 2
+PREC_F64, PREC_F32C = 0, 1
+
+DIAG_NAMES = ["E_phi_kin", "E_theta_kin", "E_grad", "E_mass", "E_pot", "E_tgrad", "E_tmass",
+              "E_coupling", "E_total", "phi_max", "P_max", "P_int", "theta_rms",
+              "Q_phi", "Q_theta", "Q_total", "s_max", "r_core", "omega_core", "Q_core",
+              "thp1u", "thp1v", "thp2u", "thp2v", "gauss_max", "gauss_l2", "E_em", "Q_flux",
+              "Q_p0", "Q_p1", "Q_p2"]
+
+# every entry point declared in include/scp_amd.h
+ABI_SYMBOLS = ["scp_abi_version", "scp_last_error", "scp_grid_bytes", "scp_grid_create",
+               "scp_grid_destroy", "scp_grid_upload", "scp_grid_download", "scp_grid_step",
+               "scp_grid_diag", "scp_grid_forces", "scp_grid_gauss_project",
+               "scp_grid_set_g_offset", "scp_grid_g_offset", "scp_grid_rescale_phi",
+               "scp_grid_steps_done", "scp_grid_time_steps", "scp_grid_step_kernel_name"]
+
+
+class ScpError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    """scp_params of include/scp_amd.h."""
+    _fields_ = [("N", C.c_int32), ("mode", C.c_int32), ("precision", C.c_int32),
+                ("bc_type", C.c_int32), ("rmode", C.c_int32), ("reserved", C.c_int32),
+                ("L", C.c_double), ("dx", C.c_double), ("dt", C.c_double),
+                ("m2", C.c_double), ("mtheta2", C.c_double), ("eta", C.c_double),
+                ("mu", C.c_double), ("kappa", C.c_double), ("g", C.c_double),
+                ("kappa_h", C.c_double), ("alpha_cs", C.c_double), ("beta_h", C.c_double),
+                ("inv_alpha", C.c_double), ("inv_beta", C.c_double), ("kappa_gamma", C.c_double),
+                ("damp_width", C.c_double), ("damp_rate", C.c_double), ("G_offset", C.c_double),
+                ("qdiag_radius", C.c_double), ("qdiag_probe1", C.c_double),
+                ("qdiag_probe2", C.c_double)]
+
+
+def make_params(N, mode, L, dt_factor, precision=PREC_F64, **kw):
+    """Parameters with the reference defaults (scp_config.h:165-215); dx and dt
+    are derived exactly as grid_alloc does (scp_sim.c:204-205)."""
+    dx = 2.0 * L / (N - 1)
+    p = Params(N=N, mode=mode, precision=precision, bc_type=kw.pop("bc_type", 2),
+               rmode=kw.pop("rmode", 0), L=L, dx=dx, dt=dt_factor * dx, m2=2.25, mtheta2=0.0,
+               eta=0.5, mu=-41.345, kappa=50.0, g=0.05, kappa_h=0.0, alpha_cs=0.0, beta_h=0.0,
+               inv_alpha=2.25, inv_beta=5.0, kappa_gamma=2.0, damp_width=3.0, damp_rate=0.01,
+               G_offset=0.0, qdiag_radius=8.0, qdiag_probe1=8.0, qdiag_probe2=16.0)
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def ncols(mode):
+    return {0: 12, 1: 24, 2: 30}[mode]
+
+
+def nacc(mode):
+    return {0: 6, 1: 12, 2: 15}[mode]
+
+
+_typed = False
+
+
+def lib():
+    global _typed
+    L = _native.load("libscp_amd.so")
+    if not _typed:
+        P = C.POINTER(Params)
+        VP = C.c_void_p
+        DPP = C.POINTER(C.c_void_p)
+        L.scp_last_error.argtypes = [C.c_char_p, C.c_int]
+        L.scp_grid_bytes.restype = C.c_int64
+        L.scp_grid_bytes.argtypes = [P]
+        L.scp_grid_create.argtypes = [P, C.POINTER(VP)]
+        L.scp_grid_destroy.argtypes = [VP]
+        L.scp_grid_upload.argtypes = [VP, DPP, C.c_int]
+        L.scp_grid_download.argtypes = [VP, DPP, C.c_int]
+        L.scp_grid_step.argtypes = [VP, C.c_int]
+        L.scp_grid_diag.argtypes = [VP, C.POINTER(C.c_double), C.c_int]
+        L.scp_grid_forces.argtypes = [VP, DPP, C.c_int]
+        L.scp_grid_gauss_project.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.scp_grid_set_g_offset.argtypes = [VP, C.c_double]
+        L.scp_grid_g_offset.restype = C.c_double
+        L.scp_grid_g_offset.argtypes = [VP]
+        L.scp_grid_rescale_phi.argtypes = [VP, C.c_double]
+        L.scp_grid_steps_done.restype = C.c_int64
+        L.scp_grid_steps_done.argtypes = [VP]
+        L.scp_grid_time_steps.argtypes = [VP, C.c_int, C.POINTER(C.c_float)]
+        L.scp_grid_step_kernel_name.restype = C.c_char_p
+        L.scp_grid_step_kernel_name.argtypes = [VP]
+        _typed = True
+    return L
+
+
+def last_error():
+    buf = C.create_string_buffer(512)
+    lib().scp_last_error(buf, 512)
+    return buf.value.decode()
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise ScpError(f"{what}: rc={rc}: {last_error()}")
+
+
+def grid_bytes(params):
+    return int(lib().scp_grid_bytes(C.byref(params)))
+
+
+def _col_ptrs(arr2d):
+    return (C.c_void_p * arr2d.shape[0])(*[arr2d[c].ctypes.data for c in range(arr2d.shape[0])])
+
+
+class Grid:
+    def __init__(self, params):
+        self.params = params
+        self.mode = params.mode
+        self.n3 = params.N ** 3
+        self._h = C.c_void_p()
+        _check(lib().scp_grid_create(C.byref(params), C.byref(self._h)), "scp_grid_create")
+
+    def close(self):
+        if self._h:
+            lib().scp_grid_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, state):
+        st = np.ascontiguousarray(np.asarray(state, dtype=np.float64).reshape(ncols(self.mode), self.n3))
+        _check(lib().scp_grid_upload(self._h, _col_ptrs(st), st.shape[0]), "scp_grid_upload")
+
+    def download(self):
+        st = np.empty((ncols(self.mode), self.n3))
+        _check(lib().scp_grid_download(self._h, _col_ptrs(st), st.shape[0]), "scp_grid_download")
+        return st
+
+    def step(self, n=1):
+        _check(lib().scp_grid_step(self._h, int(n)), "scp_grid_step")
+
+    def time_steps(self, n):
+        ms = C.c_float()
+        _check(lib().scp_grid_time_steps(self._h, int(n), C.byref(ms)), "scp_grid_time_steps")
+        return ms.value
+
+    def diag(self):
+        out = (C.c_double * len(DIAG_NAMES))()
+        _check(lib().scp_grid_diag(self._h, out, len(DIAG_NAMES)), "scp_grid_diag")
+        return dict(zip(DIAG_NAMES, list(out)))
+
+    def forces(self):
+        acc = np.empty((nacc(self.mode), self.n3))
+        _check(lib().scp_grid_forces(self._h, _col_ptrs(acc), acc.shape[0]), "scp_grid_forces")
+        return acc
+
+    def gauss_project(self):
+        it, gm = C.c_int(), C.c_double()
+        _check(lib().scp_grid_gauss_project(self._h, C.byref(it), C.byref(gm)), "scp_grid_gauss_project")
+        return it.value, gm.value
+
+    @property
+    def g_offset(self):
+        return lib().scp_grid_g_offset(self._h)
+
+    @g_offset.setter
+    def g_offset(self, v):
+        _check(lib().scp_grid_set_g_offset(self._h, float(v)), "scp_grid_set_g_offset")
+
+    def rescale_phi(self, f):
+        _check(lib().scp_grid_rescale_phi(self._h, float(f)), "scp_grid_rescale_phi")
+
+    @property
+    def steps_done(self):
+        return int(lib().scp_grid_steps_done(self._h))
+
+    @property
+    def step_kernel_name(self):
+        return lib().scp_grid_step_kernel_name(self._h).decode()

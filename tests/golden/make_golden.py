@@ -85,7 +85,7 @@ CASES = {
 
 def run_case(name, spec, workdir):
     mode, N, L, keys, balls, noise, T, init = spec
-    rng = np.random.default_rng(abs(hash(name)) % (2 ** 32) if False else sum(map(ord, name)))
+    rng = np.random.default_rng(sum(map(ord, name)))
     dtf = 0.1
     dx = 2 * L / (N - 1)
     cfg = {"N": N, "L": L, "T": T, "dt_factor": dtf, "precision": "f64", "bc_type": 2,
@@ -94,8 +94,10 @@ def run_case(name, spec, workdir):
     if mode >= 1:
         cfg["complex_phi"] = 1
     cfg.update(keys)
+    seed = None
     if init is None:
         st = seed_state(N, L, mode, rng, balls, **noise)
+        seed = st
         seed_path = os.path.join(workdir, name + "_seed.sfa")
         sfa.write_sim_sfa(seed_path, N, L, dtf * dx, [(0.0, st)], codec=sfa.CODEC_RAW)
         cfg["init"] = "sfa"
@@ -119,9 +121,15 @@ def run_case(name, spec, workdir):
     with open(cfg["diag_file"]) as fh:
         diag = fh.read()
     params = {k: v for k, v in cfg.items() if k not in ("output", "diag_file", "init_sfa")}
+    extra = {"seed": seed} if (seed is not None and mode == 2) else {}
     np.savez_compressed(os.path.join(OUT, name + ".npz"), params=json.dumps(params),
                         frame0=frame0, final=final, times=times, diag=diag,
-                        stdout=res.stdout, kvmd=json.dumps(kv))
+                        stdout=res.stdout, kvmd=json.dumps(kv), **extra)
+    if name == "complex_n8":
+        # a small archive written by the reference itself (COLZSTD f64), kept as a
+        # byte-level fixture for the reader and the header layout
+        with open(cfg["output"], "rb") as fi, open(os.path.join(OUT, "ref_complex_n8.sfa"), "wb") as fo:
+            fo.write(fi.read())
     print(f"{name}: frames={nf} t={times.tolist()} bytes={os.path.getsize(os.path.join(OUT, name + '.npz'))}")
 
 

@@ -1,0 +1,204 @@
+"""HIP step vs the reference (golden fixtures) and vs the oracle, through the C-ABI.
+
+Bar (FP64 path): bitwise for the real and complex modes (no transcendentals,
+same expression trees, no FMA contraction); for the gauged mode the only
+difference is device sin/cos vs glibc (ulp level), so the bar there is
+<= 1e-12 relative field L2 after N steps.
+"""
+import numpy as np
+import pytest
+
+from oracle import core as oracle
+from scp_amd import grid as G
+from tests import fixtures
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = fixtures.names()
+
+
+def gpu_params(fx, **over):
+    kw = dict(fx["kw"])
+    kw.update(over)
+    return G.make_params(fx["N"], fx["mode"], fx["L"], fx["dt_factor"], **kw)
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_step_matches_reference_fixture(name):
+    fx = fixtures.load(name)
+    n, _ = fixtures.nsteps(fx)
+    print(f"case {name} N={fx['N']} mode={fx['mode']} steps={n}", flush=True)
+    with G.Grid(gpu_params(fx)) as g:
+        g.upload(fx["frame0"])
+        g.step(n)
+        out = g.download()
+    if fx["mode"] == 2:
+        assert rel_l2(out, fx["final"]) <= 1e-12
+    else:
+        assert np.array_equal(out, fx["final"]), f"max diff {np.abs(out - fx['final']).max():.3e}"
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_forces_match_oracle(name):
+    fx = fixtures.load(name)
+    n, p = fixtures.nsteps(fx)
+    with G.Grid(gpu_params(fx)) as g:
+        g.upload(fx["frame0"])
+        acc = g.forces()
+    ref = oracle.forces(p, fx["frame0"])
+    if fx["mode"] == 2:
+        assert np.abs(acc - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+    else:
+        assert np.array_equal(acc, ref)
+
+
+def test_split_steps_equal_one_call():
+    """Emitting on the last launch of a call must not perturb the state."""
+    fx = fixtures.load("gauged_n8")
+    with G.Grid(gpu_params(fx)) as a, G.Grid(gpu_params(fx)) as b:
+        a.upload(fx["frame0"])
+        b.upload(fx["frame0"])
+        a.step(7)
+        a.step(13)
+        b.step(20)
+        assert np.array_equal(a.download(), b.download())
+        assert a.steps_done == 20
+
+
+def seeded(N, mode, L, seed=7, link_sigma=0.0):
+    from scp_amd import seeds
+    dx = 2 * L / (N - 1)
+    balls = [dict(x0=-0.2 * L, y0=0.1 * L, z0=0.0, amp=0.8, sigma=3 * dx, omega=0.8, phase=0.3, winding=1)]
+    if mode == 2:
+        balls.append(dict(x0=0.25 * L, y0=-0.1 * L, z0=0.05 * L, amp=0.8, sigma=3 * dx, omega=-0.8,
+                          phase=1.1, winding=-1))
+    return seeds.seed_balls(N, L, mode, balls, theta_sigma=0.01, link_sigma=link_sigma, seed=seed)
+
+
+@pytest.mark.parametrize("mode,N,steps", [(1, 24, 40), (2, 24, 40), (0, 20, 40), (1, 37, 12), (2, 33, 12)])
+def test_step_matches_oracle_larger(mode, N, steps):
+    """Larger and odd-sized (ragged tile) grids vs the oracle restatement."""
+    L = 0.25 * (N - 1)
+    st = seeded(N, max(mode, 1), L, link_sigma=0.05)
+    if mode == 0:
+        st = np.concatenate([st[:12]])
+    print(f"case mode={mode} N={N} steps={steps}", flush=True)
+    kw = dict(g=0.1) if mode == 2 else {}
+    p_o = oracle.make_params(N, mode, L, 0.1, **kw)
+    ref, _ = oracle.step(p_o, st, steps)
+    with G.Grid(G.make_params(N, mode, L, 0.1, **kw)) as g:
+        g.upload(st)
+        g.step(steps)
+        out = g.download()
+    if mode == 2:
+        assert rel_l2(out, ref) <= 1e-12
+    else:
+        assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("name", ["complex_n8", "complex_damp_n10", "gauged_n8", "gauged_damp_n10",
+                                  "real_n8", "real_terms_n8", "real_mode1_n8"])
+def test_diag_matches_oracle(name):
+    fx = fixtures.load(name)
+    n, p = fixtures.nsteps(fx)
+    with G.Grid(gpu_params(fx)) as g:
+        g.upload(fx["frame0"])
+        g.step(5)
+        d = g.diag()
+        st = g.download()
+    ref = oracle.diag(p, st)
+    for k in G.DIAG_NAMES:
+        if k in ("gauss_max", "gauss_l2"):      # rounding-noise level quantities (~1e-14)
+            assert abs(d[k] - ref[k]) <= 1e-12, k
+            continue
+        tol = 1e-12 * max(1.0, abs(ref[k]))
+        assert abs(d[k] - ref[k]) <= tol, f"{k}: gpu {d[k]!r} oracle {ref[k]!r}"
+
+
+@pytest.mark.parametrize("name", ["complex_n8", "gauged_n8", "real_n8", "gauged_damp_n10"])
+def test_diag_rows_match_reference_tsv(name):
+    """Every diag.tsv row of the reference run, to its printed precision."""
+    fx = fixtures.load(name)
+    n, p = fixtures.nsteps(fx)
+    head, rows = fixtures.parse_diag(fx["diag"])
+    every = int(round(fx["diag_dt"] / p.dt))
+    with G.Grid(gpu_params(fx)) as g:
+        if fx["mode"] == 2:
+            g.g_offset = _g_offset(p, fx["frame0"])
+        g.upload(fx["frame0"])
+        for r, row in enumerate(rows):
+            if r > 0:
+                g.step(every)
+            d = g.diag()
+            d["Q_total"] = d["Q_phi"] + d["Q_theta"]
+            for col, val in zip(head[1:], row[1:]):
+                if col in ("gauss_max", "gauss_l2"):
+                    assert d[col] <= 1e-12 and val <= 1e-12
+                    continue
+                tol = 5e-7 * abs(val) + 1e-14
+                if col in ("Q_phi", "Q_theta", "Q_total", "omega_core", "Q_core", "E_em", "Q_flux",
+                           "Q_p0", "Q_p1", "Q_p2"):
+                    tol = 5e-12 * abs(val) + 1e-12
+                assert abs(d[col] - val) <= tol, f"row {r} {col}: gpu {d[col]!r} ref {val!r}"
+
+
+def _g_offset(p, st):
+    """mean Gauss residual of a state (the frozen offset the reference measures after projection)"""
+    N = p.N
+    s = st.reshape(30, N, N, N)
+    E = s[27:30]
+    divE = sum(E[d] - np.roll(E[d], 1, axis=d) for d in range(3)) / p.dx
+    rho = 0
+    for a in range(3):
+        rho = rho + s[a] * s[18 + a] - s[12 + a] * s[6 + a] + s[3 + a] * s[21 + a] - s[15 + a] * s[9 + a]
+    return float(np.mean(divE - p.g * rho))
+
+
+def test_gauss_projection_matches_oracle():
+    N, L = 16, 7.5
+    st = seeded(N, 2, L, link_sigma=0.02)
+    st[27:30] = 0.0
+    p_o = oracle.make_params(N, 2, L, 0.1, g=0.2)
+    ref, it_ref = oracle.gauss_project(p_o, st)
+    with G.Grid(G.make_params(N, 2, L, 0.1, g=0.2)) as g:
+        g.upload(st)
+        it, gmax = g.gauss_project()
+        out = g.download()
+        off = g.g_offset
+    assert it_ref > 0 and abs(it - it_ref) <= 3
+    assert gmax <= 1e-12
+    assert np.abs(out[27:30] - ref[27:30]).max() <= 1e-11 * max(1.0, np.abs(ref[27:30]).max())
+    assert abs(off - p_o.G_offset) <= 1e-12
+
+
+def test_gauge_transform_invariance():
+    """do_test_gauge_xform (scp_sim.c:2461-2556) on the GPU forces."""
+    N, L = 12, 5.5
+    st = seeded(N, 2, L, link_sigma=0.3)
+    p_o = oracle.make_params(N, 2, L, 0.1, g=0.3)
+    st, _ = oracle.gauss_project(p_o, st)
+    params = G.make_params(N, 2, L, 0.1, g=0.3)
+    with G.Grid(params) as g:
+        g.upload(st)
+        a0 = g.forces()
+        g.upload(oracle.gauge_transform(p_o, st))
+        a1 = g.forces()
+
+    def bil(a):
+        out = [a[q] ** 2 + a[6 + q] ** 2 for q in range(3)] + [a[3 + q] ** 2 + a[9 + q] ** 2 for q in range(3)]
+        return out + [a[12 + q] for q in range(3)]
+    for x, y in zip(bil(a0), bil(a1)):
+        assert np.abs(x - y).max() / max(np.abs(x).max(), 1.0) <= 1e-12
+
+
+def test_no_cpu_fallback_symbols():
+    """The product library is the HIP one, and it exports every ABI entry point."""
+    L = G.lib()
+    for s in G.ABI_SYMBOLS:
+        assert hasattr(L, s)
+    assert L.scp_abi_version() == 1

@@ -1,0 +1,292 @@
+"""Pins of the oracle (CPU FP64 restatement) before it is trusted as the checker.
+
+Each test is chosen so a plausible restatement mistake (dropped term, wrong
+sign or index, transposed operand) fails: the reference binary's own outputs
+(golden fixtures, and a live run where the reference builds here), the
+reference's documented invariants (Noether charge, real-limit identity, gauge
+invariance, Gauss law), and closed forms (lattice plane-wave dispersion,
+uniform-field ODE, staple telescoping).
+"""
+import math
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import core as oracle
+from scp_amd import sfa
+from tests import fixtures
+
+FIXTURES = fixtures.names()
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_oracle_step_bitwise_vs_reference(name):
+    fx = fixtures.load(name)
+    n, p = fixtures.nsteps(fx)
+    st, _ = oracle.step(p, fx["frame0"], n)
+    assert np.array_equal(st, fx["final"])
+
+
+def _g_offset(p, st):
+    N = p.N
+    s = st.reshape(30, N, N, N)
+    divE = sum(s[27 + d] - np.roll(s[27 + d], 1, axis=d) for d in range(3)) / p.dx
+    rho = sum(s[a] * s[18 + a] - s[12 + a] * s[6 + a] + s[3 + a] * s[21 + a] - s[15 + a] * s[9 + a]
+              for a in range(3))
+    return float(np.mean(divE - p.g * rho))
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_oracle_diag_vs_reference_tsv(name):
+    """every diag.tsv row, to the printed precision (%.6e / %.12e)"""
+    fx = fixtures.load(name)
+    n, p = fixtures.nsteps(fx)
+    head, rows = fixtures.parse_diag(fx["diag"])
+    every = int(round(fx["diag_dt"] / p.dt))
+    if fx["mode"] == 2:
+        p.G_offset = _g_offset(p, fx["frame0"])
+    st, acc = fx["frame0"], None
+    for r, row in enumerate(rows):
+        if r > 0:
+            st, acc = oracle.step(p, st, every, acc=acc)
+        d = oracle.diag(p, st)
+        for col, val in zip(head[1:], row[1:]):
+            if col in ("gauss_max", "gauss_l2"):
+                assert d[col] <= 1e-12 and val <= 1e-12
+                continue
+            tol = 5e-7 * abs(val) + 1e-14
+            if col in ("Q_phi", "Q_theta", "Q_total", "omega_core", "Q_core", "E_em", "Q_flux",
+                       "Q_p0", "Q_p1", "Q_p2"):
+                tol = 5e-12 * abs(val) + 1e-12
+            assert abs(d[col] - val) <= tol, f"{name} row {r} {col}: {d[col]!r} vs {val!r}"
+
+
+REF_BIN = oracle.REF_BIN
+
+
+@pytest.mark.skipif(not os.path.exists(REF_BIN), reason="reference binary not built here")
+@pytest.mark.parametrize("mode", [1, 2])
+def test_oracle_vs_live_reference_binary(mode):
+    """A fresh random case (not one of the fixtures) through oracle/_ref."""
+    N, L, steps = 9, 4.0, 15
+    rng = np.random.default_rng(100 + mode)
+    st = np.zeros((oracle.ncols(mode), N ** 3))
+    st[:24] = 0.3 * rng.standard_normal((24, N ** 3))
+    if mode == 2:
+        st[6:12] = 0.0          # zero velocities: net-neutral by construction (passes C:2264)
+        st[18:24] = 0.0
+        st[24:27] = 0.4 * rng.standard_normal((3, N ** 3))
+    dx = 2 * L / (N - 1)
+    with tempfile.TemporaryDirectory() as wd:
+        seed = os.path.join(wd, "seed.sfa")
+        sfa.write_sim_sfa(seed, N, L, 0.1 * dx, [(0.0, st)])
+        cfg = os.path.join(wd, "c.cfg")
+        T = steps * 0.1 * dx
+        with open(cfg, "w") as f:
+            f.write(f"N={N}\nL={L}\nT={T}\ndt_factor=0.1\ncomplex_phi=1\nbc_type=2\nprecision=f64\n"
+                    f"init=sfa\ninit_sfa={seed}\nsnap_dt={T}\ndiag_dt={T}\noutput={wd}/o.sfa\n"
+                    f"diag_file={wd}/d.tsv\neta=0.7\nm_theta=0.3\n")
+            if mode == 2:
+                f.write("complex_gauge=1\ng_gauge=0.2\n")
+        res = subprocess.run([REF_BIN, cfg], capture_output=True, text=True, cwd=wd,
+                             env=dict(os.environ, OMP_NUM_THREADS="2"))
+        assert res.returncode == 0, res.stderr
+        with sfa.SFAFile(os.path.join(wd, "o.sfa")) as fo:
+            frame0, final = fo.read_frame(0), fo.read_frame(fo.total_frames - 1)
+    p = oracle.make_params(N, mode, L, 0.1, eta=0.7, mtheta2=0.3 * 0.3, g=0.2)
+    if mode == 2:
+        proj, _ = oracle.gauss_project(p, st)
+        assert np.abs(proj - frame0).max() <= 1e-12       # E after projection
+    out, _ = oracle.step(p, frame0, int(round(T / p.dt)))
+    assert np.array_equal(out, final)
+
+
+def test_q_conservation_eta0_periodic():
+    """Noether charge is conserved to round-off at eta=0 (v66 qcons: 7e-14 relative)."""
+    N, L = 16, 7.5
+    from scp_amd import seeds
+    st = seeds.seed_balls(N, L, 1, [dict(x0=0, y0=0, z0=0, amp=0.8, sigma=1.8, omega=1.2, phase=0.2,
+                                         winding=1)], theta_sigma=0.01, seed=3)
+    p = oracle.make_params(N, 1, L, 0.1, eta=0.0)
+    q0 = oracle.diag(p, st)["Q_total"]
+    out, _ = oracle.step(p, st, 200)
+    q1 = oracle.diag(p, out)["Q_total"]
+    assert abs(q1 - q0) / abs(q0) <= 1e-12
+    # with eta != 0 the phi and theta charges exchange but their sum is conserved too
+    p.eta = 0.5
+    out, _ = oracle.step(p, st, 200)
+    assert abs(oracle.diag(p, out)["Q_total"] - q0) / abs(q0) <= 1e-12
+
+
+def test_real_limit_identity():
+    """Complex kernel with v = tv = 0 reproduces the real kernel (v66 SPEC §9; ref 2.44e-15)."""
+    N, L = 10, 4.5
+    rng = np.random.default_rng(5)
+    real = 0.5 * rng.standard_normal((12, N ** 3))
+    cplx = np.zeros((24, N ** 3))
+    cplx[:12] = real
+    pr = oracle.make_params(N, 0, L, 0.1)
+    pc = oracle.make_params(N, 1, L, 0.1)
+    a, _ = oracle.step(pr, real, 20)
+    b, _ = oracle.step(pc, cplx, 20)
+    assert np.abs(b[12:]).max() == 0.0
+    assert np.abs(a - b[:12]).max() <= 1e-13 * max(1.0, np.abs(a).max())
+
+
+def test_plane_wave_dispersion_closed_form():
+    """Free lattice field (mu=0, eta=0): u = A cos(k x) cos(n th) with
+    cos th = 1 - Om^2 dt^2/2, Om^2 = m^2 + (4/dx^2) sin^2(k dx/2) — exact for
+    the 7-point Laplacian and velocity Verlet. Fails on a wrong Laplacian
+    index/sign, a wrong mass term, or a wrong kick/drift order."""
+    N, L = 16, 7.5
+    p = oracle.make_params(N, 0, L, 0.2, mu=0.0, eta=0.0, m2=1.3, mtheta2=0.7)
+    dx, dt = p.dx, p.dt
+    x = -L + dx * np.arange(N)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    st = np.zeros((12, N ** 3))
+    modes = [(0, 2, 0, 0.3, 1.3), (1, 3, 1, 0.2, 1.3), (3, 1, 2, 0.25, 0.7), (5, 4, 1, 0.1, 0.7)]
+    for comp, m, axis, A, mass2 in modes:
+        k = 2 * np.pi * m / (N * dx)
+        coord = [X, Y, Z][axis]
+        st[comp] = (A * np.cos(k * coord)).ravel()
+    n = 37
+    out, _ = oracle.step(p, st, n)
+    for comp, m, axis, A, mass2 in modes:
+        k = 2 * np.pi * m / (N * dx)
+        om2 = mass2 + 4.0 / dx ** 2 * math.sin(k * dx / 2) ** 2
+        th = math.acos(1 - om2 * dt * dt / 2)
+        coord = [X, Y, Z][axis]
+        expect = (A * np.cos(k * coord) * math.cos(n * th)).ravel()
+        assert np.abs(out[comp] - expect).max() <= 1e-12
+
+
+def test_uniform_field_reduces_to_site_ode():
+    """k=0 field: every site follows the same 12-dim ODE; a scalar leapfrog of
+    u'' = -m^2 u - 2 Vt'(s) u prod_{b!=a}|Phi_b|^2 (theta decoupled) reproduces it."""
+    N, L = 6, 2.5
+    u0 = np.array([0.7, 0.5, 0.6, 0.1, -0.2, 0.3])        # u, v
+    w0 = np.array([0.1, -0.3, 0.2, 0.4, 0.0, -0.1])
+    st = np.zeros((24, N ** 3))
+    for a in range(3):
+        st[a], st[12 + a] = u0[a], u0[3 + a]
+        st[6 + a], st[18 + a] = w0[a], w0[3 + a]
+    p = oracle.make_params(N, 1, L, 0.1)
+    n = 50
+    out, _ = oracle.step(p, st, n)
+
+    def acc(z):
+        s2 = [z[a] ** 2 + z[3 + a] ** 2 for a in range(3)]
+        s = s2[0] * s2[1] * s2[2]
+        vp = 0.5 * p.mu / (1 + p.kappa * s) ** 2
+        rest = [s2[1] * s2[2], s2[0] * s2[2], s2[0] * s2[1]]
+        return np.array([-p.m2 * z[i] - 2 * vp * z[i] * rest[i % 3] for i in range(6)])
+    z, v = u0.copy(), w0.copy()
+    a = acc(z)
+    for _ in range(n):
+        v = v + 0.5 * p.dt * a
+        z = z + p.dt * v
+        a = acc(z)
+        v = v + 0.5 * p.dt * a
+    for a_ in range(3):
+        assert np.abs(out[a_] - z[a_]).max() <= 1e-12
+        assert np.abs(out[12 + a_] - z[3 + a_]).max() <= 1e-12
+
+
+def _rand_gauged(N, seed, link=0.5, amp=0.4):
+    rng = np.random.default_rng(seed)
+    st = np.zeros((30, N ** 3))
+    st[:24] = amp * rng.standard_normal((24, N ** 3))
+    st[24:27] = link * rng.standard_normal((3, N ** 3))
+    st[27:30] = 0.1 * rng.standard_normal((3, N ** 3))
+    return st
+
+
+def test_gauge_transform_invariance():
+    """do_test_gauge_xform (scp_sim.c:2461-2556): |acc|^2 per complex pair, E_acc, G
+    unchanged to <= 1e-12 relative under a random lattice gauge transformation."""
+    N = 8
+    p = oracle.make_params(N, 2, 3.5, 0.1, g=0.4)
+    st = _rand_gauged(N, 1)
+    a0 = oracle.forces(p, st)
+    a1 = oracle.forces(p, oracle.gauge_transform(p, st))
+    b0 = [a0[q] ** 2 + a0[6 + q] ** 2 for q in range(3)] + [a0[3 + q] ** 2 + a0[9 + q] ** 2 for q in range(3)]
+    b1 = [a1[q] ** 2 + a1[6 + q] ** 2 for q in range(3)] + [a1[3 + q] ** 2 + a1[9 + q] ** 2 for q in range(3)]
+    for x, y in zip(b0 + list(a0[12:15]), b1 + list(a1[12:15])):
+        assert np.abs(x - y).max() / max(np.abs(x).max(), 1.0) <= 1e-12
+    # a non-covariant stencil would fail: rotating matter without moving links changes |acc|
+    st2 = st.copy()
+    c, s = math.cos(0.3), math.sin(0.3)
+    for base_re, base_im in ((0, 12), (3, 15)):
+        for a in range(3):
+            u, v = st2[base_re + a].copy(), st2[base_im + a].copy()
+            idx = np.arange(N ** 3) % 2 == 0       # site-dependent phase
+            st2[base_re + a][idx] = c * u[idx] + s * v[idx]
+            st2[base_im + a][idx] = c * v[idx] - s * u[idx]
+    a2 = oracle.forces(p, st2)
+    assert np.abs(a2[0] ** 2 + a2[6] ** 2 - b0[0]).max() > 1e-6
+
+
+def test_staple_divergence_vanishes():
+    """With matter at zero the E-kick is the staple term alone, whose backward
+    lattice divergence telescopes to zero (v69 SPEC §3.4)."""
+    N = 8
+    p = oracle.make_params(N, 2, 3.5, 0.1, g=0.3)
+    st = np.zeros((30, N ** 3))
+    st[24:27] = np.random.default_rng(4).uniform(-3, 3, (3, N ** 3))
+    a = oracle.forces(p, st)
+    K = a[12:15].reshape(3, N, N, N)
+    div = sum(K[d] - np.roll(K[d], 1, axis=d) for d in range(3))
+    assert np.abs(div).max() <= 1e-12 * np.abs(K).max()
+    assert np.abs(K).max() > 1e-3
+
+
+def test_gauss_law_conserved_by_step():
+    """After projection the residual stays at round-off for all t (v69 FINDINGS: ~1e-13)."""
+    fx = fixtures.load("gauged_g03_n8")
+    n, p = fixtures.nsteps(fx)
+    p.G_offset = _g_offset(p, fx["frame0"])
+    out, _ = oracle.step(p, fx["frame0"], 60)
+    d = oracle.diag(p, out)
+    assert d["gauss_max"] <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["gauged_n8", "gauged_g03_n8", "gauged_damp_n10"])
+def test_gauss_projection_vs_reference_frame0(name):
+    """init_gauss_project (C:2284-2400): the reference's frame 0 is the projected seed."""
+    fx = fixtures.load(name)
+    n, p = fixtures.nsteps(fx)
+    z = np.load(os.path.join(fixtures.GOLDEN, name + ".npz"))
+    proj, it = oracle.gauss_project(p, z["seed"])
+    assert it > 0
+    assert np.abs(proj - fx["frame0"]).max() <= 1e-12 * max(1.0, np.abs(fx["frame0"][27:]).max())
+    import re
+    m = re.search(r"Gauss projection: (\d+) CG iters", fx["stdout"])
+    assert m and abs(int(m.group(1)) - it) <= 2
+    d = oracle.diag(p, proj)
+    assert d["gauss_max"] <= 1e-12
+
+
+def test_tiny_g_reduces_to_complex():
+    """g -> 1e-8 reproduces the ungauged complex evolution within 1e-9 (v69 SPEC §1.3)."""
+    N = 8
+    st = _rand_gauged(N, 9, link=0.0, amp=0.3)
+    st[27:30] = 0.0
+    pg = oracle.make_params(N, 2, 3.5, 0.1, g=1e-8)
+    pc = oracle.make_params(N, 1, 3.5, 0.1)
+    a, _ = oracle.step(pg, st, 20)
+    b, _ = oracle.step(pc, st[:24], 20)
+    assert np.abs(a[:24] - b).max() <= 1e-9
+
+
+def test_g_zero_dispatches_to_complex():
+    """complex_gauge with g == 0 runs the complex step bit for bit (C:1586, 1669-1670)."""
+    N = 8
+    st = _rand_gauged(N, 10)
+    pg = oracle.make_params(N, 2, 3.5, 0.1, g=0.0)
+    pc = oracle.make_params(N, 1, 3.5, 0.1)
+    a, _ = oracle.step(pg, st, 10)
+    b, _ = oracle.step(pc, st[:24], 10)
+    assert np.array_equal(a[:24], b) and np.array_equal(a[24:], st[24:])
