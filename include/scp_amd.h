@@ -51,7 +51,8 @@ typedef struct {
                              (two-sum) position/velocity accumulation */
     int32_t bc_type;      /* 0 absorbing sphere, 2 periodic (1 unsupported: returns error) */
     int32_t rmode;        /* real-mode mass coupling: 0 const, 1 inverse, 3 density-kappa */
-    int32_t reserved;
+    int32_t lo_words;     /* SCP_PREC_F32C compensation-word storage: 0 FP32 (8 B/scalar),
+                             1 bfloat16 (6 B/scalar), 2 none (plain FP32, 4 B/scalar) */
     double L, dx, dt;     /* dx = 2L/(N-1), dt = dt_factor*dx (scp_sim.c:204-205) */
     double m2, mtheta2, eta, mu, kappa, g;
     double kappa_h, alpha_cs, beta_h, inv_alpha, inv_beta, kappa_gamma;

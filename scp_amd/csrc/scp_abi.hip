@@ -27,8 +27,14 @@ struct scp_grid {
     int NP;
     hipStream_t stream;
     int device;
-    double *P[2];
+    int f32;          // SCP_PREC_F32C
+    int lo_kind;      // LoKind of the compensation words
+    double *P[2];     // FP64 state
     double *W;
+    float *P32[2];    // FP32C state: hi words ...
+    float *W32;
+    void *Plo[2];     // ... and compensation words
+    void *Wlo;
     double *xo, *vo;
     double *aux;     // real-mode Cosserat/hardening fields (6), when needed
     double *acc;     // forces output (lazily allocated)
@@ -101,7 +107,8 @@ static int check_params(const scp_params *p) {
     if (p->mode < 0 || p->mode > 2) return set_err(-1, "mode=%d invalid", p->mode);
     if (p->precision != SCP_PREC_F64 && p->precision != SCP_PREC_F32C)
         return set_err(-1, "precision=%d invalid", p->precision);
-    if (p->precision == SCP_PREC_F32C) return set_err(-3, "FP32C step is not built into this library");
+    if (p->precision == SCP_PREC_F32C && (p->lo_words < 0 || p->lo_words > 2))
+        return set_err(-1, "lo_words=%d invalid", p->lo_words);
     if (p->bc_type == 1) return set_err(-3, "bc_type=1 (gradient pinned) is not implemented");
     if (p->bc_type != 0 && p->bc_type != 2) return set_err(-1, "bc_type=%d invalid", p->bc_type);
     if (!(p->dx > 0) || !(p->dt > 0)) return set_err(-1, "dx/dt must be positive");
@@ -114,7 +121,9 @@ extern "C" int64_t scp_grid_bytes(const scp_params *p) {
     int64_t n3 = (int64_t)p->N * p->N * p->N;
     int64_t fs = field_stride(n3);
     int NP = np_of_mode(p->mode);
-    int64_t b = 5LL * NP * fs * 8;                       // P[2], W, xo, vo
+    int64_t b = 2LL * NP * fs * 8;                       // xo, vo
+    if (p->precision == SCP_PREC_F64) b += 3LL * NP * fs * 8;           // P[2], W
+    else b += 3LL * NP * fs * (4 + (p->lo_words == 0 ? 4 : p->lo_words == 1 ? 2 : 0));
     if (p->mode == 0 && (p->alpha_cs != 0 || p->beta_h != 0)) b += 6LL * fs * 8;
     b += 1024LL * 32 * 8 + 4096;
     return b;
@@ -150,6 +159,8 @@ static KParams make_kparams(const scp_params &p, int step_mode) {
 static void free_grid(scp_grid *g) {
     if (!g) return;
     hipFree(g->P[0]); hipFree(g->P[1]); hipFree(g->W); hipFree(g->xo); hipFree(g->vo);
+    hipFree(g->P32[0]); hipFree(g->P32[1]); hipFree(g->W32);
+    hipFree(g->Plo[0]); hipFree(g->Plo[1]); hipFree(g->Wlo);
     hipFree(g->aux); hipFree(g->acc); hipFree(g->partials); hipFree(g->dout); hipFree(g->argmin);
     if (g->ev0) hipEventDestroy(g->ev0);
     if (g->ev1) hipEventDestroy(g->ev1);
@@ -176,9 +187,24 @@ extern "C" int scp_grid_create(const scp_params *p, scp_grid **out) {
     const size_t grp = fsb * g->NP;
     hipError_t e = hipSuccess;
     e = e ? e : hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
-    e = e ? e : hipMalloc(&g->P[0], grp);
-    e = e ? e : hipMalloc(&g->P[1], grp);
-    e = e ? e : hipMalloc(&g->W, grp);
+    g->f32 = p->precision == SCP_PREC_F32C;
+    g->lo_kind = p->lo_words;
+    if (!g->f32) {
+        e = e ? e : hipMalloc(&g->P[0], grp);
+        e = e ? e : hipMalloc(&g->P[1], grp);
+        e = e ? e : hipMalloc(&g->W, grp);
+    } else {
+        const size_t g32 = (size_t)g->kp.fs * 4 * g->NP;
+        const size_t glo = (size_t)g->kp.fs * g->NP * (g->lo_kind == LO_F32 ? 4 : 2);
+        e = e ? e : hipMalloc(&g->P32[0], g32);
+        e = e ? e : hipMalloc(&g->P32[1], g32);
+        e = e ? e : hipMalloc(&g->W32, g32);
+        if (g->lo_kind != LO_NONE) {
+            e = e ? e : hipMalloc(&g->Plo[0], glo);
+            e = e ? e : hipMalloc(&g->Plo[1], glo);
+            e = e ? e : hipMalloc(&g->Wlo, glo);
+        }
+    }
     e = e ? e : hipMalloc(&g->xo, grp);
     e = e ? e : hipMalloc(&g->vo, grp);
     if (p->mode == 0 && (p->alpha_cs != 0 || p->beta_h != 0)) {
@@ -196,7 +222,10 @@ extern "C" int scp_grid_create(const scp_params *p, scp_grid **out) {
         free_grid(g);
         return -2;
     }
-    snprintf(g->kname, sizeof g->kname, "k_step64<%d, 1, false>", g->step_mode);
+    if (g->f32)
+        snprintf(g->kname, sizeof g->kname, "k_step32<%d, 1, false, %d>", g->step_mode, g->lo_kind);
+    else
+        snprintf(g->kname, sizeof g->kname, "k_step64<%d, 1, false>", g->step_mode);
     *out = g;
     return 0;
 }
@@ -227,11 +256,29 @@ static int run_aux(scp_grid *g, const double *src) {
     return kerr(launch_real_aux(g->kp, src, g->aux, g->stream), "real aux");
 }
 
+static Bufs32 bufs32_for(scp_grid *g, int kind) {
+    Bufs32 b;
+    memset(&b, 0, sizeof b);
+    b.xo = g->xo; b.vo = g->vo; b.w = g->W32; b.wlo = g->Wlo; b.aux = g->aux; b.acc = g->acc;
+    if (kind == K_INIT) { b.pout = g->P32[0]; b.poutlo = g->Plo[0]; }
+    else if (kind == K_STEP) {
+        b.pin = g->P32[g->cur]; b.pinlo = g->Plo[g->cur];
+        b.pout = g->P32[g->cur ^ 1]; b.poutlo = g->Plo[g->cur ^ 1];
+    }
+    return b;
+}
+
+static int launch_kind(scp_grid *g, int kind, bool emit) {
+    if (g->f32)
+        return kerr(launch_step_f32c(g->step_mode, kind, emit, g->lo_kind, g->kp, bufs32_for(g, kind), g->stream), "step32");
+    return kerr(launch_step_f64(g->step_mode, kind, emit, g->kp, bufs_for(g, kind), g->stream), "step64");
+}
+
 // initial force + first half kick + first drift, from xo/vo
 static int run_init(scp_grid *g) {
     int rc = run_aux(g, g->xo);
     if (rc) return rc;
-    rc = kerr(launch_step_f64(g->step_mode, K_INIT, false, g->kp, bufs_for(g, K_INIT), g->stream), "init");
+    rc = launch_kind(g, K_INIT, false);
     if (rc) return rc;
     g->cur = 0;
     return 0;
@@ -270,10 +317,14 @@ extern "C" int scp_grid_download(scp_grid *g, double *const *cols, int ncols) {
 
 static int enqueue_steps(scp_grid *g, int nsteps) {
     for (int s = 0; s < nsteps; s++) {
-        int rc = run_aux(g, g->P[g->cur]);
-        if (rc) return rc;
+        int rc = 0;
+        if (g->aux) {
+            rc = g->f32 ? kerr(launch_real_aux_f32(g->kp, g->P32[g->cur], g->aux, g->stream), "real aux")
+                        : run_aux(g, g->P[g->cur]);
+            if (rc) return rc;
+        }
         const bool emit = (s == nsteps - 1);
-        rc = kerr(launch_step_f64(g->step_mode, K_STEP, emit, g->kp, bufs_for(g, K_STEP), g->stream), "step");
+        rc = launch_kind(g, K_STEP, emit);
         if (rc) return rc;
         g->cur ^= 1;
     }
@@ -425,7 +476,7 @@ extern "C" int scp_grid_forces(scp_grid *g, double *const *acc, int nacc) {
     HIPCHK(hipMemsetAsync(g->acc, 0, (size_t)g->NP * g->kp.fs * sizeof(double), g->stream));
     int rc = run_aux(g, g->xo);
     if (rc) return rc;
-    rc = kerr(launch_step_f64(g->step_mode, K_FORCES, false, g->kp, bufs_for(g, K_FORCES), g->stream), "forces");
+    rc = launch_kind(g, K_FORCES, false);
     if (rc) return rc;
     const size_t nb = (size_t)g->kp.n3 * sizeof(double);
     for (int f = 0; f < g->NP; f++)

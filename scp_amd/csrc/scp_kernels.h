@@ -49,10 +49,25 @@ struct Bufs {
 
 enum StepKind { K_INIT = 0, K_STEP = 1, K_FORCES = 2 };
 
-// Launchers (scp_step.hip). Return hipError_t as int.
+// FP32 compensated state: hi words (FP32) plus a compensation ("lo") word per
+// scalar whose storage is selected by LoKind.
+enum LoKind { LO_F32 = 0, LO_BF16 = 1, LO_NONE = 2 };
+struct Bufs32 {
+    const float *pin;    // hi words of P at the stencil level
+    float *pout;
+    float *w;
+    const void *pinlo;   // lo words (LoKind storage) of pin / pout / w
+    void *poutlo;
+    void *wlo;
+    double *xo, *vo;     // emitted FP64 state (hi + lo), INIT source
+    double *acc;
+    const double *aux;
+};
+
+// Launchers (scp_step.hip, scp_step32.hip). Return hipError_t as int.
 int launch_step_f64(int mode, int kind, bool emit, const KParams &kp, const Bufs<double> &b,
                     hipStream_t s);
-int launch_step_f32c(int mode, int kind, bool emit, const KParams &kp, const Bufs<float> &b,
+int launch_step_f32c(int mode, int kind, bool emit, int lo_kind, const KParams &kp, const Bufs32 &b,
                      hipStream_t s);
 int launch_real_aux(const KParams &kp, const double *pin, double *aux, hipStream_t s);
 int launch_real_aux_f32(const KParams &kp, const float *pin, double *aux, hipStream_t s);

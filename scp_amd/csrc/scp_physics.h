@@ -1,0 +1,358 @@
+// scp_physics.h — per-site physics of the sfa/sim step, shared by the FP64
+// kernels (R = double, compiled with -ffp-contract=off: every expression tree
+// is the reference's, so results round identically) and the FP32 kernels
+// (R = float). Geometry helpers of the z-march decomposition live here too.
+#pragma once
+
+#include "scp_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+namespace scp {
+
+#define SCP_PI 3.14159265358979323846
+
+__device__ inline void scp_sincos(double x, double *s, double *c) { sincos(x, s, c); }
+__device__ inline void scp_sincos(float x, float *s, float *c) { sincosf(x, s, c); }
+__device__ inline double scp_sin(double x) { return sin(x); }
+__device__ inline float scp_sin(float x) { return sinf(x); }
+
+constexpr int TK = 64, TJ = 4;
+
+struct Geo {
+    int N, i0, i1, j, k;
+    bool valid;
+};
+
+// XCD-aware block numbering: hardware block b runs on XCD b % 8; give each XCD
+// a contiguous range of logical tiles so j-adjacent tiles share an L2.
+__device__ inline uint32_t logical_block(uint32_t b, uint32_t nb) {
+    const uint32_t nx = 8;
+    uint32_t per = (nb + nx - 1) / nx;
+    uint32_t xcd = b % nx, pos = b / nx;
+    uint32_t l = xcd * per + pos;
+    // ragged tail: fall back to identity for the few blocks past the balanced part
+    return (nb % nx == 0) ? l : b;
+}
+
+__device__ inline Geo geometry(int N, int DI) {
+    Geo g;
+    g.N = N;
+    const uint32_t ntk = (N + TK - 1) / TK, ntj = (N + TJ - 1) / TJ;
+    const uint32_t nb = gridDim.x;
+    uint32_t lb = logical_block(blockIdx.x, nb);
+    uint32_t tk = lb % ntk; lb /= ntk;
+    uint32_t tj = lb % ntj; lb /= ntj;
+    uint32_t seg = lb;
+    g.k = tk * TK + threadIdx.x;
+    g.j = tj * TJ + threadIdx.y;
+    g.i0 = seg * DI;
+    g.i1 = min(N, g.i0 + DI);
+    g.valid = g.k < N && g.j < N;
+    return g;
+}
+
+__device__ inline uint32_t sidx(int N, int i, int j, int k) {
+    return ((uint32_t)i * (uint32_t)N + (uint32_t)j) * (uint32_t)N + (uint32_t)k;
+}
+__device__ inline int wp(int a, int N) { return a + 1 == N ? 0 : a + 1; }
+__device__ inline int wm(int a, int N) { return a == 0 ? N - 1 : a - 1; }
+
+
+// ---------------------------------------------------------------------------
+// per-site physics (expression trees of scp_sim.c, kept operation for operation)
+// ---------------------------------------------------------------------------
+
+// sin of the plaquette angle th_a(y) + th_b(y+a) - th_a(y+b) - th_b(y)  (C:1100-1102)
+template <typename R>
+__device__ inline R plaq(R ta_y, R tb_ya, R ta_yb, R tb_y) {
+    R ang = ta_y + tb_ya - ta_yb - tb_y;
+    return scp_sin(ang);
+}
+
+
+// Complex-mode accelerations at one site (compute_forces_complex, C:931-994).
+// c/ip/im: own column at planes i, i+1, i-1; X(f, idx) loads neighbour values.
+template <typename R, class LD>
+__device__ inline void forces_complex(const KParams &kp, const R *c, const R *ip,
+                                      const R *im, LD X, uint32_t jp, uint32_t jm,
+                                      uint32_t kp_, uint32_t km, R *acc) {
+    const R h1 = R(kp.h1), h2 = R(kp.h2);
+    R s2[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) s2[a] = c[F_U + a] * c[F_U + a] + c[F_V + a] * c[F_V + a];
+    R s = s2[0] * s2[1] * s2[2];
+    R den = R(1.0) + R(kp.KAPPA) * s;
+    R Vp = R(0.5) * R(kp.MU) / (den * den);
+    R rest[3] = {s2[1] * s2[2], s2[0] * s2[2], s2[0] * s2[1]};
+    // neighbour value of field f in direction d (+1/-1)
+    auto nbv = [&](int f, int d, int sg) -> R {
+        if (d == 0) return sg > 0 ? ip[f] : im[f];
+        if (d == 1) return X(f, sg > 0 ? jp : jm);
+        return X(f, sg > 0 ? kp_ : km);
+    };
+    auto lap = [&](int f) -> R {
+        return (nbv(f, 0, 1) + nbv(f, 0, -1) + nbv(f, 1, 1) + nbv(f, 1, -1) + nbv(f, 2, 1) +
+                nbv(f, 2, -1) - R(6.0) * c[f]) * h2;
+    };
+    // curl of the 3-vector at base f0, component a (C:409-415)
+    auto curl = [&](int f0, int a) -> R {
+        if (a == 0)
+            return (nbv(f0 + 2, 1, 1) - nbv(f0 + 2, 1, -1) - nbv(f0 + 1, 2, 1) + nbv(f0 + 1, 2, -1)) * h1;
+        if (a == 1)
+            return (nbv(f0 + 0, 2, 1) - nbv(f0 + 0, 2, -1) - nbv(f0 + 2, 0, 1) + nbv(f0 + 2, 0, -1)) * h1;
+        return (nbv(f0 + 1, 0, 1) - nbv(f0 + 1, 0, -1) - nbv(f0 + 0, 1, 1) + nbv(f0 + 0, 1, -1)) * h1;
+    };
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        R u = c[F_U + a], v = c[F_V + a];
+        acc[F_U + a] = lap(F_U + a) - R(kp.MASS2) * u - R(2.0) * Vp * u * rest[a] + R(kp.ETA) * curl(F_TU, a);
+        acc[F_V + a] = lap(F_V + a) - R(kp.MASS2) * v - R(2.0) * Vp * v * rest[a] + R(kp.ETA) * curl(F_TV, a);
+        acc[F_TU + a] = lap(F_TU + a) - R(kp.MT2) * c[F_TU + a] + R(kp.ETA) * curl(F_U, a);
+        acc[F_TV + a] = lap(F_TV + a) - R(kp.MT2) * c[F_TV + a] + R(kp.ETA) * curl(F_V, a);
+    }
+}
+
+// Real-mode accelerations (compute_forces, C:743-922). AUX(f, idx) loads the
+// pass-1 fields M[0..2] (f=0..2) and Q[0..2] (f=3..5) when alpha_cs/beta_h != 0.
+template <typename R, class LD, class LA>
+__device__ inline void forces_real(const KParams &kp, const R *c, const R *ip,
+                                   const R *im, LD X, LA AUX, uint32_t ci, uint32_t ipx,
+                                   uint32_t imx, uint32_t jp, uint32_t jm, uint32_t kp_,
+                                   uint32_t km, R *acc) {
+    const R h1 = R(kp.h1), h2 = R(kp.h2);
+    const R MU = R(kp.MU), KAPPA = R(kp.KAPPA), MASS2 = R(kp.MASS2), ETA = R(kp.ETA);
+    const R KH = R(kp.KH), ACS = R(kp.ACS), BH = R(kp.BH), KG = R(kp.KG);
+    const int MODE = kp.rmode;
+    auto nbv = [&](int f, int d, int sg) -> R {
+        if (d == 0) return sg > 0 ? ip[f] : im[f];
+        if (d == 1) return X(f, sg > 0 ? jp : jm);
+        return X(f, sg > 0 ? kp_ : km);
+    };
+    auto nbi = [&](int d, int sg) -> uint32_t {
+        if (d == 0) return sg > 0 ? ipx : imx;
+        if (d == 1) return sg > 0 ? jp : jm;
+        return sg > 0 ? kp_ : km;
+    };
+    auto curlF = [&](int f0, int a) -> R {
+        if (a == 0)
+            return (nbv(f0 + 2, 1, 1) - nbv(f0 + 2, 1, -1) - nbv(f0 + 1, 2, 1) + nbv(f0 + 1, 2, -1)) * h1;
+        if (a == 1)
+            return (nbv(f0 + 0, 2, 1) - nbv(f0 + 0, 2, -1) - nbv(f0 + 2, 0, 1) + nbv(f0 + 2, 0, -1)) * h1;
+        return (nbv(f0 + 1, 0, 1) - nbv(f0 + 1, 0, -1) - nbv(f0 + 0, 1, 1) + nbv(f0 + 0, 1, -1)) * h1;
+    };
+    auto curlA = [&](int f0, int a) -> R {
+        if (a == 0)
+            return (AUX(f0 + 2, nbi(1, 1)) - AUX(f0 + 2, nbi(1, -1)) - AUX(f0 + 1, nbi(2, 1)) + AUX(f0 + 1, nbi(2, -1))) * h1;
+        if (a == 1)
+            return (AUX(f0 + 0, nbi(2, 1)) - AUX(f0 + 0, nbi(2, -1)) - AUX(f0 + 2, nbi(0, 1)) + AUX(f0 + 2, nbi(0, -1))) * h1;
+        return (AUX(f0 + 1, nbi(0, 1)) - AUX(f0 + 1, nbi(0, -1)) - AUX(f0 + 0, nbi(1, 1)) + AUX(f0 + 0, nbi(1, -1))) * h1;
+    };
+    const int TU = 3;   // real mode: tu at fields 3..5
+    R p0 = c[0], p1 = c[1], p2 = c[2];
+    R sig = p0 * p0 + p1 * p1 + p2 * p2;
+    R me2 = (MODE == 1) ? R(kp.IA) / (R(1.0) + R(kp.IB) * sig) : MASS2;
+    R keff = (MODE == 3) ? KAPPA / (R(1.0) + KG * sig) : KAPPA;
+    R P = p0 * p1 * p2, P2 = P * P;
+    R den = R(1.0) + keff * P2;
+    R dVdP = MU * P / (den * den);
+    R t2c = 0;
+    if (MODE == 3 && KG > 0) {
+        R D = R(1.0) + KG * sig + KAPPA * P2;
+        t2c = MU * KG * KAPPA * P2 * P2 / (D * D);
+    }
+    R cph[3] = {curlF(0, 0), curlF(0, 1), curlF(0, 2)};
+    R hel = p0 * cph[0] + p1 * cph[1] + p2 * cph[2];
+    R dP2[3] = {0, 0, 0};
+    if (KH != 0) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            R Pp = nbv(0, d, 1) * nbv(1, d, 1) * nbv(2, d, 1);
+            R Pm = nbv(0, d, -1) * nbv(1, d, -1) * nbv(2, d, -1);
+            dP2[d] = (Pp * Pp - Pm * Pm) * h1;
+        }
+    }
+    R cM[3] = {0, 0, 0}, cQ[3] = {0, 0, 0};
+    if (ACS != 0) { cM[0] = curlA(0, 0); cM[1] = curlA(0, 1); cM[2] = curlA(0, 2); }
+    if (BH != 0) { cQ[0] = curlA(3, 0); cQ[1] = curlA(3, 1); cQ[2] = curlA(3, 2); }
+    R csq = cph[0] * cph[0] + cph[1] * cph[1] + cph[2] * cph[2];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        R lp = (nbv(a, 0, 1) + nbv(a, 0, -1) + nbv(a, 1, 1) + nbv(a, 1, -1) + nbv(a, 2, 1) +
+                     nbv(a, 2, -1) - R(6.0) * c[a]) * h2;
+        R dPda = (a == 0) ? p1 * p2 : (a == 1) ? p0 * p2 : p0 * p1;
+        R ct = curlF(TU, a);
+        R chiral = 0;
+        if (KH != 0) {
+            R t1 = R(2.0) * P * dPda * hel;
+            R t2 = R(2.0) * P2 * cph[a];
+            R t3;
+            if (a == 0) t3 = -(dP2[2] * p1 - dP2[1] * p2);
+            else if (a == 1) t3 = -(dP2[0] * p2 - dP2[2] * p0);
+            else t3 = -(dP2[1] * p0 - dP2[0] * p1);
+            chiral = KH * (t1 + t2 + t3);
+        }
+        R cos_phi = -ACS * cM[a];
+        R hard_phi = -R(2.0) * cQ[a];
+        acc[a] = lp - me2 * c[a] - dVdP * dPda - t2c * c[a] + ETA * ct + chiral + cos_phi + hard_phi;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        R lt = (nbv(TU + a, 0, 1) + nbv(TU + a, 0, -1) + nbv(TU + a, 1, 1) + nbv(TU + a, 1, -1) +
+                     nbv(TU + a, 2, 1) + nbv(TU + a, 2, -1) - R(6.0) * c[TU + a]) * h2;
+        R cp = curlF(0, a);
+        R cos_th = 0;
+        if (ACS != 0) cos_th = R(2.0) * ACS * AUX(a, ci);
+        R hard_th = -BH * csq * c[TU + a];
+        acc[TU + a] = lt - R(kp.MT2) * c[TU + a] + ETA * cp + cos_th + hard_th;
+    }
+}
+
+// Carried per-column gauge quantities from plane i-1.
+template <typename R>
+struct GaugeCarry {
+    R c0, s0;      // cos/sin th_0(x - e0)
+    R p0, p2;           // plaquette sines (0,1) and (2,0) at x - e0
+};
+
+// Gauged accelerations (compute_forces_complex_gauge, C:1053-1263), one
+// gauge, no Higgs. Writes acc[0..11] (matter, device order) and acc[12..14]
+// (E_acc), and the carry for plane i+1.
+template <typename R, class LD>
+__device__ inline void forces_gauge(const KParams &kp, const R *c, const R *ip,
+                                    const R *im, LD X, int N, int i, int j, int k,
+                                    uint32_t jp, uint32_t jm, uint32_t kp_, uint32_t km,
+                                    GaugeCarry<R> &cy, R *acc) {
+    const R h1 = R(kp.h1), h2 = R(kp.h2);
+    auto nbv = [&](int f, int d, int sg) -> R {
+        if (d == 0) return sg > 0 ? ip[f] : im[f];
+        if (d == 1) return X(f, sg > 0 ? jp : jm);
+        return X(f, sg > 0 ? kp_ : km);
+    };
+    // links and their cos/sin at x and x - e_d (C:1071-1079, 1144-1145)
+    R cP[3], sP[3], cM[3], sM[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) scp_sincos(c[F_TH + d], &sP[d], &cP[d]);
+    cM[0] = cy.c0; sM[0] = cy.s0;
+    scp_sincos(nbv(F_TH + 1, 1, -1), &sM[1], &cM[1]);
+    scp_sincos(nbv(F_TH + 2, 2, -1), &sM[2], &cM[2]);
+
+    // plaquettes (C:1088-1109) at x, x-e1, x-e2; x-e0 carried
+    const int ip1 = wp(i, N), jm1 = wm(j, N), kp1 = wp(k, N), km1 = wm(k, N), jp1 = wp(j, N);
+    R th0 = c[F_TH + 0], th1 = c[F_TH + 1], th2 = c[F_TH + 2];
+    R px0 = plaq(th0, ip[F_TH + 1], nbv(F_TH + 0, 1, 1), th1);       // (0,1) at x
+    R px1 = plaq(th1, nbv(F_TH + 2, 1, 1), nbv(F_TH + 1, 2, 1), th2); // (1,2) at x
+    R px2 = plaq(th2, nbv(F_TH + 0, 2, 1), ip[F_TH + 2], th0);       // (2,0) at x
+    // y = x - e1
+    R t1_ym = nbv(F_TH + 1, 1, -1), t0_ym = nbv(F_TH + 0, 1, -1), t2_ym = nbv(F_TH + 2, 1, -1);
+    R p0_ym = plaq(t0_ym, X(F_TH + 1, sidx(N, ip1, jm1, k)), th0, t1_ym);
+    R p1_ym = plaq(t1_ym, th2, X(F_TH + 1, sidx(N, i, jm1, kp1)), t2_ym);
+    // y = x - e2
+    R t1_zm = nbv(F_TH + 1, 2, -1), t2_zm = nbv(F_TH + 2, 2, -1), t0_zm = nbv(F_TH + 0, 2, -1);
+    R p1_zm = plaq(t1_zm, X(F_TH + 2, sidx(N, i, jp1, km1)), th1, t2_zm);
+    R p2_zm = plaq(t2_zm, th0, X(F_TH + 2, sidx(N, ip1, j, km1)), t0_zm);
+    R st[3];
+    st[0] = (px0 - p0_ym) - (px2 - p2_zm);
+    st[1] = -(px0 - cy.p0) + (px1 - p1_zm);
+    st[2] = (px2 - cy.p2) - (px1 - p1_ym);
+
+    // potential (C:1160-1166)
+    R fu[6], fv[6];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        fu[a] = c[F_U + a]; fv[a] = c[F_V + a];
+        fu[3 + a] = c[F_TU + a]; fv[3 + a] = c[F_TV + a];
+    }
+    R s2[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) s2[a] = fu[a] * fu[a] + fv[a] * fv[a];
+    R s = s2[0] * s2[1] * s2[2];
+    R den = R(1.0) + R(kp.KAPPA) * s;
+    R Vp = R(0.5) * R(kp.MU) / (den * den);
+    R rest[3] = {s2[1] * s2[2], s2[0] * s2[2], s2[0] * s2[1]};
+
+    // transported neighbours, per direction (C:1134-1157), consumed as produced
+    R lU[6], lV[6];          // Laplacian partial sums, per field f (Phi 0..2, Theta 0..2)
+    R DU[3][6], DV[3][6];    // covariant central differences (only d != component used)
+    const int C1[3] = {1, 2, 0}, C2[3] = {2, 0, 1};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        R TRp[6], TIp[6], tup[3], tvp[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {       // h=0: Phi_a (u,v); h=1: Theta_a (tu,tv)
+                const int fuix = h ? F_TU + a : F_U + a, fvix = h ? F_TV + a : F_V + a;
+                R upn = nbv(fuix, d, 1), vpn = nbv(fvix, d, 1);
+                R umn = nbv(fuix, d, -1), vmn = nbv(fvix, d, -1);
+                R trp = cP[d] * upn - sP[d] * vpn, tip = cP[d] * vpn + sP[d] * upn;
+                R trm = cM[d] * umn + sM[d] * vmn, tim = cM[d] * vmn - sM[d] * umn;
+                const int f = h * 3 + a;
+                if (d == 0) { lU[f] = trp + trm; lV[f] = tip + tim; }
+                else { lU[f] = lU[f] + trp; lU[f] = lU[f] + trm; lV[f] = lV[f] + tip; lV[f] = lV[f] + tim; }
+                DU[d][f] = (trp - trm) * h1;
+                DV[d][f] = (tip - tim) * h1;
+                TRp[f] = trp; TIp[f] = tip;
+                if (h) { tup[a] = upn; tvp[a] = vpn; }
+            }
+        }
+        // lattice current J_d and E-kick (C:1232-1250)
+        R Jg = R(0.0);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            Jg += fu[a] * TIp[a] - fv[a] * TRp[a];
+            Jg += fu[3 + a] * TIp[3 + a] - fv[3 + a] * TRp[3 + a];
+        }
+        const int d1 = C1[d], d2 = C2[d];
+        R WpR1 = cP[d] * fu[d2] + sP[d] * fv[d2], WpI1 = cP[d] * fv[d2] - sP[d] * fu[d2];
+        R WpR2 = cP[d] * fu[d1] + sP[d] * fv[d1], WpI2 = cP[d] * fv[d1] - sP[d] * fu[d1];
+        R T12 = (fu[3 + d1] * TIp[d2] - fv[3 + d1] * TRp[d2]) + (tup[d1] * WpI1 - tvp[d1] * WpR1);
+        R T21 = (fu[3 + d2] * TIp[d1] - fv[3 + d2] * TRp[d1]) + (tup[d2] * WpI2 - tvp[d2] * WpR2);
+        R Jlat = R(kp.inva) * Jg - R(0.5) * R(kp.ETA) * (T12 - T21);
+        acc[F_TH + d] = R(kp.STP) * st[d] + R(kp.GG) * Jlat;
+    }
+    // matter accelerations (C:1179-1204)
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int d1 = C1[a], d2 = C2[a];
+        R reDxT = DU[d1][3 + C2[a]] - DU[d2][3 + C1[a]];
+        R imDxT = DV[d1][3 + C2[a]] - DV[d2][3 + C1[a]];
+        R reDxP = DU[d1][C2[a]] - DU[d2][C1[a]];
+        R imDxP = DV[d1][C2[a]] - DV[d2][C1[a]];
+        R lUP = (lU[a] - R(6.0) * fu[a]) * h2;
+        R lVP = (lV[a] - R(6.0) * fv[a]) * h2;
+        R lUT = (lU[3 + a] - R(6.0) * fu[3 + a]) * h2;
+        R lVT = (lV[3 + a] - R(6.0) * fv[3 + a]) * h2;
+        acc[F_U + a] = lUP - R(kp.MASS2) * fu[a] - R(2.0) * Vp * fu[a] * rest[a] + R(kp.ETA) * reDxT;
+        acc[F_V + a] = lVP - R(kp.MASS2) * fv[a] - R(2.0) * Vp * fv[a] * rest[a] + R(kp.ETA) * imDxT;
+        acc[F_TU + a] = lUT - R(kp.MT2) * fu[3 + a] + R(kp.ETA) * reDxP;
+        acc[F_TV + a] = lVT - R(kp.MT2) * fv[3 + a] + R(kp.ETA) * imDxP;
+    }
+    cy.c0 = cP[0]; cy.s0 = sP[0]; cy.p0 = px0; cy.p2 = px2;
+}
+
+// carry for the segment's first plane: quantities at (i-1, j, k)
+template <typename R, class LD>
+__device__ inline void gauge_prologue(const KParams &kp, const R *cm, const R *c0, LD X,
+                                      int N, int im1, int j, int k, GaugeCarry<R> &cy) {
+    scp_sincos(cm[F_TH + 0], &cy.s0, &cy.c0);
+    const int jp1 = wp(j, N), kp1 = wp(k, N);
+    cy.p0 = plaq(cm[F_TH + 0], c0[F_TH + 1], X(F_TH + 0, sidx(N, im1, jp1, k)), cm[F_TH + 1]);
+    cy.p2 = plaq(cm[F_TH + 2], X(F_TH + 0, sidx(N, im1, j, kp1)), c0[F_TH + 2], cm[F_TH + 0]);
+    (void)kp;
+}
+
+// sponge factor of scp_sim.c:1447-1453 (1.0 and false outside the sponge)
+__device__ inline bool sponge(const KParams &kp, int i, int j, int k, double &d) {
+    double x = -kp.L + i * kp.dx, y = -kp.L + j * kp.dx, z = -kp.L + k * kp.dx;
+    double r = sqrt(x * x + y * y + z * z);
+    if (r > kp.Rd) {
+        double s = (r - kp.Rd) / kp.DW;
+        if (s > 1) s = 1;
+        d = 1.0 - kp.DR * s * s;
+        return true;
+    }
+    return false;
+}
+
+}  // namespace scp

@@ -1,0 +1,119 @@
+"""The drop-in binary (scp_amd/bin/scp_sim) on the reference's own configs.
+
+Same cfg keys, init=sfa seed, diag.tsv and SFA snapshot as the reference run
+stored in tests/golden: the snapshots must match the reference frames (bitwise
+for real/complex, <= 1e-12 relative for gauged) and every diag.tsv value must
+match to the printed precision.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from scp_amd import sfa
+from tests import fixtures
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "scp_amd", "bin", "scp_sim")
+
+
+def run_cfg(tmp_path, fx, extra=None, seed_state=None, args=()):
+    cfg = dict(fx["params"])
+    out = tmp_path / "out.sfa"
+    diag = tmp_path / "diag.tsv"
+    cfg["output"] = str(out)
+    cfg["diag_file"] = str(diag)
+    if cfg.get("init") == "sfa":
+        seed = tmp_path / "seed.sfa"
+        N, L = fx["N"], fx["L"]
+        st = seed_state if seed_state is not None else fx["frame0"]
+        sfa.write_sim_sfa(seed, N, L, 0.1 * 2 * L / (N - 1), [(0.0, st)])
+        cfg["init_sfa"] = str(seed)
+    if extra:
+        cfg.update(extra)
+    path = tmp_path / "run.cfg"
+    path.write_text("".join(f"{k} = {v}\n" for k, v in cfg.items()))
+    res = subprocess.run([BIN, str(path), *args], capture_output=True, text=True, cwd=tmp_path, timeout=300)
+    return res, out, diag
+
+
+def compare_diag(text_ref, text_mine, skip=()):
+    h1, r1 = fixtures.parse_diag(text_ref)
+    h2, r2 = fixtures.parse_diag(text_mine)
+    assert h1 == h2
+    assert len(r1) == len(r2)
+    for a, b in zip(r1, r2):
+        for col, x, y in zip(h1, a, b):
+            if col in ("gauss_max", "gauss_l2"):
+                assert x <= 1e-12 and y <= 1e-12
+                continue
+            if col in skip:
+                continue
+            assert abs(x - y) <= 5e-7 * abs(x) + 1e-12, (col, x, y)
+
+
+@pytest.mark.parametrize("name", ["complex_n8", "real_n8", "real_oscillon_n10", "gauged_n8", "gauged_damp_n10",
+                                  "complex_damp_n10", "real_terms_n8"])
+def test_cli_reproduces_reference_run(tmp_path, name):
+    fx = fixtures.load(name)
+    seed = None
+    if fx["mode"] == 2:
+        z = np.load(os.path.join(fixtures.GOLDEN, name + ".npz"))
+        seed = z["seed"]
+    res, out, diag = run_cfg(tmp_path, fx, seed_state=seed)
+    print(res.stdout[-2000:], res.stderr[-2000:], flush=True)
+    assert res.returncode == 0
+    assert "=== COMPLETE ===" in res.stdout and "Wall:" in res.stdout
+    with sfa.SFAFile(out) as f:
+        assert f.total_frames == 2
+        a0, a1 = f.read_frame(0), f.read_frame(1)
+        (ff, pairs), = f.kvmd()
+    assert [tuple(p) for p in pairs] == [tuple(p) for p in fx["kvmd"][0][1]]
+    if fx["mode"] == 2:
+        for mine, ref in ((a0, fx["frame0"]), (a1, fx["final"])):
+            assert np.linalg.norm(mine - ref) / np.linalg.norm(ref) <= 1e-12
+    else:
+        assert np.array_equal(a0, fx["frame0"]) and np.array_equal(a1, fx["final"])
+    compare_diag(fx["diag"], diag.read_text())
+
+
+def test_cli_restart_from_snapshot_is_exact(tmp_path):
+    """scp_sim out.sfa -T ...: KVMD config + last frame; f64 restart continues bit for bit."""
+    fx = fixtures.load("complex_n8")
+    full = tmp_path / "full"
+    half = tmp_path / "half"
+    full.mkdir()
+    half.mkdir()
+    T = fx["T"]
+    r1, out_full, _ = run_cfg(full, fx, extra={"T": 2 * T, "snap_dt": 2 * T})
+    assert r1.returncode == 0
+    r2, out_half, _ = run_cfg(half, fx, extra={"T": T, "snap_dt": T})
+    assert r2.returncode == 0
+    res = subprocess.run([BIN, str(out_half), "-output", str(half / "cont.sfa"), "-diag_file",
+                          str(half / "cont.tsv"), "-snap_dt", str(T), "-T", str(T)],
+                         capture_output=True, text=True, cwd=half, timeout=300)
+    assert res.returncode == 0, res.stderr
+    assert "Loading parameters from SFA" in res.stdout
+    with sfa.SFAFile(out_full) as f, sfa.SFAFile(half / "cont.sfa") as g:
+        assert np.array_equal(f.read_frame(f.total_frames - 1), g.read_frame(g.total_frames - 1))
+
+
+def test_cli_gauge_xform_selftest(tmp_path):
+    fx = fixtures.load("gauged_g03_n8")
+    z = np.load(os.path.join(fixtures.GOLDEN, "gauged_g03_n8.npz"))
+    res, _, _ = run_cfg(tmp_path, fx, extra={"test_gauge_xform": 1}, seed_state=z["seed"])
+    assert "test_gauge_xform: PASS" in res.stdout, res.stdout
+    assert res.returncode == 0
+
+
+def test_cli_refuses_net_charged_periodic_seed(tmp_path):
+    fx = fixtures.load("gauged_n8")
+    z = np.load(os.path.join(fixtures.GOLDEN, "gauged_n8.npz"))
+    seed = z["seed"].copy()
+    seed[18:21] *= 2.0        # unbalance the charge
+    res, _, _ = run_cfg(tmp_path, fx, seed_state=seed)
+    assert res.returncode == 1
+    assert "requires net-neutral seed" in res.stderr

@@ -1,0 +1,80 @@
+"""FP32 compensated step vs the FP64 step (which is itself pinned to the reference).
+
+Stated bars (every FP32C result carries its error against FP64):
+  field L2 (relative)            <= 2e-5   after 200 steps of a gauged 32^3 pair
+  energy, |dQ|/Q (relative)      <= 1e-6   same run
+  Gauss residual                 <= 1e-5   (FP32 stencil: only FP32-exact)
+and the compensation words must not do worse than plain FP32 over a long run.
+"""
+import numpy as np
+import pytest
+
+from scp_amd import grid as G
+from scp_amd import seeds
+
+pytestmark = pytest.mark.gpu
+
+
+def pair_state(N, L, link=0.05, seed=21):
+    dx = 2 * L / (N - 1)
+    balls = [dict(x0=-0.25 * L, y0=0.05 * L, z0=0.0, amp=0.8, sigma=3 * dx, omega=0.8, phase=0.3, winding=1),
+             dict(x0=0.25 * L, y0=-0.05 * L, z0=0.1 * L, amp=0.8, sigma=3 * dx, omega=-0.8, phase=1.3,
+                  winding=-1)]
+    return seeds.seed_balls(N, L, 2, balls, theta_sigma=0.01, link_sigma=link, seed=seed)
+
+
+def run(params, st, steps, project=True):
+    with G.Grid(params) as g:
+        g.upload(st)
+        if project:
+            g.gauss_project()
+        d0 = g.diag()
+        g.step(steps)
+        return g.download(), d0, g.diag()
+
+
+@pytest.mark.parametrize("lo", [0, 1])
+def test_f32c_gauged_error_vs_f64(lo):
+    N, L, steps = 32, 7.75, 200
+    st = pair_state(N, L)
+    p64 = G.make_params(N, G.MODE_GAUGED, L, 0.2, g=0.1)
+    p32 = G.make_params(N, G.MODE_GAUGED, L, 0.2, precision=G.PREC_F32C, lo_words=lo, g=0.1)
+    a, d0, d64 = run(p64, st, steps)
+    b, e0, d32 = run(p32, st, steps)
+    field = np.linalg.norm(b - a) / np.linalg.norm(a)
+    dE = abs(d32["E_total"] - d64["E_total"]) / abs(d64["E_total"])
+    qs = sum(abs(d64[k]) for k in ("Q_p0", "Q_p1", "Q_p2"))
+    dQ = abs(d32["Q_total"] - d64["Q_total"]) / qs
+    print(f"lo={lo}: field L2 {field:.3e} dE/E {dE:.3e} dQ/|Q| {dQ:.3e} gauss {d32['gauss_max']:.3e}")
+    assert field <= 2e-5
+    assert dE <= 1e-6
+    assert dQ <= 1e-6
+    assert d32["gauss_max"] <= 1e-5
+
+
+def test_compensation_reduces_long_run_drift():
+    """Over a long run the compensated words must not do worse than plain FP32
+    on the conserved totals (both measured against FP64)."""
+    N, L, steps = 16, 3.75, 1500
+    st = pair_state(N, L, link=0.02)
+    p64 = G.make_params(N, G.MODE_GAUGED, L, 0.2, g=0.1)
+    a, _, d64 = run(p64, st, steps)
+    err = {}
+    for lo in (0, 2):
+        p = G.make_params(N, G.MODE_GAUGED, L, 0.2, precision=G.PREC_F32C, lo_words=lo, g=0.1)
+        b, _, d = run(p, st, steps)
+        err[lo] = (np.linalg.norm(b - a) / np.linalg.norm(a), abs(d["E_total"] - d64["E_total"]))
+    print(f"compensated {err[0]}, plain {err[2]}")
+    assert err[0][0] <= err[2][0] and err[0][1] <= 1e-6 * abs(d64["E_total"])
+
+
+def test_f32c_complex_and_real_modes_run():
+    for mode, N in ((1, 20), (0, 20)):
+        L = 0.25 * (N - 1)
+        st = pair_state(N, L)[: {1: 24, 0: 12}[mode]]
+        p64 = G.make_params(N, mode, L, 0.2)
+        p32 = G.make_params(N, mode, L, 0.2, precision=G.PREC_F32C)
+        a, _, d64 = run(p64, st, 50, project=False)
+        b, _, d32 = run(p32, st, 50, project=False)
+        assert np.linalg.norm(b - a) / np.linalg.norm(a) <= 2e-5
+        assert abs(d32["E_total"] - d64["E_total"]) <= 1e-6 * abs(d64["E_total"])
