@@ -113,7 +113,8 @@ def test_cli_refuses_net_charged_periodic_seed(tmp_path):
     fx = fixtures.load("gauged_n8")
     z = np.load(os.path.join(fixtures.GOLDEN, "gauged_n8.npz"))
     seed = z["seed"].copy()
-    seed[18:21] *= 2.0        # unbalance the charge
+    half = seed.shape[1] // 2
+    seed[18:21, :half] *= 2.0  # unbalance the charge: only the ball at x < 0 spins faster
     res, _, _ = run_cfg(tmp_path, fx, seed_state=seed)
     assert res.returncode == 1
     assert "requires net-neutral seed" in res.stderr
