@@ -9,9 +9,12 @@ host from a fixed seed and resident in HBM before the timed region.
 A "step" is one full leapfrog step of the whole grid (one fused launch).
 value = site-updates/s over K timed steps (HIP events on the grid's stream).
 roofline: algorithmic HBM bytes per site-update = every persistent state
-scalar read and written once at its storage width (gauged: 30 scalars ->
-480 B in FP64, 240 B in FP32), times N^3 sites per launch, over the average
-launch time; peak = 8 TB/s. cpu_baseline: the reference CPU binary built from
+scalar read and written once at its storage width (gauged: 30 scalars; FP64
+8 B -> 480 B; FP32C 4 B hi + 4/2/0 B compensation word for lo_words 0/1/2 ->
+480/360/240 B), times N^3 sites per launch, over the average launch time;
+peak = 8 TB/s. traffic: measured HBM bytes per launch (rocprofv3 FETCH_SIZE
+and WRITE_SIZE scaled by a plain copy's counters), read from the counter
+summary profiles/hbm_traffic.txt when it holds a line for this kernel and size. cpu_baseline: the reference CPU binary built from
 source (oracle/_ref/scp_sim) on a bounded 64^3 gauged sample on this host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--prec f64|f32c]
@@ -31,7 +34,25 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_SITE = {("gauged", "f64"): 480, ("gauged", "f32c"): 240}
+
+def bytes_per_site(prec, lo_words):
+    """30 gauged state scalars, each read once and written once per step."""
+    width = 8 if prec == "f64" else 4 + {0: 4, 1: 2, 2: 0}[lo_words]
+    return 2 * 30 * width
+
+
+def measured_traffic(kernel, N):
+    """HBM bytes per launch for `kernel` at N^3 from profiles/hbm_traffic.txt
+    (lines 'traffic <kernel without spaces> N=<N> <bytes>'), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            p = line.split()
+            if len(p) == 4 and p[0] == "traffic" and p[1] == kernel.replace(" ", "") and p[2] == f"N={N}":
+                return float(p[3])
+    return None
 
 
 def log(msg):
@@ -71,6 +92,23 @@ def cpu_baseline(seconds_hint=15.0):
                       f"process wall {wall:.1f}s"}
 
 
+def copy_calibration(gib, reps=3):
+    """A plain device-to-device copy of known size: rocprof's FETCH_SIZE/WRITE_SIZE
+    for it give the scale factor between the counters and bytes moved."""
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    dt = (time.time() - t0) / reps
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "reps": reps,
+                      "GBps": 2 * n * 4 / dt / 1e9}), flush=True)
+
+
 def build_state(N, L, seed=1002):
     from scp_amd import seeds
     balls = seeds.draw_balls(N, L, 64, seed)
@@ -85,17 +123,24 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--prec", default="f64", choices=["f64", "f32c"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--lo-words", type=int, default=0, help="FP32C compensation words: 0 f32, 1 bf16, 2 none")
+    ap.add_argument("--copy-calib", type=float, default=0.0,
+                    help="instead of the step: time a plain device copy of this many GiB (HBM counter calibration)")
     a = ap.parse_args()
+    if a.copy_calib > 0:
+        copy_calibration(a.copy_calib)
+        return
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU (slab halo exchange "
-                                   "over RCCL) is a design, see DESIGN.md", "n_gpus": a.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU only: the multi-GPU form (slabs along "
+                                   "the march axis with one-plane halo exchange over RCCL) is not built",
+                          "n_gpus": a.gpus}))
         sys.exit(2)
     from scp_amd import grid as G
 
     N = a.n
     L = 0.25 * (N - 1)            # dx = 0.5
     prec = G.PREC_F64 if a.prec == "f64" else G.PREC_F32C
-    params = G.make_params(N, G.MODE_GAUGED, L, 0.2, precision=prec, g=0.05)
+    params = G.make_params(N, G.MODE_GAUGED, L, 0.2, precision=prec, g=0.05, lo_words=a.lo_words)
     log(f"seeding {N}^3 gauged state on the host")
     t0 = time.time()
     st = build_state(N, L)
@@ -115,13 +160,9 @@ def main():
     ms_step = ms / a.steps
     sites = float(N) ** 3
     value = sites / (ms_step * 1e-3) / 1e9
-    bps = BYTES_PER_SITE[("gauged", a.prec)]
+    bps = bytes_per_site(a.prec, a.lo_words)
     achieved = bps * sites / (ms_step * 1e-3) / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", f"traffic_gauged_{a.prec}_{N}.json")
-    if os.path.exists(tf):
-        with open(tf) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+    traffic = measured_traffic(g.step_kernel_name, N)
     cpu = None if a.no_cpu_baseline else cpu_baseline()
     out = {
         "metric": "Gsite-updates/s & %HBM peak, 512^3 gauged, FP32c vs FP64; p50/p99 job latency",

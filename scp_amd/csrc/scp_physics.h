@@ -13,9 +13,18 @@ namespace scp {
 #define SCP_PI 3.14159265358979323846
 
 __device__ inline void scp_sincos(double x, double *s, double *c) { sincos(x, s, c); }
+#ifdef SCP_F32_HW_TRIG
+// hardware v_sin/v_cos (argument scaled to revolutions): ~1e-6 absolute
+__device__ inline void scp_sincos(float x, float *s, float *c) { *s = __sinf(x); *c = __cosf(x); }
+#else
 __device__ inline void scp_sincos(float x, float *s, float *c) { sincosf(x, s, c); }
+#endif
 __device__ inline double scp_sin(double x) { return sin(x); }
+#ifdef SCP_F32_HW_TRIG
+__device__ inline float scp_sin(float x) { return __sinf(x); }
+#else
 __device__ inline float scp_sin(float x) { return sinf(x); }
+#endif
 
 constexpr int TK = 64, TJ = 4;
 

@@ -6,15 +6,16 @@
 // at all (LO_NONE, plain FP32 for comparison). The stencil — forces, curls,
 // covariant transport, plaquettes — is evaluated in FP32 on the hi words; the
 // leapfrog accumulations v += h a, x += dt v, E and link updates are
-// error-free TwoSum additions in FP32 (Knuth), so increments far below
-// ulp(x) are not lost. FP64 appears only in INIT (from the FP64 host state)
-// and in the emitted snapshot state (hi + lo), which the FP64 diagnostics
-// (energy, Q, Gauss residual) read.
+// compensated FP32 additions (Fast2Sum / Kahan with the product fused), so
+// increments far below ulp(x) accumulate in lo instead of being lost. Between
+// two emitted steps the two half kicks fuse into one W += dt a. FP64 appears
+// only in INIT (from the FP64 host state) and in the emitted snapshot state
+// (hi + lo), which the FP64 diagnostics (energy, Q, Gauss residual) read.
 //
 // Same schedule and z-march decomposition as the FP64 kernel (scp_step.hip).
-// Compiled with FMA contraction allowed; the TwoSum code has no products,
-// and the products that feed it are formed with __fmul_rn plus an exact
-// fmaf residual, so contraction cannot break the error-free transformations.
+// Compiled with FMA contraction allowed: the compensated sums contain no
+// products (the increment is formed by an explicit fmaf), so contraction
+// cannot change them.
 #include "scp_physics.h"
 
 #include <hip/hip_runtime.h>
@@ -25,27 +26,18 @@ constexpr float TWO_PI_HI = 6.28318548202514648f;      // fl32(2 pi)
 constexpr float TWO_PI_LO = -1.7484555314695172e-07f;  // 2 pi - TWO_PI_HI
 constexpr float INV_TWO_PI = 0.159154943091895336f;
 
-__device__ inline void two_sum(float a, float b, float &s, float &e) {
-    s = a + b;
-    float bb = s - a;
-    e = (a - (s - bb)) + (b - bb);
-}
 __device__ inline void fast_two_sum(float a, float b, float &s, float &e) {
     s = a + b;
     e = b - (s - a);
 }
-// (hi, lo) += (d1 + d2), d2 a small correction term
-__device__ inline void df_add(float &hi, float &lo, float d1, float d2) {
-    float s, e;
-    two_sum(hi, d1, s, e);
-    e = e + (lo + d2);
-    fast_two_sum(s, e, hi, lo);
-}
-// (hi, lo) += c * (xh + xl), with the exact residual of the leading product
-__device__ inline void df_axpy(float &hi, float &lo, float c, float xh, float xl) {
-    float p = __fmul_rn(c, xh);
-    float r = fmaf(c, xh, -p) + __fmul_rn(c, xl);
-    df_add(hi, lo, p, r);
+// (hi, lo) += c*x with one FMA and a Fast2Sum (Kahan/Dekker): exact in the
+// sum whenever |hi| >= |c*x + lo|, which is the case that matters — an
+// increment far below ulp(hi) lands in lo instead of being rounded away.
+__device__ inline void kahan_add(float &hi, float &lo, float c, float x) {
+    float y = fmaf(c, x, lo);
+    float t = hi + y;
+    lo = y - (t - hi);
+    hi = t;
 }
 // (hi, lo) *= d
 __device__ inline void df_scale(float &hi, float &lo, float d) {
@@ -155,22 +147,28 @@ k_step32(KParams kp, Bufs32 b, int DI) {
 #pragma unroll
             for (int f = 0; f < NP; f++) {
                 const size_t o = (size_t)f * fs + ci;
-                float vh = __builtin_nontemporal_load(&b.w[o]), vl = LS::get(b.wlo, o);
-                df_axpy(vh, vl, hdt, acc[f], 0.0f);                 // stage 4: v = W + dt/2 a
-                if (damp) df_scale(vh, vl, dmp);                    // stage 5
-                float wh = vh, wl = vl;
-                df_axpy(wh, wl, hdt, acc[f], 0.0f);                 // next stage 1
+                float wh = __builtin_nontemporal_load(&b.w[o]), wl = LS::get(b.wlo, o);
+                float vh = 0.0f, vl = 0.0f;
+                if (EMIT || damp) {
+                    vh = wh; vl = wl;
+                    kahan_add(vh, vl, hdt, acc[f]);                // stage 4: v = W + dt/2 a
+                    if (damp) df_scale(vh, vl, dmp);               // stage 5
+                    wh = vh; wl = vl;
+                    kahan_add(wh, wl, hdt, acc[f]);                // next stage 1
+                } else {
+                    kahan_add(wh, wl, dt, acc[f]);                 // both half kicks at once
+                }
                 float xh = c0[f], xl = LS::get(b.pinlo, o);
                 const float xh0 = xh, xl0 = xl;
                 if (MODE == 2 && f >= F_TH) {                       // link drift + wrap
-                    df_axpy(xh, xl, gad, wh, wl);
+                    kahan_add(xh, xl, gad, wh);
                     float n = rintf(xh * INV_TWO_PI);
                     if (n != 0.0f) {
-                        df_add(xh, xl, -n * TWO_PI_HI, 0.0f);
-                        df_add(xh, xl, -n * TWO_PI_LO, 0.0f);
+                        kahan_add(xh, xl, -n, TWO_PI_HI);
+                        kahan_add(xh, xl, -n, TWO_PI_LO);
                     }
                 } else {
-                    df_axpy(xh, xl, dt, wh, wl);                    // next stage 2
+                    kahan_add(xh, xl, dt, wh);                      // next stage 2
                 }
                 __builtin_nontemporal_store(wh, &b.w[o]);
                 LS::put(b.wlo, o, wl);
