@@ -53,6 +53,9 @@ typedef struct {
     int32_t rmode;        /* real-mode mass coupling: 0 const, 1 inverse, 3 density-kappa */
     int32_t lo_words;     /* SCP_PREC_F32C compensation-word storage: 0 FP32 (8 B/scalar),
                              1 bfloat16 (6 B/scalar), 2 none (plain FP32, 4 B/scalar) */
+    int32_t step_kernel;  /* 0: LDS-staged march (default; complex and gauged steps),
+                             1: direct-neighbour march (every mode; the real mode always uses it) */
+    int32_t reserved;     /* must be 0 */
     double L, dx, dt;     /* dx = 2L/(N-1), dt = dt_factor*dx (scp_sim.c:204-205) */
     double m2, mtheta2, eta, mu, kappa, g;
     double kappa_h, alpha_cs, beta_h, inv_alpha, inv_beta, kappa_gamma;

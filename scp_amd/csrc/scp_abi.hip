@@ -48,6 +48,12 @@ struct scp_grid {
     char kname[96];
 };
 
+// K_STEP of the complex and gauged steps runs the LDS-staged march unless the
+// caller asked for the direct-neighbour kernel
+static bool use_march(const scp_grid *g, int kind) {
+    return kind == K_STEP && g->p.step_kernel == 0 && (g->step_mode == 1 || g->step_mode == 2);
+}
+
 static thread_local char g_err[512];
 
 static int set_err(int code, const char *fmt, ...) {
@@ -222,7 +228,12 @@ extern "C" int scp_grid_create(const scp_params *p, scp_grid **out) {
         free_grid(g);
         return -2;
     }
-    if (g->f32)
+    if (use_march(g, K_STEP)) {
+        if (g->f32)
+            snprintf(g->kname, sizeof g->kname, "k_march<scp::S32<%d>, %d, false>", g->lo_kind, g->step_mode);
+        else
+            snprintf(g->kname, sizeof g->kname, "k_march<scp::S64, %d, false>", g->step_mode);
+    } else if (g->f32)
         snprintf(g->kname, sizeof g->kname, "k_step32<%d, 1, false, %d>", g->step_mode, g->lo_kind);
     else
         snprintf(g->kname, sizeof g->kname, "k_step64<%d, 1, false>", g->step_mode);
@@ -269,6 +280,12 @@ static Bufs32 bufs32_for(scp_grid *g, int kind) {
 }
 
 static int launch_kind(scp_grid *g, int kind, bool emit) {
+    if (use_march(g, kind)) {
+        if (g->f32)
+            return kerr(launch_march_f32c(g->step_mode, emit, g->lo_kind, g->kp, bufs32_for(g, kind), 0, g->stream),
+                        "march32");
+        return kerr(launch_march_f64(g->step_mode, emit, g->kp, bufs_for(g, kind), 0, g->stream), "march64");
+    }
     if (g->f32)
         return kerr(launch_step_f32c(g->step_mode, kind, emit, g->lo_kind, g->kp, bufs32_for(g, kind), g->stream), "step32");
     return kerr(launch_step_f64(g->step_mode, kind, emit, g->kp, bufs_for(g, kind), g->stream), "step64");

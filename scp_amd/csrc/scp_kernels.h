@@ -69,6 +69,11 @@ int launch_step_f64(int mode, int kind, bool emit, const KParams &kp, const Bufs
                     hipStream_t s);
 int launch_step_f32c(int mode, int kind, bool emit, int lo_kind, const KParams &kp, const Bufs32 &b,
                      hipStream_t s);
+// LDS-staged march (scp_march.hip): K_STEP of the complex (1) and gauged (2)
+// modes; DI <= 0 picks the segment depth.
+int launch_march_f64(int mode, bool emit, const KParams &kp, const Bufs<double> &b, int DI, hipStream_t s);
+int launch_march_f32c(int mode, bool emit, int lo_kind, const KParams &kp, const Bufs32 &b, int DI,
+                      hipStream_t s);
 int launch_real_aux(const KParams &kp, const double *pin, double *aux, hipStream_t s);
 int launch_real_aux_f32(const KParams &kp, const float *pin, double *aux, hipStream_t s);
 

@@ -1,0 +1,443 @@
+// scp_march.h — the LDS-staged z-march form of the fused leapfrog step
+// (K_STEP launches of the complex and gauged modes, FP64 and FP32C storage).
+//
+// Same schedule and per-site arithmetic as scp_step.hip / scp_step32.hip (the
+// expression trees of scp_sim.c:931-994, 1053-1263, 1583-1716); what changes is
+// where the neighbour values come from:
+//
+//   * a block is a 64(k) x 4(j) tile that marches along i over DI planes;
+//   * the own column of planes i-1, i, i+1 lives in registers (the march queue);
+//   * plane i of the tile plus a one-cell halo ring (rows j0-1, j0+4, columns
+//     k0-1, k0+64, corners included) is staged in LDS, so the j/k neighbours
+//     and the in-plane link diagonals are LDS reads instead of global loads;
+//   * three LDS plane buffers rotate: while plane i is read, plane i+1 (whose
+//     tile and halo are needed for the two staple diagonals th_1(i+1, j-1, k)
+//     and th_2(i+1, j, k-1)) sits beside it, and the buffer of plane i-2 is
+//     refilled with plane i+1 at the top of the iteration — one barrier per
+//     plane;
+//   * the own column and the halo of plane i+2 are loaded into registers one
+//     iteration ahead, so HBM latency is covered by a whole plane of compute.
+//
+// Global traffic per site and step: the state read and written once plus the
+// halo ring (2*64 + 2*6 cells per 256 sites, read from L2 / MALL: the adjacent
+// tiles stream the same planes at the same time).
+//
+// Ragged tiles (N not a multiple of 64 or 4) keep the scheme: the right halo
+// column sits at column W+1 and the bottom halo row at row H+1, where W x H is
+// the valid part of the tile, so every valid site finds its wrapped neighbour
+// at the same relative LDS offset.
+#pragma once
+
+#include "scp_physics.h"
+
+#include <hip/hip_runtime.h>
+
+namespace scp {
+
+constexpr int MTJ = 4;                 // j rows per block
+constexpr int MR = MTJ + 2;            // staged rows
+constexpr int MC = TK + 2;             // staged columns
+constexpr int MPL = MR * MC;           // staged cells per field
+constexpr int MNT = TK * MTJ;          // threads per block
+constexpr int MHC = 2 * TK + 2 * MR;   // halo cells per plane
+constexpr int MNBUF = 3;
+
+struct MTile {
+    int N, k0, j0, W, H, i0, i1, j, k, own;
+    bool valid;
+};
+
+__device__ inline int wrapc(int x, int N) { return x < 0 ? x + N : (x >= N ? x - N : x); }
+
+__device__ inline MTile mtile(int N, int DI) {
+    MTile t;
+    t.N = N;
+    const uint32_t ntk = (N + TK - 1) / TK, ntj = (N + MTJ - 1) / MTJ;
+    uint32_t lb = logical_block(blockIdx.x, gridDim.x);
+    const uint32_t tk = lb % ntk; lb /= ntk;
+    const uint32_t tj = lb % ntj; lb /= ntj;
+    t.k0 = tk * TK;
+    t.j0 = tj * MTJ;
+    t.W = min(TK, N - t.k0);
+    t.H = min(MTJ, N - t.j0);
+    t.i0 = lb * DI;
+    t.i1 = min(N, t.i0 + DI);
+    t.k = t.k0 + threadIdx.x;
+    t.j = t.j0 + threadIdx.y;
+    t.valid = (int)threadIdx.x < t.W && (int)threadIdx.y < t.H;
+    t.own = (threadIdx.y + 1) * MC + threadIdx.x + 1;
+    return t;
+}
+
+// Byte-offset addressing: every array is fields of fs elements; a load names a
+// wave-uniform field base (scalar registers) and a 32-bit byte offset in the
+// field (one vector register), so the march needs no 64-bit vector address
+// arithmetic. Valid while one field stays below 4 GiB (checked on the host).
+template <typename V>
+__device__ inline const V *fbase(const V *p, int f, uint32_t fs) { return p + (size_t)f * fs; }
+template <typename V>
+__device__ inline V *fbase(V *p, int f, uint32_t fs) { return p + (size_t)f * fs; }
+template <typename V>
+__device__ inline V ldo(const V *base, uint32_t e) {
+    return *(const V *)((const char *)base + e * (uint32_t)sizeof(V));
+}
+template <typename V>
+__device__ inline V ldo_nt(const V *base, uint32_t e) {
+    return __builtin_nontemporal_load((const V *)((const char *)base + e * (uint32_t)sizeof(V)));
+}
+template <typename V>
+__device__ inline void sto_nt(V *base, uint32_t e, V v) {
+    __builtin_nontemporal_store(v, (V *)((char *)base + e * (uint32_t)sizeof(V)));
+}
+
+// The halo ring of one plane, spread over the block.
+//   rows (j0-1 and j0+H): (field, side) pairs p = wave + 4q, lanes along k —
+//     a wave-uniform field and a coalesced 64-element row per load;
+//   columns (k0-1 and k0+W, rows j0-1 .. j0+H): one (field, side, row) cell
+//     per thread for the first NF*2*MR threads.
+template <int NF>
+struct Halo {
+    static constexpr int RQ = (2 * NF + 3) / 4;   // row pairs per wave
+    int wave;
+    uint32_t rtop, rbot;      // in-plane element offsets of this lane's top/bottom halo cell
+    int lrow_top, lrow_bot;   // LDS offsets (field 0) of this lane's halo row cells
+    bool rok;                 // lane inside the tile's valid width
+    int cf;                   // column cell: field, or -1
+    uint32_t cg;              // column cell: in-plane element offset
+    int cl;                   // column cell: LDS offset
+    __device__ void init(const MTile &t) {
+        const int N = t.N, tx = threadIdx.x, tid = threadIdx.y * TK + threadIdx.x;
+        wave = __builtin_amdgcn_readfirstlane(threadIdx.y);
+        rok = tx < t.W;
+        const int gk = rok ? t.k0 + tx : t.k0;
+        rtop = (uint32_t)wrapc(t.j0 - 1, N) * N + gk;
+        rbot = (uint32_t)wrapc(t.j0 + t.H, N) * N + gk;
+        lrow_top = tx + 1;
+        lrow_bot = (t.H + 1) * MC + tx + 1;
+        cf = -1; cg = 0; cl = 0;
+        if (tid < NF * 2 * MR) {
+            const int f = tid / (2 * MR), rem = tid % (2 * MR), side = rem / MR, r = rem % MR;
+            if (r <= t.H + 1) {
+                cf = f;
+                const int gj = wrapc(t.j0 - 1 + r, N);
+                const int ck = side ? wrapc(t.k0 + t.W, N) : wrapc(t.k0 - 1, N);
+                cg = (uint32_t)gj * N + ck;
+                cl = f * MPL + r * MC + (side ? t.W + 1 : 0);
+            }
+        }
+    }
+};
+
+// own column of a staged plane, indexable by field
+template <typename R>
+struct LdsCol {
+    const R *p;
+    __device__ R operator[](int f) const { return p[f * MPL]; }
+};
+
+// neighbour provider over the staged planes (see GlobalNB for the contract)
+template <typename R>
+struct LdsNB {
+    const R *c;   // plane i buffer, at the own cell
+    const R *n;   // plane i+1 buffer, at the own cell
+    __device__ R face(int f, int d, int sg) const { return c[f * MPL + (d == 1 ? sg * MC : sg)]; }
+    __device__ R th1_jm_kp() const { return c[(F_TH + 1) * MPL - MC + 1]; }
+    __device__ R th2_jp_km() const { return c[(F_TH + 2) * MPL + MC - 1]; }
+    __device__ R th1_ip_jm() const { return n[(F_TH + 1) * MPL - MC]; }
+    __device__ R th2_ip_km() const { return n[(F_TH + 2) * MPL - 1]; }
+};
+
+// ---------------------------------------------------------------------------
+// leapfrog updates of one scalar (identical to the direct-neighbour kernels)
+// ---------------------------------------------------------------------------
+
+constexpr float TWO_PI_HI_M = 6.28318548202514648f;      // fl32(2 pi)
+constexpr float TWO_PI_LO_M = -1.7484555314695172e-07f;  // 2 pi - TWO_PI_HI_M
+constexpr float INV_TWO_PI_M = 0.159154943091895336f;
+
+__device__ inline void kahan_add_m(float &hi, float &lo, float c, float x) {
+    float y = fmaf(c, x, lo);
+    float t = hi + y;
+    lo = y - (t - hi);
+    hi = t;
+}
+__device__ inline void df_scale_m(float &hi, float &lo, float d) {
+    float p = __fmul_rn(hi, d);
+    float e = fmaf(hi, d, -p) + __fmul_rn(lo, d);
+    hi = p + e;
+    lo = e - (hi - p);
+}
+
+struct S64 {
+    static constexpr int MINB = 1;             // the staged planes fill the LDS: one block per CU
+    static constexpr bool REGQ = true;         // registers to spare: own column of planes i, i+1 in registers
+    using T = double;
+    using B = Bufs<double>;
+    struct Own { double w; };
+    __device__ static const double *pin(const B &b) { return b.pin; }
+    __device__ static Own own_load(const B &b, int f, uint32_t fs, uint32_t e) {
+        return {ldo_nt(fbase(b.w, f, fs), e)};
+    }
+    template <int MODE, bool EMIT>
+    __device__ static void update(const KParams &kp, const B &b, int f, uint32_t fs, uint32_t e, double a,
+                                  double x, Own s, bool damp, double dmp) {
+        double v = s.w;
+        v += kp.hdt * a;                               // stage 4 (C:1680-1696)
+        if (damp) v *= dmp;                            // stage 5 (C:1454-1460)
+        const double w = v + kp.hdt * a;               // next stage 1 (C:1597-1613)
+        double xn;
+        if (MODE == 2 && f >= F_TH) {                  // next stage 2, links (C:1641-1651)
+            const double t = x + kp.gad * w;
+            xn = t - 2.0 * SCP_PI * rint(t / (2.0 * SCP_PI));
+        } else {
+            xn = x + kp.dt * w;                        // next stage 2 (C:1629-1638)
+        }
+        sto_nt(fbase(b.w, f, fs), e, w);
+        sto_nt(fbase(b.pout, f, fs), e, xn);
+        if constexpr (EMIT) {
+            __builtin_nontemporal_store(x, &b.xo[(size_t)f * fs + e]);
+            __builtin_nontemporal_store(v, &b.vo[(size_t)f * fs + e]);
+        }
+    }
+};
+
+template <int LOK> struct LoIO;
+template <> struct LoIO<LO_F32> {
+    __device__ static float get(const void *p, int f, uint32_t fs, uint32_t e) {
+        return ldo_nt(fbase((const float *)p, f, fs), e);
+    }
+    __device__ static void put(void *p, int f, uint32_t fs, uint32_t e, float v) {
+        sto_nt(fbase((float *)p, f, fs), e, v);
+    }
+};
+template <> struct LoIO<LO_BF16> {
+    __device__ static float get(const void *p, int f, uint32_t fs, uint32_t e) {
+        return __uint_as_float((uint32_t)ldo_nt(fbase((const uint16_t *)p, f, fs), e) << 16);
+    }
+    __device__ static void put(void *p, int f, uint32_t fs, uint32_t e, float v) {   // round to nearest even
+        uint32_t u = __float_as_uint(v);
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        sto_nt(fbase((uint16_t *)p, f, fs), e, (uint16_t)(u >> 16));
+    }
+};
+template <> struct LoIO<LO_NONE> {
+    __device__ static float get(const void *, int, uint32_t, uint32_t) { return 0.0f; }
+    __device__ static void put(void *, int, uint32_t, uint32_t, float) {}
+};
+
+template <int LOK>
+struct S32 {
+    static constexpr int MINB = 2;             // two blocks per CU: registers capped at 256
+    static constexpr bool REGQ = false;        // own column of planes i, i+1 read back from LDS
+    using T = float;
+    using B = Bufs32;
+    struct Own { float wh, wl, xl; };
+    __device__ static const float *pin(const B &b) { return b.pin; }
+    __device__ static Own own_load(const B &b, int f, uint32_t fs, uint32_t e) {
+        return {ldo_nt(fbase(b.w, f, fs), e), LoIO<LOK>::get(b.wlo, f, fs, e), LoIO<LOK>::get(b.pinlo, f, fs, e)};
+    }
+    template <int MODE, bool EMIT>
+    __device__ static void update(const KParams &kp, const B &b, int f, uint32_t fs, uint32_t e, float a,
+                                  float xh, Own s, bool damp, double dmpd) {
+        const float hdt = (float)kp.hdt, dt = (float)kp.dt, gad = (float)kp.gad;
+        float wh = s.wh, wl = s.wl, vh = 0.0f, vl = 0.0f;
+        if (EMIT || damp) {
+            vh = wh; vl = wl;
+            kahan_add_m(vh, vl, hdt, a);                   // stage 4: v = W + dt/2 a
+            if (damp) df_scale_m(vh, vl, (float)dmpd);     // stage 5
+            wh = vh; wl = vl;
+            kahan_add_m(wh, wl, hdt, a);                   // next stage 1
+        } else {
+            kahan_add_m(wh, wl, dt, a);                    // both half kicks at once
+        }
+        float xl = s.xl;
+        const float xh0 = xh, xl0 = xl;
+        if (MODE == 2 && f >= F_TH) {                       // link drift + wrap
+            kahan_add_m(xh, xl, gad, wh);
+            const float n = rintf(xh * INV_TWO_PI_M);
+            if (n != 0.0f) {
+                kahan_add_m(xh, xl, -n, TWO_PI_HI_M);
+                kahan_add_m(xh, xl, -n, TWO_PI_LO_M);
+            }
+        } else {
+            kahan_add_m(xh, xl, dt, wh);                    // next stage 2
+        }
+        sto_nt(fbase(b.w, f, fs), e, wh);
+        LoIO<LOK>::put(b.wlo, f, fs, e, wl);
+        sto_nt(fbase(b.pout, f, fs), e, xh);
+        LoIO<LOK>::put(b.poutlo, f, fs, e, xl);
+        if constexpr (EMIT) {
+            __builtin_nontemporal_store((double)xh0 + (double)xl0, &b.xo[(size_t)f * fs + e]);
+            __builtin_nontemporal_store((double)vh + (double)vl, &b.vo[(size_t)f * fs + e]);
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// the kernel
+// ---------------------------------------------------------------------------
+
+template <class S, int MODE, bool EMIT>
+__global__ void __launch_bounds__(MNT, S::MINB)
+k_march(KParams kp, typename S::B b, int DI) {
+    using T = typename S::T;
+    constexpr int NF = MODE == 2 ? 15 : 12;   // staged fields (matter + links)
+    constexpr int NM = 12;                    // matter fields (the i-1 plane is matter only)
+    constexpr int RQ = Halo<NF>::RQ;
+    __shared__ T lds[MNBUF * NF * MPL];
+
+    const int N = kp.N;
+    const MTile t = mtile(N, DI);
+    if (t.i0 >= N) return;                    // whole block: uniform
+    Halo<NF> hs;
+    hs.init(t);
+    const uint32_t fs = kp.fs, NN = (uint32_t)N * (uint32_t)N;
+    const T *__restrict__ pin = S::pin(b);
+    const uint32_t col = t.valid ? (uint32_t)t.j * (uint32_t)N + (uint32_t)t.k : 0u;
+    auto ld = [&](int f, uint32_t e) -> T { return ldo(fbase(pin, f, fs), e); };
+    // halo loads of one plane: row pairs (wave-uniform field) and this thread's column cell
+    auto halo_rows = [&](uint32_t pe, T *hr) {
+#pragma unroll
+        for (int q = 0; q < RQ; q++) {
+            const int pr = hs.wave + 4 * q;
+            hr[q] = T(0);
+            if (pr < 2 * NF && hs.rok) hr[q] = ld(pr >> 1, pe + ((pr & 1) ? hs.rbot : hs.rtop));
+        }
+    };
+    auto halo_col = [&](uint32_t pe) -> T {
+        return hs.cf >= 0 ? pin[(size_t)hs.cf * fs + pe + hs.cg] : T(0);
+    };
+    auto publish = [&](T *buf, const T *own, const T *hr, T hc) {
+        if (t.valid) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) buf[f * MPL + t.own] = own[f];
+        }
+#pragma unroll
+        for (int q = 0; q < RQ; q++) {
+            const int pr = hs.wave + 4 * q;
+            if (pr < 2 * NF && hs.rok) buf[(pr >> 1) * MPL + ((pr & 1) ? hs.lrow_bot : hs.lrow_top)] = hr[q];
+        }
+        if (hs.cf >= 0) buf[hs.cl] = hc;
+    };
+
+    // prologue: planes i0-1 (own column), i0 (own + halo), i0+1 (own + halo)
+    const int im0 = wm(t.i0, N), ip0 = wp(t.i0, N);
+    T cm[NM], cp[NF], hr[RQ], hc;
+    T cq[S::REGQ ? NF : 1];     // register copy of plane i (register-queue policy)
+    GaugeCarry<T> cy{};
+    {
+        const uint32_t em = (uint32_t)im0 * NN + col, e0 = (uint32_t)t.i0 * NN + col, ep = (uint32_t)ip0 * NN + col;
+        T c0[NF];
+        if (t.valid) {
+#pragma unroll
+            for (int f = 0; f < NM; f++) cm[f] = ld(f, em);
+#pragma unroll
+            for (int f = 0; f < NF; f++) { c0[f] = ld(f, e0); cp[f] = ld(f, ep); }
+            if constexpr (MODE == 2) {
+                const uint32_t jpo = (uint32_t)im0 * NN + (uint32_t)wp(t.j, N) * N + t.k;
+                const uint32_t kpo = (uint32_t)im0 * NN + (uint32_t)t.j * N + wp(t.k, N);
+                gauge_prologue(ld(F_TH + 0, em), ld(F_TH + 1, em), ld(F_TH + 2, em), c0[F_TH + 1], c0[F_TH + 2],
+                               ld(F_TH + 0, jpo), ld(F_TH + 0, kpo), cy);
+            }
+        } else {
+#pragma unroll
+            for (int f = 0; f < NM; f++) cm[f] = T(0);
+#pragma unroll
+            for (int f = 0; f < NF; f++) { c0[f] = T(0); cp[f] = T(0); }
+        }
+        T h0r[RQ];
+        halo_rows((uint32_t)t.i0 * NN, h0r);
+        const T h0c = halo_col((uint32_t)t.i0 * NN);
+        halo_rows((uint32_t)ip0 * NN, hr);
+        hc = halo_col((uint32_t)ip0 * NN);
+        publish(lds, c0, h0r, h0c);
+        if constexpr (S::REGQ) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) cq[f] = c0[f];
+        }
+    }
+
+    int bb = 0;
+    for (int i = t.i0; i < t.i1; i++) {
+        const int nbuf = (bb + 1 == MNBUF) ? 0 : bb + 1;
+        const uint32_t p2e = (uint32_t)wp(wp(i, N), N) * NN;
+        const uint32_t e = (uint32_t)i * NN + col;
+        // prefetch plane i+2 (own column + halo) and this plane's own-site velocity words
+        T cpp[NF], hrr[RQ];
+        typename S::Own sw[NF];
+        if (t.valid) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) cpp[f] = ld(f, p2e + col);
+#pragma unroll
+            for (int f = 0; f < NF; f++) sw[f] = S::own_load(b, f, fs, e);
+        } else {
+#pragma unroll
+            for (int f = 0; f < NF; f++) cpp[f] = T(0);
+        }
+        halo_rows(p2e, hrr);
+        const T hcc = halo_col(p2e);
+        // plane i+1 into the buffer that held plane i-2
+        publish(lds + nbuf * NF * MPL, cp, hr, hc);
+        __syncthreads();
+
+        // own column of planes i and i+1: registers (REGQ) or read back from the staged tiles
+        auto compute = [&](const auto &c0, const auto &c1) {
+            const LdsNB<T> nb{lds + bb * NF * MPL + t.own, lds + nbuf * NF * MPL + t.own};
+            T acc[NF];
+            if constexpr (MODE == 2)
+                forces_gauge(kp, c0, c1, cm, nb, cy, acc);
+            else
+                forces_complex(kp, c0, c1, cm, nb, acc);
+            // separate damped / undamped paths: a select between them would keep both live
+            double dmp = 1.0;
+            if (kp.damp && sponge(kp, i, t.j, t.k, dmp)) {
+#pragma unroll
+                for (int f = 0; f < NF; f++)
+                    S::template update<MODE, EMIT>(kp, b, f, fs, e, acc[f], c0[f], sw[f], true, dmp);
+            } else {
+#pragma unroll
+                for (int f = 0; f < NF; f++)
+                    S::template update<MODE, EMIT>(kp, b, f, fs, e, acc[f], c0[f], sw[f], false, 1.0);
+            }
+        };
+        if constexpr (S::REGQ) {
+            if (t.valid) compute(cq, cp);
+#pragma unroll
+            for (int f = 0; f < NM; f++) cm[f] = cq[f];
+#pragma unroll
+            for (int f = 0; f < NF; f++) { cq[f] = cp[f]; cp[f] = cpp[f]; }
+        } else {
+            const LdsCol<T> c0{lds + bb * NF * MPL + t.own}, c1{lds + nbuf * NF * MPL + t.own};
+            if (t.valid) compute(c0, c1);
+#pragma unroll
+            for (int f = 0; f < NM; f++) cm[f] = c0[f];
+#pragma unroll
+            for (int f = 0; f < NF; f++) cp[f] = cpp[f];
+        }
+#pragma unroll
+        for (int q = 0; q < RQ; q++) hr[q] = hrr[q];
+        hc = hcc;
+        bb = nbuf;
+    }
+}
+
+// planes per block: deep enough to amortise the 3-plane prologue, shallow
+// enough for >= ~8 blocks per CU
+inline int march_di(int N) {
+    const uint32_t tiles = ((N + TK - 1) / TK) * ((N + MTJ - 1) / MTJ);
+    int di = N;
+    while (di > 16 && (uint64_t)tiles * ((N + di - 1) / di) < 8192) di /= 2;
+    return di;
+}
+
+template <class S, int MODE>
+inline int launch_march_t(bool emit, const KParams &kp, const typename S::B &b, int DI, hipStream_t s) {
+    if (DI <= 0) DI = march_di(kp.N);
+    const uint32_t ntk = (kp.N + TK - 1) / TK, ntj = (kp.N + MTJ - 1) / MTJ, nseg = (kp.N + DI - 1) / DI;
+    const dim3 grid(ntk * ntj * nseg), block(TK, MTJ);
+    if (emit) hipLaunchKernelGGL((k_march<S, MODE, true>), grid, block, 0, s, kp, b, DI);
+    else hipLaunchKernelGGL((k_march<S, MODE, false>), grid, block, 0, s, kp, b, DI);
+    return (int)hipGetLastError();
+}
+
+}  // namespace scp

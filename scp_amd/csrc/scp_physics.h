@@ -80,12 +80,33 @@ __device__ inline R plaq(R ta_y, R tb_ya, R ta_yb, R tb_y) {
 }
 
 
-// Complex-mode accelerations at one site (compute_forces_complex, C:931-994).
-// c/ip/im: own column at planes i, i+1, i-1; X(f, idx) loads neighbour values.
+// In-plane neighbour access for the per-site physics. A neighbour provider NB
+// has face(f, d, sg): field f at the site one step along d (1 = j, 2 = k) in
+// direction sg; and the four link diagonals the staples need:
+//   th1_jm_kp: th_1(i, j-1, k+1)   th2_jp_km: th_2(i, j+1, k-1)
+//   th1_ip_jm: th_1(i+1, j-1, k)   th2_ip_km: th_2(i+1, j, k-1)
+// GlobalNB reads them from the device arrays; the LDS-staged kernels read them
+// from the staged plane tiles.
 template <typename R, class LD>
-__device__ inline void forces_complex(const KParams &kp, const R *c, const R *ip,
-                                      const R *im, LD X, uint32_t jp, uint32_t jm,
-                                      uint32_t kp_, uint32_t km, R *acc) {
+struct GlobalNB {
+    LD X;
+    int N, i, j, k;
+    uint32_t jp, jm, kp, km;
+    __device__ R face(int f, int d, int sg) const {
+        return X(f, d == 1 ? (sg > 0 ? jp : jm) : (sg > 0 ? kp : km));
+    }
+    __device__ R th1_jm_kp() const { return X(F_TH + 1, sidx(N, i, wm(j, N), wp(k, N))); }
+    __device__ R th2_jp_km() const { return X(F_TH + 2, sidx(N, i, wp(j, N), wm(k, N))); }
+    __device__ R th1_ip_jm() const { return X(F_TH + 1, sidx(N, wp(i, N), wm(j, N), k)); }
+    __device__ R th2_ip_km() const { return X(F_TH + 2, sidx(N, wp(i, N), j, wm(k, N))); }
+};
+
+// Complex-mode accelerations at one site (compute_forces_complex, C:931-994).
+// c/ip/im: own column at planes i, i+1, i-1 (anything indexable by field);
+// nb: in-plane neighbours.
+template <typename R, class CV, class PV, class MV, class NB>
+__device__ inline void forces_complex(const KParams &kp, const CV &c, const PV &ip,
+                                      const MV &im, const NB &nb, R *acc) {
     const R h1 = R(kp.h1), h2 = R(kp.h2);
     R s2[3];
 #pragma unroll
@@ -97,8 +118,7 @@ __device__ inline void forces_complex(const KParams &kp, const R *c, const R *ip
     // neighbour value of field f in direction d (+1/-1)
     auto nbv = [&](int f, int d, int sg) -> R {
         if (d == 0) return sg > 0 ? ip[f] : im[f];
-        if (d == 1) return X(f, sg > 0 ? jp : jm);
-        return X(f, sg > 0 ? kp_ : km);
+        return nb.face(f, d, sg);
     };
     auto lap = [&](int f) -> R {
         return (nbv(f, 0, 1) + nbv(f, 0, -1) + nbv(f, 1, 1) + nbv(f, 1, -1) + nbv(f, 2, 1) +
@@ -227,16 +247,13 @@ struct GaugeCarry {
 // Gauged accelerations (compute_forces_complex_gauge, C:1053-1263), one
 // gauge, no Higgs. Writes acc[0..11] (matter, device order) and acc[12..14]
 // (E_acc), and the carry for plane i+1.
-template <typename R, class LD>
-__device__ inline void forces_gauge(const KParams &kp, const R *c, const R *ip,
-                                    const R *im, LD X, int N, int i, int j, int k,
-                                    uint32_t jp, uint32_t jm, uint32_t kp_, uint32_t km,
-                                    GaugeCarry<R> &cy, R *acc) {
+template <typename R, class CV, class PV, class MV, class NB>
+__device__ inline void forces_gauge(const KParams &kp, const CV &c, const PV &ip,
+                                    const MV &im, const NB &nb, GaugeCarry<R> &cy, R *acc) {
     const R h1 = R(kp.h1), h2 = R(kp.h2);
     auto nbv = [&](int f, int d, int sg) -> R {
         if (d == 0) return sg > 0 ? ip[f] : im[f];
-        if (d == 1) return X(f, sg > 0 ? jp : jm);
-        return X(f, sg > 0 ? kp_ : km);
+        return nb.face(f, d, sg);
     };
     // links and their cos/sin at x and x - e_d (C:1071-1079, 1144-1145)
     R cP[3], sP[3], cM[3], sM[3];
@@ -247,19 +264,18 @@ __device__ inline void forces_gauge(const KParams &kp, const R *c, const R *ip,
     scp_sincos(nbv(F_TH + 2, 2, -1), &sM[2], &cM[2]);
 
     // plaquettes (C:1088-1109) at x, x-e1, x-e2; x-e0 carried
-    const int ip1 = wp(i, N), jm1 = wm(j, N), kp1 = wp(k, N), km1 = wm(k, N), jp1 = wp(j, N);
     R th0 = c[F_TH + 0], th1 = c[F_TH + 1], th2 = c[F_TH + 2];
     R px0 = plaq(th0, ip[F_TH + 1], nbv(F_TH + 0, 1, 1), th1);       // (0,1) at x
     R px1 = plaq(th1, nbv(F_TH + 2, 1, 1), nbv(F_TH + 1, 2, 1), th2); // (1,2) at x
     R px2 = plaq(th2, nbv(F_TH + 0, 2, 1), ip[F_TH + 2], th0);       // (2,0) at x
     // y = x - e1
     R t1_ym = nbv(F_TH + 1, 1, -1), t0_ym = nbv(F_TH + 0, 1, -1), t2_ym = nbv(F_TH + 2, 1, -1);
-    R p0_ym = plaq(t0_ym, X(F_TH + 1, sidx(N, ip1, jm1, k)), th0, t1_ym);
-    R p1_ym = plaq(t1_ym, th2, X(F_TH + 1, sidx(N, i, jm1, kp1)), t2_ym);
+    R p0_ym = plaq(t0_ym, nb.th1_ip_jm(), th0, t1_ym);
+    R p1_ym = plaq(t1_ym, th2, nb.th1_jm_kp(), t2_ym);
     // y = x - e2
     R t1_zm = nbv(F_TH + 1, 2, -1), t2_zm = nbv(F_TH + 2, 2, -1), t0_zm = nbv(F_TH + 0, 2, -1);
-    R p1_zm = plaq(t1_zm, X(F_TH + 2, sidx(N, i, jp1, km1)), th1, t2_zm);
-    R p2_zm = plaq(t2_zm, th0, X(F_TH + 2, sidx(N, ip1, j, km1)), t0_zm);
+    R p1_zm = plaq(t1_zm, nb.th2_jp_km(), th1, t2_zm);
+    R p2_zm = plaq(t2_zm, th0, nb.th2_ip_km(), t0_zm);
     R st[3];
     st[0] = (px0 - p0_ym) - (px2 - p2_zm);
     st[1] = -(px0 - cy.p0) + (px1 - p1_zm);
@@ -340,15 +356,14 @@ __device__ inline void forces_gauge(const KParams &kp, const R *c, const R *ip,
     cy.c0 = cP[0]; cy.s0 = sP[0]; cy.p0 = px0; cy.p2 = px2;
 }
 
-// carry for the segment's first plane: quantities at (i-1, j, k)
-template <typename R, class LD>
-__device__ inline void gauge_prologue(const KParams &kp, const R *cm, const R *c0, LD X,
-                                      int N, int im1, int j, int k, GaugeCarry<R> &cy) {
-    scp_sincos(cm[F_TH + 0], &cy.s0, &cy.c0);
-    const int jp1 = wp(j, N), kp1 = wp(k, N);
-    cy.p0 = plaq(cm[F_TH + 0], c0[F_TH + 1], X(F_TH + 0, sidx(N, im1, jp1, k)), cm[F_TH + 1]);
-    cy.p2 = plaq(cm[F_TH + 2], X(F_TH + 0, sidx(N, im1, j, kp1)), c0[F_TH + 2], cm[F_TH + 0]);
-    (void)kp;
+// carry for the segment's first plane, from the links at (i-1, j, k) (mt0..2),
+// th_1, th_2 at (i, j, k) and th_0 at (i-1, j+1, k), (i-1, j, k+1)
+template <typename R>
+__device__ inline void gauge_prologue(R mt0, R mt1, R mt2, R th1, R th2, R t0_mjp, R t0_mkp,
+                                      GaugeCarry<R> &cy) {
+    scp_sincos(mt0, &cy.s0, &cy.c0);
+    cy.p0 = plaq(mt0, th1, t0_mjp, mt1);
+    cy.p2 = plaq(mt2, t0_mkp, th2, mt0);
 }
 
 // sponge factor of scp_sim.c:1447-1453 (1.0 and false outside the sponge)

@@ -92,7 +92,11 @@ k_step32(KParams kp, Bufs32 b, int DI) {
         for (int f = 0; f < NP; f++) { cm[f] = X(f, a); c0[f] = X(f, c); }
     }
     GaugeCarry<float> cy{};
-    if constexpr (MODE == 2) gauge_prologue(kp, cm, c0, X, N, wm(g.i0, N), j, k, cy);
+    if constexpr (MODE == 2) {
+        const int im0 = wm(g.i0, N);
+        gauge_prologue(cm[F_TH + 0], cm[F_TH + 1], cm[F_TH + 2], c0[F_TH + 1], c0[F_TH + 2],
+                       X(F_TH + 0, sidx(N, im0, jp1, k)), X(F_TH + 0, sidx(N, im0, j, kp1)), cy);
+    }
     const float hdt = (float)kp.hdt, dt = (float)kp.dt, gad = (float)kp.gad;
 
     for (int i = g.i0; i < g.i1; i++) {
@@ -106,10 +110,11 @@ k_step32(KParams kp, Bufs32 b, int DI) {
         const uint32_t jp = sidx(N, i, jp1, k), jm = sidx(N, i, jm1, k);
         const uint32_t kpx = sidx(N, i, j, kp1), kmx = sidx(N, i, j, km1);
         float acc[NP];
+        const GlobalNB<float, decltype(X)> nb{X, N, i, j, k, jp, jm, kpx, kmx};
         if constexpr (MODE == 2)
-            forces_gauge(kp, c0, cp, cm, X, N, i, j, k, jp, jm, kpx, kmx, cy, acc);
+            forces_gauge(kp, c0, cp, cm, nb, cy, acc);
         else if constexpr (MODE == 1)
-            forces_complex(kp, c0, cp, cm, X, jp, jm, kpx, kmx, acc);
+            forces_complex(kp, c0, cp, cm, nb, acc);
         else
             forces_real(kp, c0, cp, cm, X, AUX, ci, sidx(N, ip1, j, k), sidx(N, im1, j, k), jp, jm,
                         kpx, kmx, acc);
