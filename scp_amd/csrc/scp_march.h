@@ -155,10 +155,25 @@ constexpr float TWO_PI_HI_M = 6.28318548202514648f;      // fl32(2 pi)
 constexpr float TWO_PI_LO_M = -1.7484555314695172e-07f;  // 2 pi - TWO_PI_HI_M
 constexpr float INV_TWO_PI_M = 0.159154943091895336f;
 
-__device__ inline void kahan_add_m(float &hi, float &lo, float c, float x) {
-    float y = fmaf(c, x, lo);
-    float t = hi + y;
-    lo = y - (t - hi);
+// A constant as an unevaluated pair c = h + l of FP32 words (the FP64 step
+// constants dt, dt/2, -g dx dt split once), so increments are formed with the
+// FP64 constant and not with its FP32 rounding.
+struct DF {
+    float h, l;
+};
+__device__ inline DF df_split(double c) {
+    const float h = (float)c;
+    return {h, (float)(c - (double)h)};
+}
+
+// (hi, lo) += c * (xh + xl): the increment is formed with FMAs and added by
+// Knuth's TwoSum, which is exact for any magnitudes (velocities and fields
+// pass through zero, so |hi| >= |increment| cannot be assumed).
+__device__ inline void comp_add(float &hi, float &lo, DF c, float xh, float xl) {
+    const float y = fmaf(c.h, xh, fmaf(c.l, xh, fmaf(c.h, xl, lo)));
+    const float t = hi + y;
+    const float bp = t - hi;
+    lo = (hi - (t - bp)) + (y - bp);
     hi = t;
 }
 __device__ inline void df_scale_m(float &hi, float &lo, float d) {
@@ -171,6 +186,7 @@ __device__ inline void df_scale_m(float &hi, float &lo, float d) {
 struct S64 {
     static constexpr int MINB = 1;             // the staged planes fill the LDS: one block per CU
     static constexpr bool REGQ = true;         // registers to spare: own column of planes i, i+1 in registers
+    static constexpr bool SPLIT_DAMP = false;
     using T = double;
     using B = Bufs<double>;
     struct Own { double w; };
@@ -229,6 +245,7 @@ template <int LOK>
 struct S32 {
     static constexpr int MINB = 2;             // two blocks per CU: registers capped at 256
     static constexpr bool REGQ = false;        // own column of planes i, i+1 read back from LDS
+    static constexpr bool SPLIT_DAMP = true;
     using T = float;
     using B = Bufs32;
     struct Own { float wh, wl, xl; };
@@ -239,28 +256,27 @@ struct S32 {
     template <int MODE, bool EMIT>
     __device__ static void update(const KParams &kp, const B &b, int f, uint32_t fs, uint32_t e, float a,
                                   float xh, Own s, bool damp, double dmpd) {
-        const float hdt = (float)kp.hdt, dt = (float)kp.dt, gad = (float)kp.gad;
+        const DF hdt = df_split(kp.hdt), dt = df_split(kp.dt), gad = df_split(kp.gad);
         float wh = s.wh, wl = s.wl, vh = 0.0f, vl = 0.0f;
         if (EMIT || damp) {
             vh = wh; vl = wl;
-            kahan_add_m(vh, vl, hdt, a);                   // stage 4: v = W + dt/2 a
+            comp_add(vh, vl, hdt, a, 0.0f);                   // stage 4: v = W + dt/2 a
             if (damp) df_scale_m(vh, vl, (float)dmpd);     // stage 5
             wh = vh; wl = vl;
-            kahan_add_m(wh, wl, hdt, a);                   // next stage 1
+            comp_add(wh, wl, hdt, a, 0.0f);                   // next stage 1
         } else {
-            kahan_add_m(wh, wl, dt, a);                    // both half kicks at once
+            comp_add(wh, wl, dt, a, 0.0f);                   // both half kicks at once
         }
         float xl = s.xl;
         const float xh0 = xh, xl0 = xl;
         if (MODE == 2 && f >= F_TH) {                       // link drift + wrap
-            kahan_add_m(xh, xl, gad, wh);
+            comp_add(xh, xl, gad, wh, wl);
             const float n = rintf(xh * INV_TWO_PI_M);
             if (n != 0.0f) {
-                kahan_add_m(xh, xl, -n, TWO_PI_HI_M);
-                kahan_add_m(xh, xl, -n, TWO_PI_LO_M);
+                comp_add(xh, xl, DF{-n, 0.0f}, TWO_PI_HI_M, TWO_PI_LO_M);
             }
         } else {
-            kahan_add_m(xh, xl, dt, wh);                    // next stage 2
+            comp_add(xh, xl, dt, wh, wl);                   // next stage 2
         }
         sto_nt(fbase(b.w, f, fs), e, wh);
         LoIO<LOK>::put(b.wlo, f, fs, e, wl);
@@ -388,16 +404,23 @@ k_march(KParams kp, typename S::B b, int DI) {
                 forces_gauge(kp, c0, c1, cm, nb, cy, acc);
             else
                 forces_complex(kp, c0, c1, cm, nb, acc);
-            // separate damped / undamped paths: a select between them would keep both live
             double dmp = 1.0;
-            if (kp.damp && sponge(kp, i, t.j, t.k, dmp)) {
+            if constexpr (S::SPLIT_DAMP) {
+                // separate damped / undamped paths: a select between them would keep both live
+                if (kp.damp && sponge(kp, i, t.j, t.k, dmp)) {
 #pragma unroll
-                for (int f = 0; f < NF; f++)
-                    S::template update<MODE, EMIT>(kp, b, f, fs, e, acc[f], c0[f], sw[f], true, dmp);
+                    for (int f = 0; f < NF; f++)
+                        S::template update<MODE, EMIT>(kp, b, f, fs, e, acc[f], c0[f], sw[f], true, dmp);
+                } else {
+#pragma unroll
+                    for (int f = 0; f < NF; f++)
+                        S::template update<MODE, EMIT>(kp, b, f, fs, e, acc[f], c0[f], sw[f], false, 1.0);
+                }
             } else {
+                const bool damp = kp.damp && sponge(kp, i, t.j, t.k, dmp);
 #pragma unroll
                 for (int f = 0; f < NF; f++)
-                    S::template update<MODE, EMIT>(kp, b, f, fs, e, acc[f], c0[f], sw[f], false, 1.0);
+                    S::template update<MODE, EMIT>(kp, b, f, fs, e, acc[f], c0[f], sw[f], damp, dmp);
             }
         };
         if constexpr (S::REGQ) {

@@ -8,7 +8,22 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace scp {
+
+// FP32 stencils use difference forms: sum_n (x_n - x) instead of sum_n x_n -
+// 6x, and transported differences U x_n - x = (x_n - x) + (U - 1) x_n with
+// cos(th) - 1 formed without cancellation. For smooth fields x_n - x is exact
+// in FP32 (Sterbenz), so the Laplacian carries a rounding error relative to
+// itself instead of to the field (~|x| eps h2 per site). FP64 keeps the
+// reference's expression trees (and so its rounding) exactly.
+template <typename R>
+constexpr bool diff_form() { return std::is_same<R, float>::value; }
+
+// cos(th) - 1 from (c, s) = (cos th, sin th) without cancellation near th = 0
+template <typename R>
+__device__ inline R cos_m1(R c, R s) { return c >= R(0) ? -(s * s) / (R(1) + c) : c - R(1); }
 
 #define SCP_PI 3.14159265358979323846
 
@@ -121,6 +136,11 @@ __device__ inline void forces_complex(const KParams &kp, const CV &c, const PV &
         return nb.face(f, d, sg);
     };
     auto lap = [&](int f) -> R {
+        if constexpr (diff_form<R>()) {
+            const R x = c[f];
+            return (((nbv(f, 0, 1) - x) + (nbv(f, 0, -1) - x)) + ((nbv(f, 1, 1) - x) + (nbv(f, 1, -1) - x)) +
+                    ((nbv(f, 2, 1) - x) + (nbv(f, 2, -1) - x))) * h2;
+        }
         return (nbv(f, 0, 1) + nbv(f, 0, -1) + nbv(f, 1, 1) + nbv(f, 1, -1) + nbv(f, 2, 1) +
                 nbv(f, 2, -1) - R(6.0) * c[f]) * h2;
     };
@@ -207,8 +227,15 @@ __device__ inline void forces_real(const KParams &kp, const R *c, const R *ip,
     R csq = cph[0] * cph[0] + cph[1] * cph[1] + cph[2] * cph[2];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        R lp = (nbv(a, 0, 1) + nbv(a, 0, -1) + nbv(a, 1, 1) + nbv(a, 1, -1) + nbv(a, 2, 1) +
-                     nbv(a, 2, -1) - R(6.0) * c[a]) * h2;
+        R lp;
+        if constexpr (diff_form<R>()) {
+            const R x = c[a];
+            lp = (((nbv(a, 0, 1) - x) + (nbv(a, 0, -1) - x)) + ((nbv(a, 1, 1) - x) + (nbv(a, 1, -1) - x)) +
+                  ((nbv(a, 2, 1) - x) + (nbv(a, 2, -1) - x))) * h2;
+        } else {
+            lp = (nbv(a, 0, 1) + nbv(a, 0, -1) + nbv(a, 1, 1) + nbv(a, 1, -1) + nbv(a, 2, 1) +
+                  nbv(a, 2, -1) - R(6.0) * c[a]) * h2;
+        }
         R dPda = (a == 0) ? p1 * p2 : (a == 1) ? p0 * p2 : p0 * p1;
         R ct = curlF(TU, a);
         R chiral = 0;
@@ -227,8 +254,16 @@ __device__ inline void forces_real(const KParams &kp, const R *c, const R *ip,
     }
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        R lt = (nbv(TU + a, 0, 1) + nbv(TU + a, 0, -1) + nbv(TU + a, 1, 1) + nbv(TU + a, 1, -1) +
-                     nbv(TU + a, 2, 1) + nbv(TU + a, 2, -1) - R(6.0) * c[TU + a]) * h2;
+        R lt;
+        if constexpr (diff_form<R>()) {
+            const R x = c[TU + a];
+            lt = (((nbv(TU + a, 0, 1) - x) + (nbv(TU + a, 0, -1) - x)) +
+                  ((nbv(TU + a, 1, 1) - x) + (nbv(TU + a, 1, -1) - x)) +
+                  ((nbv(TU + a, 2, 1) - x) + (nbv(TU + a, 2, -1) - x))) * h2;
+        } else {
+            lt = (nbv(TU + a, 0, 1) + nbv(TU + a, 0, -1) + nbv(TU + a, 1, 1) + nbv(TU + a, 1, -1) +
+                  nbv(TU + a, 2, 1) + nbv(TU + a, 2, -1) - R(6.0) * c[TU + a]) * h2;
+        }
         R cp = curlF(0, a);
         R cos_th = 0;
         if (ACS != 0) cos_th = R(2.0) * ACS * AUX(a, ci);
@@ -303,6 +338,10 @@ __device__ inline void forces_gauge(const KParams &kp, const CV &c, const PV &ip
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         R TRp[6], TIp[6], tup[3], tvp[3];
+        R Jg = R(0.0);
+        // FP32: transported differences U x_n - x (see diff_form)
+        const R cmP = diff_form<R>() ? cos_m1(cP[d], sP[d]) : R(0);
+        const R cmM = diff_form<R>() ? cos_m1(cM[d], sM[d]) : R(0);
 #pragma unroll
         for (int a = 0; a < 3; a++) {
 #pragma unroll
@@ -310,23 +349,38 @@ __device__ inline void forces_gauge(const KParams &kp, const CV &c, const PV &ip
                 const int fuix = h ? F_TU + a : F_U + a, fvix = h ? F_TV + a : F_V + a;
                 R upn = nbv(fuix, d, 1), vpn = nbv(fvix, d, 1);
                 R umn = nbv(fuix, d, -1), vmn = nbv(fvix, d, -1);
-                R trp = cP[d] * upn - sP[d] * vpn, tip = cP[d] * vpn + sP[d] * upn;
-                R trm = cM[d] * umn + sM[d] * vmn, tim = cM[d] * vmn - sM[d] * umn;
                 const int f = h * 3 + a;
-                if (d == 0) { lU[f] = trp + trm; lV[f] = tip + tim; }
-                else { lU[f] = lU[f] + trp; lU[f] = lU[f] + trm; lV[f] = lV[f] + tip; lV[f] = lV[f] + tim; }
-                DU[d][f] = (trp - trm) * h1;
-                DV[d][f] = (tip - tim) * h1;
-                TRp[f] = trp; TIp[f] = tip;
+                if constexpr (diff_form<R>()) {
+                    const R u = fu[f], v = fv[f];
+                    const R epr = (upn - u) + (cmP * upn - sP[d] * vpn);   // trp - u
+                    const R epi = (vpn - v) + (cmP * vpn + sP[d] * upn);   // tip - v
+                    const R emr = (umn - u) + (cmM * umn + sM[d] * vmn);   // trm - u
+                    const R emi = (vmn - v) + (cmM * vmn - sM[d] * umn);   // tim - v
+                    if (d == 0) { lU[f] = epr + emr; lV[f] = epi + emi; }
+                    else { lU[f] = lU[f] + (epr + emr); lV[f] = lV[f] + (epi + emi); }
+                    DU[d][f] = (epr - emr) * h1;
+                    DV[d][f] = (epi - emi) * h1;
+                    TRp[f] = u + epr; TIp[f] = v + epi;
+                    Jg += u * epi - v * epr;                               // = u tip - v trp
+                } else {
+                    R trp = cP[d] * upn - sP[d] * vpn, tip = cP[d] * vpn + sP[d] * upn;
+                    R trm = cM[d] * umn + sM[d] * vmn, tim = cM[d] * vmn - sM[d] * umn;
+                    if (d == 0) { lU[f] = trp + trm; lV[f] = tip + tim; }
+                    else { lU[f] = lU[f] + trp; lU[f] = lU[f] + trm; lV[f] = lV[f] + tip; lV[f] = lV[f] + tim; }
+                    DU[d][f] = (trp - trm) * h1;
+                    DV[d][f] = (tip - tim) * h1;
+                    TRp[f] = trp; TIp[f] = tip;
+                }
                 if (h) { tup[a] = upn; tvp[a] = vpn; }
             }
         }
         // lattice current J_d and E-kick (C:1232-1250)
-        R Jg = R(0.0);
+        if constexpr (!diff_form<R>()) {
 #pragma unroll
-        for (int a = 0; a < 3; a++) {
-            Jg += fu[a] * TIp[a] - fv[a] * TRp[a];
-            Jg += fu[3 + a] * TIp[3 + a] - fv[3 + a] * TRp[3 + a];
+            for (int a = 0; a < 3; a++) {
+                Jg += fu[a] * TIp[a] - fv[a] * TRp[a];
+                Jg += fu[3 + a] * TIp[3 + a] - fv[3 + a] * TRp[3 + a];
+            }
         }
         const int d1 = C1[d], d2 = C2[d];
         R WpR1 = cP[d] * fu[d2] + sP[d] * fv[d2], WpI1 = cP[d] * fv[d2] - sP[d] * fu[d2];
@@ -344,10 +398,15 @@ __device__ inline void forces_gauge(const KParams &kp, const CV &c, const PV &ip
         R imDxT = DV[d1][3 + C2[a]] - DV[d2][3 + C1[a]];
         R reDxP = DU[d1][C2[a]] - DU[d2][C1[a]];
         R imDxP = DV[d1][C2[a]] - DV[d2][C1[a]];
-        R lUP = (lU[a] - R(6.0) * fu[a]) * h2;
-        R lVP = (lV[a] - R(6.0) * fv[a]) * h2;
-        R lUT = (lU[3 + a] - R(6.0) * fu[3 + a]) * h2;
-        R lVT = (lV[3 + a] - R(6.0) * fv[3 + a]) * h2;
+        R lUP, lVP, lUT, lVT;
+        if constexpr (diff_form<R>()) {     // lU, lV already hold sum_n (U x_n - x)
+            lUP = lU[a] * h2; lVP = lV[a] * h2; lUT = lU[3 + a] * h2; lVT = lV[3 + a] * h2;
+        } else {
+            lUP = (lU[a] - R(6.0) * fu[a]) * h2;
+            lVP = (lV[a] - R(6.0) * fv[a]) * h2;
+            lUT = (lU[3 + a] - R(6.0) * fu[3 + a]) * h2;
+            lVT = (lV[3 + a] - R(6.0) * fv[3 + a]) * h2;
+        }
         acc[F_U + a] = lUP - R(kp.MASS2) * fu[a] - R(2.0) * Vp * fu[a] * rest[a] + R(kp.ETA) * reDxT;
         acc[F_V + a] = lVP - R(kp.MASS2) * fv[a] - R(2.0) * Vp * fv[a] * rest[a] + R(kp.ETA) * imDxT;
         acc[F_TU + a] = lUT - R(kp.MT2) * fu[3 + a] + R(kp.ETA) * reDxP;

@@ -6,7 +6,7 @@
 // at all (LO_NONE, plain FP32 for comparison). The stencil — forces, curls,
 // covariant transport, plaquettes — is evaluated in FP32 on the hi words; the
 // leapfrog accumulations v += h a, x += dt v, E and link updates are
-// compensated FP32 additions (Fast2Sum / Kahan with the product fused), so
+// compensated FP32 additions (TwoSum with the increment formed by FMAs), so
 // increments far below ulp(x) accumulate in lo instead of being lost. Between
 // two emitted steps the two half kicks fuse into one W += dt a. FP64 appears
 // only in INIT (from the FP64 host state) and in the emitted snapshot state
@@ -30,13 +30,25 @@ __device__ inline void fast_two_sum(float a, float b, float &s, float &e) {
     s = a + b;
     e = b - (s - a);
 }
-// (hi, lo) += c*x with one FMA and a Fast2Sum (Kahan/Dekker): exact in the
-// sum whenever |hi| >= |c*x + lo|, which is the case that matters — an
-// increment far below ulp(hi) lands in lo instead of being rounded away.
-__device__ inline void kahan_add(float &hi, float &lo, float c, float x) {
-    float y = fmaf(c, x, lo);
-    float t = hi + y;
-    lo = y - (t - hi);
+// A constant as an unevaluated pair c = h + l of FP32 words (the FP64 step
+// constants dt, dt/2, -g dx dt split once), so increments are formed with the
+// FP64 constant and not with its FP32 rounding.
+struct DF {
+    float h, l;
+};
+__device__ inline DF df_split(double c) {
+    const float h = (float)c;
+    return {h, (float)(c - (double)h)};
+}
+
+// (hi, lo) += c * (xh + xl): the increment is formed with FMAs and added by
+// Knuth's TwoSum, which is exact for any magnitudes (velocities and fields
+// pass through zero, so |hi| >= |increment| cannot be assumed).
+__device__ inline void comp_add(float &hi, float &lo, DF c, float xh, float xl) {
+    const float y = fmaf(c.h, xh, fmaf(c.l, xh, fmaf(c.h, xl, lo)));
+    const float t = hi + y;
+    const float bp = t - hi;
+    lo = (hi - (t - bp)) + (y - bp);
     hi = t;
 }
 // (hi, lo) *= d
@@ -97,7 +109,7 @@ k_step32(KParams kp, Bufs32 b, int DI) {
         gauge_prologue(cm[F_TH + 0], cm[F_TH + 1], cm[F_TH + 2], c0[F_TH + 1], c0[F_TH + 2],
                        X(F_TH + 0, sidx(N, im0, jp1, k)), X(F_TH + 0, sidx(N, im0, j, kp1)), cy);
     }
-    const float hdt = (float)kp.hdt, dt = (float)kp.dt, gad = (float)kp.gad;
+    const DF hdt = df_split(kp.hdt), dt = df_split(kp.dt), gad = df_split(kp.gad);
 
     for (int i = g.i0; i < g.i1; i++) {
         const int ip1 = wp(i, N), im1 = wm(i, N);
@@ -156,24 +168,23 @@ k_step32(KParams kp, Bufs32 b, int DI) {
                 float vh = 0.0f, vl = 0.0f;
                 if (EMIT || damp) {
                     vh = wh; vl = wl;
-                    kahan_add(vh, vl, hdt, acc[f]);                // stage 4: v = W + dt/2 a
+                    comp_add(vh, vl, hdt, acc[f], 0.0f);                // stage 4: v = W + dt/2 a
                     if (damp) df_scale(vh, vl, dmp);               // stage 5
                     wh = vh; wl = vl;
-                    kahan_add(wh, wl, hdt, acc[f]);                // next stage 1
+                    comp_add(wh, wl, hdt, acc[f], 0.0f);                // next stage 1
                 } else {
-                    kahan_add(wh, wl, dt, acc[f]);                 // both half kicks at once
+                    comp_add(wh, wl, dt, acc[f], 0.0f);                // both half kicks at once
                 }
                 float xh = c0[f], xl = LS::get(b.pinlo, o);
                 const float xh0 = xh, xl0 = xl;
                 if (MODE == 2 && f >= F_TH) {                       // link drift + wrap
-                    kahan_add(xh, xl, gad, wh);
+                    comp_add(xh, xl, gad, wh, wl);
                     float n = rintf(xh * INV_TWO_PI);
                     if (n != 0.0f) {
-                        kahan_add(xh, xl, -n, TWO_PI_HI);
-                        kahan_add(xh, xl, -n, TWO_PI_LO);
+                        comp_add(xh, xl, DF{-n, 0.0f}, TWO_PI_HI, TWO_PI_LO);
                     }
                 } else {
-                    kahan_add(xh, xl, dt, wh);                      // next stage 2
+                    comp_add(xh, xl, dt, wh, wl);                     // next stage 2
                 }
                 __builtin_nontemporal_store(wh, &b.w[o]);
                 LS::put(b.wlo, o, wl);

@@ -2,7 +2,8 @@
 
 Stated bars (every FP32C result carries its error against FP64):
   field L2 (relative)            <= 2e-5   after 200 steps of a gauged 32^3 ball
-  energy, |dQ|/Q (relative)      <= 1e-6   same run
+  energy, |dQ|/Q (relative)      <= 1e-6   same run, and over 10^4 steps of the
+                                           128^3 two-ball run and of a 48^3 ball
   Gauss residual                 <= 1e-4   absolute (the FP32 current conserves
                                            Gauss's law only to FP32 rounding; the
                                            FP64 reduction reports it)
@@ -85,3 +86,52 @@ def test_f32c_complex_and_real_modes_run():
         b, _, d32 = run(p32, st, 50, project=False)
         assert np.linalg.norm(b - a) / np.linalg.norm(a) <= 2e-5
         assert abs(d32["E_total"] - d64["E_total"]) <= 1e-6 * abs(d64["E_total"])
+
+
+def q_halves(st, N):
+    """Noether charge of Phi in the x < 0 and x >= 0 halves (FP64, host)."""
+    q = sum(st[a] * st[18 + a] - st[12 + a] * st[6 + a] for a in range(3)).reshape(N, N, N)
+    return q[: N // 2].sum(), q[N // 2:].sum()
+
+
+@pytest.mark.parametrize("lo", [1, 0])
+def test_f32c_ten_thousand_steps_two_balls(lo):
+    """128^3 gauged, two opposite Q-balls 24 cells apart (omega = +-0.8), links
+    N(0, 1e-3), E = 0, Gauss-projected; 10^4 steps FP32C vs FP64."""
+    N, steps = 128, 10000
+    L = 0.25 * (N - 1)
+    balls = [dict(x0=-6.0, y0=0.0, z0=0.0, amp=0.8, sigma=3.0, omega=0.8, phase=0.0, winding=1),
+             dict(x0=6.0, y0=0.0, z0=0.0, amp=0.8, sigma=3.0, omega=-0.8, phase=0.0, winding=-1)]
+    st = seeds.seed_balls(N, L, 2, balls, theta_sigma=0.01, link_sigma=1e-3, seed=7)
+    p64 = G.make_params(N, G.MODE_GAUGED, L, 0.2, g=0.05)
+    p32 = G.make_params(N, G.MODE_GAUGED, L, 0.2, precision=G.PREC_F32C, lo_words=lo, g=0.05)
+    a, _, d64 = run(p64, st, steps)
+    b, _, d32 = run(p32, st, steps)
+    qs = sum(abs(q) for q in q_halves(a, N))          # charge scale: |Q| of each ball
+    field = np.linalg.norm(b - a) / np.linalg.norm(a)
+    dE = abs(d32["E_total"] - d64["E_total"]) / abs(d64["E_total"])
+    dQ = abs(d32["Q_total"] - d64["Q_total"]) / qs
+    print(f"lo={lo}: field L2 {field:.3e} dE/E {dE:.3e} |dQ|/|Q_ball| {dQ:.3e} gauss {d32['gauss_max']:.3e}")
+    assert dE <= 1e-6
+    assert dQ <= 1e-6
+    assert d32["gauss_max"] <= 1e-4
+    assert field <= 1e-4
+
+
+def test_f32c_ten_thousand_steps_charged_ball():
+    """One charged ball (net Q != 0, jellium offset from the projection), 48^3,
+    10^4 steps: |dQ|/Q and dE/E against FP64."""
+    N, steps = 48, 10000
+    L = 0.25 * (N - 1)
+    st = ball_state(N, L, link=1e-3)
+    p64 = G.make_params(N, G.MODE_GAUGED, L, 0.2, g=0.05)
+    p32 = G.make_params(N, G.MODE_GAUGED, L, 0.2, precision=G.PREC_F32C, lo_words=1, g=0.05)
+    a, e0, d64 = run(p64, st, steps)
+    b, _, d32 = run(p32, st, steps)
+    dE = abs(d32["E_total"] - d64["E_total"]) / abs(d64["E_total"])
+    dQ = abs(d32["Q_total"] - d64["Q_total"]) / abs(d64["Q_total"])
+    print(f"charged ball: Q {d64['Q_total']:.6e} dE/E {dE:.3e} |dQ|/Q {dQ:.3e} gauss {d32['gauss_max']:.3e}")
+    assert abs(d64["Q_total"]) > 1.0
+    assert dE <= 1e-6
+    assert dQ <= 1e-6
+    assert d32["gauss_max"] <= 1e-4
