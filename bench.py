@@ -7,17 +7,21 @@ theta noise 0.01, links at identity, E = 0 — synthetic data generated on the
 host from a fixed seed and resident in HBM before the timed region.
 
 A "step" is one full leapfrog step of the whole grid (one fused launch).
-value = site-updates/s over K timed steps (HIP events on the grid's stream).
+value = site-updates/s over K timed steps (HIP events on the grid's stream)
+of the headline precision: FP32C with bfloat16 compensation words by default;
+the FP64 path is timed on the same state after it ("fp64", "speedup_vs_fp64").
 roofline: algorithmic HBM bytes per site-update = every persistent state
 scalar read and written once at its storage width (gauged: 30 scalars; FP64
 8 B -> 480 B; FP32C 4 B hi + 4/2/0 B compensation word for lo_words 0/1/2 ->
 480/360/240 B), times N^3 sites per launch, over the average launch time;
 peak = 8 TB/s. traffic: measured HBM bytes per launch (rocprofv3 FETCH_SIZE
 and WRITE_SIZE scaled by a plain copy's counters), read from the counter
-summary profiles/hbm_traffic.txt when it holds a line for this kernel and size. cpu_baseline: the reference CPU binary built from
-source (oracle/_ref/scp_sim) on a bounded 64^3 gauged sample on this host.
+summary profiles/hbm_traffic.txt when it holds a line for this kernel and
+size. cpu_baseline: the reference CPU binary built from source
+(oracle/_ref/scp_sim) on a bounded 64^3 gauged sample on this host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--prec f64|f32c]
+                       [--lo-words 0|1|2] [--no-compare] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -115,15 +119,43 @@ def build_state(N, L, seed=1002):
     return seeds.seed_balls(N, L, 2, balls, theta_sigma=0.01, link_sigma=0.0, seed=seed)
 
 
+def run_one(G, N, L, st, prec, lo_words, steps, warmup):
+    """Upload the state, warm up, time `steps` launches with HIP events on the
+    grid's stream; returns the timing and the roofline figures."""
+    params = G.make_params(N, G.MODE_GAUGED, L, 0.2, g=0.05, lo_words=lo_words,
+                           precision=G.PREC_F64 if prec == "f64" else G.PREC_F32C)
+    log(f"{prec} lo_words={lo_words}: device bytes {G.grid_bytes(params) / 1e9:.1f} GB")
+    with G.Grid(params) as g:
+        g.upload(st)
+        if warmup > 0:
+            g.step(warmup)
+        w0 = time.time()
+        ms = g.time_steps(steps)
+        wall = time.time() - w0
+        d = g.diag()
+        kernel = g.step_kernel_name
+    ms_step = ms / steps
+    sites = float(N) ** 3
+    bps = bytes_per_site(prec, lo_words)
+    achieved = bps * sites / (ms_step * 1e-3) / 1e9
+    log(f"{prec}: {steps} steps {ms:.2f} ms (wall {wall * 1e3:.1f} ms) -> {ms_step:.3f} ms/step; "
+        f"E_total={d['E_total']:.9e} Q_total={d['Q_total']:.6e}")
+    return {"ms_per_step": ms_step, "value": sites / (ms_step * 1e-3) / 1e9, "kernel": kernel,
+            "bytes_per_site": bps, "achieved": achieved, "traffic": measured_traffic(kernel, N)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=512)
-    ap.add_argument("--prec", default="f64", choices=["f64", "f32c"])
+    ap.add_argument("--prec", default="f32c", choices=["f64", "f32c"],
+                    help="precision of the headline value (default: FP32C, the production form)")
+    ap.add_argument("--lo-words", type=int, default=1,
+                    help="FP32C compensation words: 0 FP32, 1 bfloat16 (default), 2 none (plain FP32)")
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--lo-words", type=int, default=0, help="FP32C compensation words: 0 f32, 1 bf16, 2 none")
     ap.add_argument("--copy-calib", type=float, default=0.0,
                     help="instead of the step: time a plain device copy of this many GiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -139,44 +171,34 @@ def main():
 
     N = a.n
     L = 0.25 * (N - 1)            # dx = 0.5
-    prec = G.PREC_F64 if a.prec == "f64" else G.PREC_F32C
-    params = G.make_params(N, G.MODE_GAUGED, L, 0.2, precision=prec, g=0.05, lo_words=a.lo_words)
     log(f"seeding {N}^3 gauged state on the host")
     t0 = time.time()
     st = build_state(N, L)
-    log(f"seeded in {time.time() - t0:.1f}s; device bytes {G.grid_bytes(params) / 1e9:.1f} GB")
-    g = G.Grid(params)
-    g.upload(st)
+    log(f"seeded in {time.time() - t0:.1f}s")
+    head = run_one(G, N, L, st, a.prec, a.lo_words, a.steps, a.warmup)
+    cmp64 = None
+    if a.prec == "f32c" and not a.no_compare:
+        cmp64 = run_one(G, N, L, st, "f64", 0, a.steps, a.warmup)
     del st
-    log(f"uploaded; warmup {a.warmup} steps")
-    if a.warmup > 0:
-        g.step(a.warmup)
-    log(f"timing {a.steps} steps")
-    w0 = time.time()
-    ms = g.time_steps(a.steps)
-    wall = time.time() - w0
-    d = g.diag()
-    log(f"timed {ms:.2f} ms (wall {wall * 1e3:.1f} ms); E_total={d['E_total']:.6e} Q_total={d['Q_total']:.6e}")
-    ms_step = ms / a.steps
-    sites = float(N) ** 3
-    value = sites / (ms_step * 1e-3) / 1e9
-    bps = bytes_per_site(a.prec, a.lo_words)
-    achieved = bps * sites / (ms_step * 1e-3) / 1e9
-    traffic = measured_traffic(g.step_kernel_name, N)
     cpu = None if a.no_cpu_baseline else cpu_baseline()
+    dtype = "f64" if a.prec == "f64" else f"f32c ({['f32', 'bf16', 'no'][a.lo_words]} compensation words)"
     out = {
         "metric": "Gsite-updates/s & %HBM peak, 512^3 gauged, FP32c vs FP64; p50/p99 job latency",
-        "value": value, "unit": "Gsite-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": a.prec, "data": "synthetic (seeded Gaussian Q-balls, generated on the host)",
+        "value": head["value"], "unit": "Gsite-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": dtype, "data": "synthetic (seeded Gaussian Q-balls, generated on the host)",
         "config": {"workload": f"gauged {N}^3 periodic leapfrog step, 64 Q-balls, dx=0.5 dt=0.1",
-                   "N": N, "mode": "gauged", "precision": a.prec, "g": 0.05},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": g.step_kernel_name, "bytes_per_site": bps},
+                   "N": N, "mode": "gauged", "precision": a.prec, "lo_words": a.lo_words, "g": 0.05},
+        "roofline": {"bound": "hbm", "achieved": head["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": head["achieved"] / HBM_PEAK_GBS, "traffic": head["traffic"],
+                     "kernel": head["kernel"], "bytes_per_site": head["bytes_per_site"]},
         "cpu_baseline": cpu,
     }
-    g.close()
+    if cmp64:
+        out["fp64"] = {"value": cmp64["value"], "ms_per_step": cmp64["ms_per_step"],
+                       "roofline_frac": cmp64["achieved"] / HBM_PEAK_GBS, "kernel": cmp64["kernel"],
+                       "traffic": cmp64["traffic"]}
+        out["speedup_vs_fp64"] = head["value"] / cmp64["value"]
     print(json.dumps(out), flush=True)
 
 

@@ -49,9 +49,13 @@ struct scp_grid {
 };
 
 // K_STEP of the complex and gauged steps runs the LDS-staged march unless the
-// caller asked for the direct-neighbour kernel
+// caller asked for the direct-neighbour kernel; the march addresses a field
+// with 32-bit byte offsets, so one field must stay below 4 GiB (N <= 812 in
+// FP64, N <= 1023 in FP32C); larger grids run the direct-neighbour kernel.
 static bool use_march(const scp_grid *g, int kind) {
-    return kind == K_STEP && g->p.step_kernel == 0 && (g->step_mode == 1 || g->step_mode == 2);
+    const uint64_t field_bytes = (uint64_t)g->kp.fs * (g->f32 ? 4u : 8u);
+    return kind == K_STEP && g->p.step_kernel == 0 && (g->step_mode == 1 || g->step_mode == 2) &&
+           field_bytes < (1ull << 32);
 }
 
 static thread_local char g_err[512];
@@ -115,6 +119,8 @@ static int check_params(const scp_params *p) {
         return set_err(-1, "precision=%d invalid", p->precision);
     if (p->precision == SCP_PREC_F32C && (p->lo_words < 0 || p->lo_words > 2))
         return set_err(-1, "lo_words=%d invalid", p->lo_words);
+    if (p->step_kernel < 0 || p->step_kernel > 1) return set_err(-1, "step_kernel=%d invalid", p->step_kernel);
+    if (p->reserved != 0) return set_err(-1, "reserved must be 0");
     if (p->bc_type == 1) return set_err(-3, "bc_type=1 (gradient pinned) is not implemented");
     if (p->bc_type != 0 && p->bc_type != 2) return set_err(-1, "bc_type=%d invalid", p->bc_type);
     if (!(p->dx > 0) || !(p->dt > 0)) return set_err(-1, "dx/dt must be positive");
@@ -230,9 +236,9 @@ extern "C" int scp_grid_create(const scp_params *p, scp_grid **out) {
     }
     if (use_march(g, K_STEP)) {
         if (g->f32)
-            snprintf(g->kname, sizeof g->kname, "k_march<scp::S32<%d>, %d, false>", g->lo_kind, g->step_mode);
+            snprintf(g->kname, sizeof g->kname, "k_march<scp::S32<%d>, 1, %d, false>", g->lo_kind, g->step_mode);
         else
-            snprintf(g->kname, sizeof g->kname, "k_march<scp::S64, %d, false>", g->step_mode);
+            snprintf(g->kname, sizeof g->kname, "k_march<scp::S64, 1, %d, false>", g->step_mode);
     } else if (g->f32)
         snprintf(g->kname, sizeof g->kname, "k_step32<%d, 1, false, %d>", g->step_mode, g->lo_kind);
     else

@@ -74,6 +74,9 @@ int launch_step_f32c(int mode, int kind, bool emit, int lo_kind, const KParams &
 int launch_march_f64(int mode, bool emit, const KParams &kp, const Bufs<double> &b, int DI, hipStream_t s);
 int launch_march_f32c(int mode, bool emit, int lo_kind, const KParams &kp, const Bufs32 &b, int DI,
                       hipStream_t s);
+// the same with the tile width chosen: kw waves (64 sites each) per tile row
+int launch_march_f32c_kw(int kw, int mode, bool emit, int lo_kind, const KParams &kp, const Bufs32 &b, int DI,
+                         hipStream_t s);
 int launch_real_aux(const KParams &kp, const double *pin, double *aux, hipStream_t s);
 int launch_real_aux_f32(const KParams &kp, const float *pin, double *aux, hipStream_t s);
 

@@ -5,7 +5,8 @@
 // expression trees of scp_sim.c:931-994, 1053-1263, 1583-1716); what changes is
 // where the neighbour values come from:
 //
-//   * a block is a 64(k) x 4(j) tile that marches along i over DI planes;
+//   * a block is a 64*KW(k) x 4(j) tile that marches along i over DI planes
+//     (KW = 1 in production: two FP32C blocks or one FP64 block per CU);
 //   * the own column of planes i-1, i, i+1 lives in registers (the march queue);
 //   * plane i of the tile plus a one-cell halo ring (rows j0-1, j0+4, columns
 //     k0-1, k0+64, corners included) is staged in LDS, so the j/k neighbours
@@ -19,8 +20,8 @@
 //     iteration ahead, so HBM latency is covered by a whole plane of compute.
 //
 // Global traffic per site and step: the state read and written once plus the
-// halo ring (2*64 + 2*6 cells per 256 sites, read from L2 / MALL: the adjacent
-// tiles stream the same planes at the same time).
+// halo ring (2*64*KW + 2*6 cells per 256*KW sites, read from L2 / MALL: the
+// adjacent tiles stream the same planes at the same time).
 //
 // Ragged tiles (N not a multiple of 64 or 4) keep the scheme: the right halo
 // column sits at column W+1 and the bottom halo row at row H+1, where W x H is
@@ -36,11 +37,18 @@ namespace scp {
 
 constexpr int MTJ = 4;                 // j rows per block
 constexpr int MR = MTJ + 2;            // staged rows
-constexpr int MC = TK + 2;             // staged columns
-constexpr int MPL = MR * MC;           // staged cells per field
-constexpr int MNT = TK * MTJ;          // threads per block
-constexpr int MHC = 2 * TK + 2 * MR;   // halo cells per plane
 constexpr int MNBUF = 3;
+
+// Tile geometry: KW waves along k per row (a row of 64*KW sites; wider rows
+// give the HBM longer contiguous runs per field), MTJ rows.
+template <int KW>
+struct MG {
+    static constexpr int TKM = TK * KW;        // sites along k
+    static constexpr int MC = TKM + 2;         // staged columns
+    static constexpr int MPL = MR * MC;        // staged cells per field
+    static constexpr int MNT = TKM * MTJ;      // threads per block
+    static constexpr int NW = MNT / 64;        // waves per block
+};
 
 struct MTile {
     int N, k0, j0, W, H, i0, i1, j, k, own;
@@ -49,23 +57,25 @@ struct MTile {
 
 __device__ inline int wrapc(int x, int N) { return x < 0 ? x + N : (x >= N ? x - N : x); }
 
+template <int KW>
 __device__ inline MTile mtile(int N, int DI) {
+    using G = MG<KW>;
     MTile t;
     t.N = N;
-    const uint32_t ntk = (N + TK - 1) / TK, ntj = (N + MTJ - 1) / MTJ;
+    const uint32_t ntk = (N + G::TKM - 1) / G::TKM, ntj = (N + MTJ - 1) / MTJ;
     uint32_t lb = logical_block(blockIdx.x, gridDim.x);
     const uint32_t tk = lb % ntk; lb /= ntk;
     const uint32_t tj = lb % ntj; lb /= ntj;
-    t.k0 = tk * TK;
+    t.k0 = tk * G::TKM;
     t.j0 = tj * MTJ;
-    t.W = min(TK, N - t.k0);
+    t.W = min(G::TKM, N - t.k0);
     t.H = min(MTJ, N - t.j0);
     t.i0 = lb * DI;
     t.i1 = min(N, t.i0 + DI);
     t.k = t.k0 + threadIdx.x;
     t.j = t.j0 + threadIdx.y;
     t.valid = (int)threadIdx.x < t.W && (int)threadIdx.y < t.H;
-    t.own = (threadIdx.y + 1) * MC + threadIdx.x + 1;
+    t.own = (threadIdx.y + 1) * G::MC + threadIdx.x + 1;
     return t;
 }
 
@@ -91,29 +101,31 @@ __device__ inline void sto_nt(V *base, uint32_t e, V v) {
 }
 
 // The halo ring of one plane, spread over the block.
-//   rows (j0-1 and j0+H): (field, side) pairs p = wave + 4q, lanes along k —
-//     a wave-uniform field and a coalesced 64-element row per load;
+//   rows (j0-1 and j0+H): 64-site segments (field, side, half), segment
+//     s = wave + NW*q, lanes along k — a wave-uniform field and a coalesced
+//     row segment per load;
 //   columns (k0-1 and k0+W, rows j0-1 .. j0+H): one (field, side, row) cell
 //     per thread for the first NF*2*MR threads.
-template <int NF>
+template <int NF, int KW>
 struct Halo {
-    static constexpr int RQ = (2 * NF + 3) / 4;   // row pairs per wave
-    int wave;
-    uint32_t rtop, rbot;      // in-plane element offsets of this lane's top/bottom halo cell
-    int lrow_top, lrow_bot;   // LDS offsets (field 0) of this lane's halo row cells
-    bool rok;                 // lane inside the tile's valid width
+    using G = MG<KW>;
+    static constexpr int NSEG = NF * 2 * KW;              // row segments per plane
+    static constexpr int RQ = (NSEG + G::NW - 1) / G::NW; // row segments per wave
+    int wave, lane, W;
+    uint32_t rtop, rbot;      // in-plane element offsets of this lane's top/bottom halo cell (half 0)
+    int lrow_top, lrow_bot;   // LDS offsets (field 0, half 0) of this lane's halo row cells
     int cf;                   // column cell: field, or -1
     uint32_t cg;              // column cell: in-plane element offset
     int cl;                   // column cell: LDS offset
     __device__ void init(const MTile &t) {
-        const int N = t.N, tx = threadIdx.x, tid = threadIdx.y * TK + threadIdx.x;
-        wave = __builtin_amdgcn_readfirstlane(threadIdx.y);
-        rok = tx < t.W;
-        const int gk = rok ? t.k0 + tx : t.k0;
-        rtop = (uint32_t)wrapc(t.j0 - 1, N) * N + gk;
-        rbot = (uint32_t)wrapc(t.j0 + t.H, N) * N + gk;
-        lrow_top = tx + 1;
-        lrow_bot = (t.H + 1) * MC + tx + 1;
+        const int N = t.N, tid = threadIdx.y * G::TKM + threadIdx.x;
+        wave = __builtin_amdgcn_readfirstlane(tid / 64);
+        lane = tid & 63;
+        W = t.W;
+        rtop = (uint32_t)wrapc(t.j0 - 1, N) * N + t.k0 + lane;
+        rbot = (uint32_t)wrapc(t.j0 + t.H, N) * N + t.k0 + lane;
+        lrow_top = lane + 1;
+        lrow_bot = (t.H + 1) * G::MC + lane + 1;
         cf = -1; cg = 0; cl = 0;
         if (tid < NF * 2 * MR) {
             const int f = tid / (2 * MR), rem = tid % (2 * MR), side = rem / MR, r = rem % MR;
@@ -122,22 +134,34 @@ struct Halo {
                 const int gj = wrapc(t.j0 - 1 + r, N);
                 const int ck = side ? wrapc(t.k0 + t.W, N) : wrapc(t.k0 - 1, N);
                 cg = (uint32_t)gj * N + ck;
-                cl = f * MPL + r * MC + (side ? t.W + 1 : 0);
+                cl = f * G::MPL + r * G::MC + (side ? t.W + 1 : 0);
             }
         }
+    }
+    // segment q of this wave: field, global offset, LDS offset; false if idle
+    __device__ bool seg(int q, int &f, uint32_t &g, int &l) const {
+        const int sg = wave + G::NW * q;
+        if (sg >= NSEG) return false;
+        f = sg / (2 * KW);
+        const int rem = sg % (2 * KW), side = rem / KW, half = rem % KW;
+        if (half * 64 + lane >= W) return false;
+        g = (side ? rbot : rtop) + half * 64;
+        l = (side ? lrow_bot : lrow_top) + half * 64;
+        return true;
     }
 };
 
 // own column of a staged plane, indexable by field
-template <typename R>
+template <typename R, int KW>
 struct LdsCol {
     const R *p;
-    __device__ R operator[](int f) const { return p[f * MPL]; }
+    __device__ R operator[](int f) const { return p[f * MG<KW>::MPL]; }
 };
 
 // neighbour provider over the staged planes (see GlobalNB for the contract)
-template <typename R>
+template <typename R, int KW>
 struct LdsNB {
+    static constexpr int MPL = MG<KW>::MPL, MC = MG<KW>::MC;
     const R *c;   // plane i buffer, at the own cell
     const R *n;   // plane i+1 buffer, at the own cell
     __device__ R face(int f, int d, int sg) const { return c[f * MPL + (d == 1 ? sg * MC : sg)]; }
@@ -293,19 +317,20 @@ struct S32 {
 // the kernel
 // ---------------------------------------------------------------------------
 
-template <class S, int MODE, bool EMIT>
-__global__ void __launch_bounds__(MNT, S::MINB)
+template <class S, int KW, int MODE, bool EMIT>
+__global__ void __launch_bounds__(MG<KW>::MNT, KW == 1 ? S::MINB : 1)
 k_march(KParams kp, typename S::B b, int DI) {
     using T = typename S::T;
     constexpr int NF = MODE == 2 ? 15 : 12;   // staged fields (matter + links)
     constexpr int NM = 12;                    // matter fields (the i-1 plane is matter only)
-    constexpr int RQ = Halo<NF>::RQ;
+    constexpr int MPL = MG<KW>::MPL;
+    constexpr int RQ = Halo<NF, KW>::RQ;
     __shared__ T lds[MNBUF * NF * MPL];
 
     const int N = kp.N;
-    const MTile t = mtile(N, DI);
+    const MTile t = mtile<KW>(N, DI);
     if (t.i0 >= N) return;                    // whole block: uniform
-    Halo<NF> hs;
+    Halo<NF, KW> hs;
     hs.init(t);
     const uint32_t fs = kp.fs, NN = (uint32_t)N * (uint32_t)N;
     const T *__restrict__ pin = S::pin(b);
@@ -315,9 +340,9 @@ k_march(KParams kp, typename S::B b, int DI) {
     auto halo_rows = [&](uint32_t pe, T *hr) {
 #pragma unroll
         for (int q = 0; q < RQ; q++) {
-            const int pr = hs.wave + 4 * q;
-            hr[q] = T(0);
-            if (pr < 2 * NF && hs.rok) hr[q] = ld(pr >> 1, pe + ((pr & 1) ? hs.rbot : hs.rtop));
+            int f, l;
+            uint32_t g;
+            hr[q] = hs.seg(q, f, g, l) ? ld(f, pe + g) : T(0);
         }
     };
     auto halo_col = [&](uint32_t pe) -> T {
@@ -330,8 +355,9 @@ k_march(KParams kp, typename S::B b, int DI) {
         }
 #pragma unroll
         for (int q = 0; q < RQ; q++) {
-            const int pr = hs.wave + 4 * q;
-            if (pr < 2 * NF && hs.rok) buf[(pr >> 1) * MPL + ((pr & 1) ? hs.lrow_bot : hs.lrow_top)] = hr[q];
+            int f, l;
+            uint32_t g;
+            if (hs.seg(q, f, g, l)) buf[f * MPL + l] = hr[q];
         }
         if (hs.cf >= 0) buf[hs.cl] = hc;
     };
@@ -398,7 +424,7 @@ k_march(KParams kp, typename S::B b, int DI) {
 
         // own column of planes i and i+1: registers (REGQ) or read back from the staged tiles
         auto compute = [&](const auto &c0, const auto &c1) {
-            const LdsNB<T> nb{lds + bb * NF * MPL + t.own, lds + nbuf * NF * MPL + t.own};
+            const LdsNB<T, KW> nb{lds + bb * NF * MPL + t.own, lds + nbuf * NF * MPL + t.own};
             T acc[NF];
             if constexpr (MODE == 2)
                 forces_gauge(kp, c0, c1, cm, nb, cy, acc);
@@ -430,7 +456,7 @@ k_march(KParams kp, typename S::B b, int DI) {
 #pragma unroll
             for (int f = 0; f < NF; f++) { cq[f] = cp[f]; cp[f] = cpp[f]; }
         } else {
-            const LdsCol<T> c0{lds + bb * NF * MPL + t.own}, c1{lds + nbuf * NF * MPL + t.own};
+            const LdsCol<T, KW> c0{lds + bb * NF * MPL + t.own}, c1{lds + nbuf * NF * MPL + t.own};
             if (t.valid) compute(c0, c1);
 #pragma unroll
             for (int f = 0; f < NM; f++) cm[f] = c0[f];
@@ -444,22 +470,24 @@ k_march(KParams kp, typename S::B b, int DI) {
     }
 }
 
-// planes per block: deep enough to amortise the 3-plane prologue, shallow
-// enough for >= ~8 blocks per CU
-inline int march_di(int N) {
-    const uint32_t tiles = ((N + TK - 1) / TK) * ((N + MTJ - 1) / MTJ);
+// planes per block: deep enough to amortise the 3-plane prologue (3/DI extra
+// plane reads), shallow enough for >= ~16 blocks per CU (measured at 512^3:
+// DI = 128, 4096 blocks, within 1 % of the best over DI = 16..128)
+inline int march_di(int N, int KW) {
+    const uint32_t tiles = ((N + TK * KW - 1) / (TK * KW)) * ((N + MTJ - 1) / MTJ);
     int di = N;
-    while (di > 16 && (uint64_t)tiles * ((N + di - 1) / di) < 8192) di /= 2;
+    while (di > 16 && (uint64_t)tiles * ((N + di - 1) / di) < 4096u / KW) di /= 2;
     return di;
 }
 
-template <class S, int MODE>
+template <class S, int KW, int MODE>
 inline int launch_march_t(bool emit, const KParams &kp, const typename S::B &b, int DI, hipStream_t s) {
-    if (DI <= 0) DI = march_di(kp.N);
-    const uint32_t ntk = (kp.N + TK - 1) / TK, ntj = (kp.N + MTJ - 1) / MTJ, nseg = (kp.N + DI - 1) / DI;
-    const dim3 grid(ntk * ntj * nseg), block(TK, MTJ);
-    if (emit) hipLaunchKernelGGL((k_march<S, MODE, true>), grid, block, 0, s, kp, b, DI);
-    else hipLaunchKernelGGL((k_march<S, MODE, false>), grid, block, 0, s, kp, b, DI);
+    using G = MG<KW>;
+    if (DI <= 0) DI = march_di(kp.N, KW);
+    const uint32_t ntk = (kp.N + G::TKM - 1) / G::TKM, ntj = (kp.N + MTJ - 1) / MTJ, nseg = (kp.N + DI - 1) / DI;
+    const dim3 grid(ntk * ntj * nseg), block(G::TKM, MTJ);
+    if (emit) hipLaunchKernelGGL((k_march<S, KW, MODE, true>), grid, block, 0, s, kp, b, DI);
+    else hipLaunchKernelGGL((k_march<S, KW, MODE, false>), grid, block, 0, s, kp, b, DI);
     return (int)hipGetLastError();
 }
 

@@ -1,20 +1,35 @@
 // scp_march32.hip — FP32C instances of the LDS-staged march (built with FMA
 // contraction: the compensated sums form their products with explicit fmaf).
+// Default tile: one wave per row (64 x 4 sites, 71 KB of LDS, two blocks of
+// four waves per CU: while one block waits at its plane barrier the other
+// computes). Two waves per row (128 x 4, one block of eight waves per CU)
+// measured 20 % slower at 512^3 and is kept for the tuning harness.
 #include "scp_march.h"
 
 namespace scp {
 
+template <int LOK, int KW>
+static int launch_lo(int mode, bool emit, const KParams &kp, const Bufs32 &b, int DI, hipStream_t s) {
+    return mode == 2 ? launch_march_t<S32<LOK>, KW, 2>(emit, kp, b, DI, s)
+                     : launch_march_t<S32<LOK>, KW, 1>(emit, kp, b, DI, s);
+}
+
+template <int KW>
+static int launch_kw(int mode, bool emit, int lo_kind, const KParams &kp, const Bufs32 &b, int DI, hipStream_t s) {
+    if (lo_kind == LO_F32) return launch_lo<LO_F32, KW>(mode, emit, kp, b, DI, s);
+    if (lo_kind == LO_BF16) return launch_lo<LO_BF16, KW>(mode, emit, kp, b, DI, s);
+    return launch_lo<LO_NONE, KW>(mode, emit, kp, b, DI, s);
+}
+
+int launch_march_f32c_kw(int kw, int mode, bool emit, int lo_kind, const KParams &kp, const Bufs32 &b, int DI,
+                         hipStream_t s) {
+    if (mode != 1 && mode != 2) return (int)hipErrorInvalidValue;
+    return kw == 1 ? launch_kw<1>(mode, emit, lo_kind, kp, b, DI, s) : launch_kw<2>(mode, emit, lo_kind, kp, b, DI, s);
+}
+
 int launch_march_f32c(int mode, bool emit, int lo_kind, const KParams &kp, const Bufs32 &b, int DI,
                       hipStream_t s) {
-    if (mode != 1 && mode != 2) return (int)hipErrorInvalidValue;
-    if (lo_kind == LO_F32)
-        return mode == 2 ? launch_march_t<S32<LO_F32>, 2>(emit, kp, b, DI, s)
-                         : launch_march_t<S32<LO_F32>, 1>(emit, kp, b, DI, s);
-    if (lo_kind == LO_BF16)
-        return mode == 2 ? launch_march_t<S32<LO_BF16>, 2>(emit, kp, b, DI, s)
-                         : launch_march_t<S32<LO_BF16>, 1>(emit, kp, b, DI, s);
-    return mode == 2 ? launch_march_t<S32<LO_NONE>, 2>(emit, kp, b, DI, s)
-                     : launch_march_t<S32<LO_NONE>, 1>(emit, kp, b, DI, s);
+    return launch_march_f32c_kw(1, mode, emit, lo_kind, kp, b, DI, s);
 }
 
 }  // namespace scp

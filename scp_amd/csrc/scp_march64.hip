@@ -1,12 +1,13 @@
 // scp_march64.hip — FP64 instances of the LDS-staged march (no FMA contraction:
-// the FP64 step rounds exactly like the reference CPU path).
+// the FP64 step rounds exactly like the reference CPU path). One wave per row
+// (64 x 4 tiles): three staged FP64 planes of 15 fields use 142 KB of LDS.
 #include "scp_march.h"
 
 namespace scp {
 
 int launch_march_f64(int mode, bool emit, const KParams &kp, const Bufs<double> &b, int DI, hipStream_t s) {
-    if (mode == 2) return launch_march_t<S64, 2>(emit, kp, b, DI, s);
-    if (mode == 1) return launch_march_t<S64, 1>(emit, kp, b, DI, s);
+    if (mode == 2) return launch_march_t<S64, 1, 2>(emit, kp, b, DI, s);
+    if (mode == 1) return launch_march_t<S64, 1, 1>(emit, kp, b, DI, s);
     return (int)hipErrorInvalidValue;
 }
 
