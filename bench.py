@@ -144,6 +144,62 @@ def run_one(G, N, L, st, prec, lo_words, steps, warmup):
             "bytes_per_site": bps, "achieved": achieved, "traffic": measured_traffic(kernel, N)}
 
 
+def sweep_bench(a):
+    """Batched parameter sweep: independent gauged grids of edge drawn from
+    {32,48,64,80,96} (seeded), one Q-ball each (amp U[0.6,0.9], width U[2,4]
+    cells, omega U[0.6,0.95]), FP32C, advanced together by one batch (one
+    launch per step over all tiles, chains replayed from HIP graphs, depth and
+    launch form autotuned). Compares with stepping each grid on its own."""
+    from scp_amd import grid as G
+    from scp_amd import seeds
+    rng = np.random.default_rng(a.seed)
+    sizes = rng.choice([32, 48, 64, 80, 96], size=a.grids)
+    grids, sites = [], 0
+    t0 = time.time()
+    for i, N in enumerate(sizes):
+        N = int(N)
+        L = 0.25 * (N - 1)
+        dx = 0.5
+        b = dict(x0=float(rng.uniform(-0.2, 0.2) * L), y0=float(rng.uniform(-0.2, 0.2) * L),
+                 z0=float(rng.uniform(-0.2, 0.2) * L), amp=float(rng.uniform(0.6, 0.9)),
+                 sigma=float(rng.uniform(2.0, 4.0)) * dx, omega=float(rng.uniform(0.6, 0.95)),
+                 phase=float(rng.uniform(0, 2 * np.pi)), winding=1)
+        st = seeds.seed_balls(N, L, 2, [b], theta_sigma=0.01, link_sigma=0.0, seed=a.seed + i)
+        g = G.Grid(G.make_params(N, G.MODE_GAUGED, L, 0.2, g=0.05, precision=G.PREC_F32C, lo_words=a.lo_words))
+        g.upload(st)
+        grids.append(g)
+        sites += N ** 3
+    log(f"{len(grids)} grids, {sites / 1e6:.1f} M sites, set up in {time.time() - t0:.1f}s")
+    with G.Batch(grids) as b:
+        taken = b.tune()
+        cfg = b.config
+        log(f"autotune: {cfg} ({taken} steps)")
+        b.step(a.steps)                      # captures the chain graph
+        ms = b.time_steps(a.steps)           # replay
+    ms_step = ms / a.steps
+    # the same grids stepped one at a time (each its own launch per step)
+    t1 = time.time()
+    ms_alone = 0.0
+    for g in grids:
+        ms_alone += g.time_steps(a.steps)
+    ms_alone_step = ms_alone / a.steps
+    for g in grids:
+        g.close()
+    out = {"metric": "aggregate Gsite-updates/s, batched sweep of independent gauged grids",
+           "value": sites / (ms_step * 1e-3) / 1e9, "unit": "Gsite-updates/s", "n_gpus": 1,
+           "steps": a.steps, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": f"f32c ({['f32', 'bf16', 'no'][a.lo_words]} compensation words)",
+           "data": "synthetic (seeded single Q-ball per grid)",
+           "config": {"workload": f"{len(grids)} gauged grids, edges from {{32,48,64,80,96}}, one Q-ball each",
+                      "grids": len(grids), "sites": sites, "seed": a.seed, "tuned": cfg},
+           "roofline": {"bound": "hbm", "achieved": bytes_per_site("f32c", a.lo_words) * sites / (ms_step * 1e-3) / 1e9,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": bytes_per_site("f32c", a.lo_words) * sites / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+           "unbatched": {"value": sites / (ms_alone_step * 1e-3) / 1e9, "ms_per_step": ms_alone_step},
+           "batch_speedup": ms_alone_step / ms_step}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +212,10 @@ def main():
                     help="FP32C compensation words: 0 FP32, 1 bfloat16 (default), 2 none (plain FP32)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="step", choices=["step", "sweep"],
+                    help="step: the 512^3 gauged step (default); sweep: batched independent grids")
+    ap.add_argument("--grids", type=int, default=256, help="sweep: number of grids")
+    ap.add_argument("--seed", type=int, default=3, help="sweep: seed of sizes and balls")
     ap.add_argument("--copy-calib", type=float, default=0.0,
                     help="instead of the step: time a plain device copy of this many GiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -167,6 +227,9 @@ def main():
                                    "the march axis with one-plane halo exchange over RCCL) is not built",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.workload == "sweep":
+        sweep_bench(a)
+        return
     from scp_amd import grid as G
 
     N = a.n

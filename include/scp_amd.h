@@ -114,6 +114,36 @@ int  scp_grid_time_steps(scp_grid *g, int nsteps, float *ms_total);
 /* Name of the step kernel as rocprof reports it (for profile matching). */
 const char *scp_grid_step_kernel_name(const scp_grid *g);
 
+/* Batches: many grids advanced together. No single-call counterpart in the
+ * reference, where every job is its own scp_sim process started by the Go
+ * runner (LocalExecutor.executeRun, sfa/runner/local.go:239-330) with at most
+ * max(1, nproc/16) running at once (localMaxConcurrent, local.go:51-64). A
+ * batch holds grids of one class (kernel mode, precision, lo_words) that run
+ * the LDS-staged march; one launch per step covers every member's tiles, and
+ * chains of steps replay captured HIP graphs. Grids stay owned by the caller;
+ * a grid belongs to at most one batch and leaves it when destroyed. Results
+ * are bitwise those of scp_grid_step on each grid alone. */
+typedef struct scp_batch scp_batch;
+
+int  scp_batch_create(scp_batch **out);
+int  scp_batch_destroy(scp_batch *b);
+int  scp_batch_add(scp_batch *b, scp_grid *g);
+int  scp_batch_remove(scp_batch *b, scp_grid *g);
+int  scp_batch_size(const scp_batch *b);
+/* Member i (order of addition; removals close the gaps) advances nsteps[i] >= 0
+ * steps; n must equal scp_batch_size. */
+int  scp_batch_step(scp_batch *b, const int *nsteps, int n);
+int  scp_batch_step_all(scp_batch *b, int nsteps);
+/* scp_batch_step_all timed with HIP events around the graph replay. */
+int  scp_batch_time_steps(scp_batch *b, int nsteps, float *ms_total);
+/* Autotuner: planes per block (16, 32, 64, 128) x launch form (one block per
+ * tile, or persistent resident blocks over contiguous tile ranges), each
+ * timed on the batch's own steps; every member advances *steps_taken steps. */
+int  scp_batch_tune(scp_batch *b, int *steps_taken);
+/* di = 0: per-grid default depth; persistent: 0 / 1. */
+int  scp_batch_set_config(scp_batch *b, int di, int persistent);
+int  scp_batch_config(const scp_batch *b, int *di, int *persistent, int *tuned);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,8 +6,7 @@
 //   xo,vo positions and velocities of the last completed step (FP64), written
 //         by the last launch of every scp_grid_step call and by upload
 // See scp_step.hip for the step schedule.
-#include "scp_amd.h"
-#include "scp_kernels.h"
+#include "scp_grid.h"
 
 #include <hip/hip_runtime.h>
 
@@ -19,40 +18,14 @@
 
 using namespace scp;
 
-struct scp_grid {
-    scp_params p;
-    KParams kp;
-    int mode;        // storage mode (0/1/2), number of columns follows it
-    int step_mode;   // kernel mode: 2 only when gauged with g != 0
-    int NP;
-    hipStream_t stream;
-    int device;
-    int f32;          // SCP_PREC_F32C
-    int lo_kind;      // LoKind of the compensation words
-    double *P[2];     // FP64 state
-    double *W;
-    float *P32[2];    // FP32C state: hi words ...
-    float *W32;
-    void *Plo[2];     // ... and compensation words
-    void *Wlo;
-    double *xo, *vo;
-    double *aux;     // real-mode Cosserat/hardening fields (6), when needed
-    double *acc;     // forces output (lazily allocated)
-    double *partials;
-    double *dout;
-    unsigned long long *argmin;
-    int cur;
-    int nblocks_diag;
-    int64_t steps;
-    hipEvent_t ev0, ev1;
-    char kname[96];
-};
+namespace scp_internal {
+
 
 // K_STEP of the complex and gauged steps runs the LDS-staged march unless the
 // caller asked for the direct-neighbour kernel; the march addresses a field
 // with 32-bit byte offsets, so one field must stay below 4 GiB (N <= 812 in
 // FP64, N <= 1023 in FP32C); larger grids run the direct-neighbour kernel.
-static bool use_march(const scp_grid *g, int kind) {
+bool use_march(const scp_grid *g, int kind) {
     const uint64_t field_bytes = (uint64_t)g->kp.fs * (g->f32 ? 4u : 8u);
     return kind == K_STEP && g->p.step_kernel == 0 && (g->step_mode == 1 || g->step_mode == 2) &&
            field_bytes < (1ull << 32);
@@ -60,7 +33,7 @@ static bool use_march(const scp_grid *g, int kind) {
 
 static thread_local char g_err[512];
 
-static int set_err(int code, const char *fmt, ...) {
+int set_err(int code, const char *fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
     vsnprintf(g_err, sizeof g_err, fmt, ap);
@@ -68,18 +41,14 @@ static int set_err(int code, const char *fmt, ...) {
     return code;
 }
 
-#define HIPCHK(x)                                                                    \
-    do {                                                                             \
-        hipError_t e_ = (x);                                                         \
-        if (e_ != hipSuccess)                                                        \
-            return set_err(-2, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_),  \
-                           __FILE__, __LINE__);                                      \
-    } while (0)
-
-static int kerr(int rc, const char *what) {
+int kerr(int rc, const char *what) {
     if (rc != 0) return set_err(-2, "%s launch failed: %s", what, hipGetErrorString((hipError_t)rc));
     return 0;
 }
+
+}  // namespace scp_internal
+
+using namespace scp_internal;
 
 extern "C" int scp_abi_version(void) { return SCP_ABI_VERSION; }
 
@@ -249,12 +218,13 @@ extern "C" int scp_grid_create(const scp_params *p, scp_grid **out) {
 
 extern "C" int scp_grid_destroy(scp_grid *g) {
     if (!g) return 0;
+    if (g->batch) scp_batch_remove(g->batch, g);   // a destroyed grid leaves its batch
     hipStreamSynchronize(g->stream);
     free_grid(g);
     return 0;
 }
 
-static Bufs<double> bufs_for(scp_grid *g, int kind) {
+Bufs<double> scp_internal::bufs_for(scp_grid *g, int kind) {
     Bufs<double> b;
     memset(&b, 0, sizeof b);
     b.xo = g->xo;
@@ -273,7 +243,7 @@ static int run_aux(scp_grid *g, const double *src) {
     return kerr(launch_real_aux(g->kp, src, g->aux, g->stream), "real aux");
 }
 
-static Bufs32 bufs32_for(scp_grid *g, int kind) {
+Bufs32 scp_internal::bufs32_for(scp_grid *g, int kind) {
     Bufs32 b;
     memset(&b, 0, sizeof b);
     b.xo = g->xo; b.vo = g->vo; b.w = g->W32; b.wlo = g->Wlo; b.aux = g->aux; b.acc = g->acc;

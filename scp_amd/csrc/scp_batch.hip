@@ -1,0 +1,329 @@
+// scp_batch.hip — many grids advanced together (include/scp_amd.h, scp_batch_*).
+//
+// A batch holds grids of one class (kernel mode, storage, compensation words)
+// that the LDS-staged march can run. One launch per step covers every tile of
+// every member: a device tile table lists (grid, local tile) pairs and a
+// device descriptor per grid carries its constants, buffers, P parity and the
+// number of launches it takes part in, so members of different sizes and
+// different step counts share launches. A chain of L steps is captured once
+// as a HIP graph and replayed (only the descriptors change between chains,
+// copied in before the replay). The segment depth DI and the launch form
+// (one block per tile, or a persistent grid of resident blocks walking
+// contiguous tile ranges) are picked by scp_batch_tune from timed steps of the
+// batch itself; results do not depend on either (the per-site arithmetic and
+// the carried link quantities are the same for every segmentation).
+#include "scp_grid.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <vector>
+
+using namespace scp;
+using namespace scp_internal;
+
+struct scp_batch {
+    std::vector<scp_grid *> members;
+    int step_mode = -1, f32 = -1, lo_kind = -1;   // class, fixed by the first member
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int ncu = 256;
+    // tables (device) and the pinned host copy of the descriptors
+    void *d_grids = nullptr, *h_grids = nullptr;
+    size_t grids_bytes = 0;
+    BatchTile *d_tiles = nullptr;
+    size_t tiles_cap = 0;
+    uint32_t ntiles = 0;
+    bool stale = true;          // member set or configuration changed since the tables were built
+    int DI = 0;                 // 0: per-grid default (march_di)
+    int persistent = 0;
+    bool tuned = false;
+    std::map<int, hipGraphExec_t> graphs;   // chain length -> replayable chain (valid for the current tables)
+};
+
+static void drop_graphs(scp_batch *b) {
+    for (auto &kv : b->graphs) hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
+}
+
+static size_t desc_size(const scp_batch *b) {
+    return b->f32 ? sizeof(BatchGrid<Bufs32>) : sizeof(BatchGrid<Bufs<double>>);
+}
+
+static int grid_di(const scp_batch *b, const scp_grid *g) {
+    int di = b->DI > 0 ? b->DI : march_di(g->p.N, 1);
+    return std::max(1, std::min(di, g->p.N));
+}
+
+// rebuild descriptor and tile tables for the current members (device copies)
+static int build_tables(scp_batch *b) {
+    const size_t n = b->members.size();
+    std::vector<BatchTile> tiles;
+    for (size_t m = 0; m < n; m++) {
+        const scp_grid *g = b->members[m];
+        const int N = g->p.N, di = grid_di(b, g);
+        const uint32_t nt = march_tiles_k(N) * march_tiles_j(N) * (uint32_t)((N + di - 1) / di);
+        for (uint32_t t = 0; t < nt; t++) tiles.push_back({(uint32_t)m, t});
+    }
+    drop_graphs(b);
+    const size_t gb = std::max<size_t>(1, n) * desc_size(b);
+    if (gb > b->grids_bytes) {
+        hipFree(b->d_grids);
+        hipHostFree(b->h_grids);
+        b->d_grids = nullptr;
+        b->h_grids = nullptr;
+        b->grids_bytes = 0;
+        HIPCHK(hipMalloc(&b->d_grids, gb));
+        HIPCHK(hipHostMalloc(&b->h_grids, gb, hipHostMallocDefault));
+        b->grids_bytes = gb;
+    }
+    if (tiles.size() > b->tiles_cap) {
+        hipFree(b->d_tiles);
+        b->d_tiles = nullptr;
+        b->tiles_cap = 0;
+        HIPCHK(hipMalloc(&b->d_tiles, std::max<size_t>(1, tiles.size()) * sizeof(BatchTile)));
+        b->tiles_cap = tiles.size();
+    }
+    if (!tiles.empty())
+        HIPCHK(hipMemcpy(b->d_tiles, tiles.data(), tiles.size() * sizeof(BatchTile), hipMemcpyHostToDevice));
+    b->ntiles = (uint32_t)tiles.size();
+    b->stale = false;
+    return 0;
+}
+
+template <class B>
+static void fill_desc(BatchGrid<B> *d, const scp_batch *b, scp_grid *g, int nq, B (*bufs)(scp_grid *, int)) {
+    memset(d, 0, sizeof *d);
+    d->kp = g->kp;
+    const int cur = g->cur;
+    for (int par = 0; par < 2; par++) {
+        g->cur = par;
+        d->buf[par] = bufs(g, K_STEP);
+    }
+    g->cur = cur;
+    d->DI = grid_di(b, g);
+    d->cur0 = cur;
+    d->nq = nq;
+    d->ntk = march_tiles_k(g->p.N);
+    d->ntj = march_tiles_j(g->p.N);
+}
+
+static int launch_one(scp_batch *b, int q) {
+    const uint32_t nblocks = b->persistent ? std::min<uint32_t>(b->ntiles, (uint32_t)(b->f32 ? 2 : 1) * b->ncu)
+                                           : b->ntiles;
+    const uint32_t per = (b->ntiles + nblocks - 1) / nblocks;
+    const int rc = b->f32 ? launch_march_batch_f32c(b->step_mode, b->lo_kind, (const BatchGrid<Bufs32> *)b->d_grids,
+                                                    b->d_tiles, b->ntiles, nblocks, per, q, b->stream)
+                          : launch_march_batch_f64(b->step_mode, (const BatchGrid<Bufs<double>> *)b->d_grids,
+                                                   b->d_tiles, b->ntiles, nblocks, per, q, b->stream);
+    return kerr(rc, "batch march");
+}
+
+// Advance member m by nsteps[m] (0: not at all). graph: replay a captured chain.
+static int run_chain(scp_batch *b, const int *nsteps, bool graph, bool timed = false) {
+    const size_t n = b->members.size();
+    if (n == 0) return 0;
+    int L = 0;
+    for (size_t m = 0; m < n; m++) {
+        if (nsteps[m] < 0) return set_err(-1, "nsteps[%zu]=%d < 0", m, nsteps[m]);
+        L = std::max(L, nsteps[m]);
+    }
+    if (L == 0) return 0;
+    if (b->stale) {
+        int rc = build_tables(b);
+        if (rc) return rc;
+    }
+    for (size_t m = 0; m < n; m++) {
+        scp_grid *g = b->members[m];
+        HIPCHK(hipStreamSynchronize(g->stream));      // the member's own work comes first
+        if (b->f32)
+            fill_desc((BatchGrid<Bufs32> *)b->h_grids + m, b, g, nsteps[m], bufs32_for);
+        else
+            fill_desc((BatchGrid<Bufs<double>> *)b->h_grids + m, b, g, nsteps[m], bufs_for);
+    }
+    HIPCHK(hipMemcpyAsync(b->d_grids, b->h_grids, n * desc_size(b), hipMemcpyHostToDevice, b->stream));
+    if (graph && !b->graphs.count(L)) {
+        // capture the chain on first use (kernel arguments: the table pointers and q)
+        hipGraph_t gr = nullptr;
+        hipGraphExec_t ex = nullptr;
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int rc = 0;
+        for (int q = 0; q < L && !rc; q++) rc = launch_one(b, q);
+        hipError_t e = hipStreamEndCapture(b->stream, &gr);
+        if (rc) {
+            if (gr) hipGraphDestroy(gr);
+            return rc;
+        }
+        if (e != hipSuccess) return set_err(-2, "graph capture failed: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        hipGraphDestroy(gr);
+        if (e != hipSuccess) return set_err(-2, "graph instantiate failed: %s", hipGetErrorString(e));
+        b->graphs.emplace(L, ex);
+    }
+    if (timed) HIPCHK(hipEventRecord(b->ev0, b->stream));
+    if (graph) {
+        HIPCHK(hipGraphLaunch(b->graphs[L], b->stream));
+    } else {
+        for (int q = 0; q < L; q++) {
+            int rc = launch_one(b, q);
+            if (rc) return rc;
+        }
+    }
+    if (timed) HIPCHK(hipEventRecord(b->ev1, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (size_t m = 0; m < n; m++) {
+        scp_grid *g = b->members[m];
+        g->cur ^= (nsteps[m] & 1);
+        g->steps += nsteps[m];
+    }
+    return 0;
+}
+
+extern "C" int scp_batch_create(scp_batch **out) {
+    if (!out) return set_err(-1, "null out");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(-4, "no HIP device");
+    scp_batch *b = new scp_batch;
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+        b->ncu = prop.multiProcessorCount;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e != hipSuccess) {
+        if (b->stream) hipStreamDestroy(b->stream);
+        delete b;
+        return set_err(-2, "batch create failed: %s", hipGetErrorString(e));
+    }
+    *out = b;
+    return 0;
+}
+
+extern "C" int scp_batch_destroy(scp_batch *b) {
+    if (!b) return 0;
+    hipStreamSynchronize(b->stream);
+    for (scp_grid *g : b->members) g->batch = nullptr;
+    drop_graphs(b);
+    hipFree(b->d_grids);
+    hipHostFree(b->h_grids);
+    hipFree(b->d_tiles);
+    hipEventDestroy(b->ev0);
+    hipEventDestroy(b->ev1);
+    hipStreamDestroy(b->stream);
+    delete b;
+    return 0;
+}
+
+extern "C" int scp_batch_add(scp_batch *b, scp_grid *g) {
+    if (!b || !g) return set_err(-1, "null argument");
+    if (!use_march(g, K_STEP))
+        return set_err(-1, "grid cannot join a batch: the batched step is the LDS-staged march "
+                           "(complex or gauged mode, step_kernel 0, one field below 4 GiB)");
+    if (g->batch) return set_err(-1, "grid is already a member of a batch");
+    if (b->members.empty()) {
+        b->step_mode = g->step_mode;
+        b->f32 = g->f32;
+        b->lo_kind = g->f32 ? g->lo_kind : 0;
+    } else if (g->step_mode != b->step_mode || g->f32 != b->f32 || (g->f32 && g->lo_kind != b->lo_kind)) {
+        return set_err(-1, "grid class (mode %d, f32c %d, lo %d) differs from the batch's (%d, %d, %d)",
+                       g->step_mode, g->f32, g->lo_kind, b->step_mode, b->f32, b->lo_kind);
+    }
+    b->members.push_back(g);
+    g->batch = b;
+    b->stale = true;
+    return 0;
+}
+
+extern "C" int scp_batch_remove(scp_batch *b, scp_grid *g) {
+    if (!b || !g) return set_err(-1, "null argument");
+    auto it = std::find(b->members.begin(), b->members.end(), g);
+    if (it == b->members.end()) return set_err(-1, "grid is not a member");
+    b->members.erase(it);
+    g->batch = nullptr;
+    b->stale = true;
+    return 0;
+}
+
+extern "C" int scp_batch_size(const scp_batch *b) { return b ? (int)b->members.size() : -1; }
+
+extern "C" int scp_batch_step(scp_batch *b, const int *nsteps, int n) {
+    if (!b || (!nsteps && n)) return set_err(-1, "null argument");
+    if (n != (int)b->members.size()) return set_err(-1, "n=%d but the batch has %zu members", n, b->members.size());
+    return run_chain(b, nsteps, true);
+}
+
+extern "C" int scp_batch_step_all(scp_batch *b, int nsteps) {
+    if (!b) return set_err(-1, "null batch");
+    if (nsteps < 1) return set_err(-1, "nsteps=%d must be >= 1", nsteps);
+    std::vector<int> v(b->members.size(), nsteps);
+    return run_chain(b, v.data(), true);
+}
+
+extern "C" int scp_batch_time_steps(scp_batch *b, int nsteps, float *ms) {
+    if (!b || !ms) return set_err(-1, "null argument");
+    if (nsteps < 1) return set_err(-1, "nsteps=%d must be >= 1", nsteps);
+    std::vector<int> v(b->members.size(), nsteps);
+    // events bracket the graph replay only (capture on first use is outside)
+    int rc = run_chain(b, v.data(), true, true);
+    if (rc) return rc;
+    HIPCHK(hipEventElapsedTime(ms, b->ev0, b->ev1));
+    return 0;
+}
+
+extern "C" int scp_batch_set_config(scp_batch *b, int di, int persistent) {
+    if (!b) return set_err(-1, "null batch");
+    if (di < 0) return set_err(-1, "di=%d < 0", di);
+    b->DI = di;
+    b->persistent = persistent ? 1 : 0;
+    b->stale = true;
+    return 0;
+}
+
+extern "C" int scp_batch_config(const scp_batch *b, int *di, int *persistent, int *tuned) {
+    if (!b) return set_err(-1, "null batch");
+    if (di) *di = b->DI;
+    if (persistent) *persistent = b->persistent;
+    if (tuned) *tuned = b->tuned ? 1 : 0;
+    return 0;
+}
+
+extern "C" int scp_batch_tune(scp_batch *b, int *steps_taken) {
+    if (!b) return set_err(-1, "null batch");
+    if (steps_taken) *steps_taken = 0;
+    if (b->members.empty()) return set_err(-1, "empty batch");
+    static const int DIS[] = {16, 32, 64, 128};
+    std::vector<int> one(b->members.size(), 1);
+    float best = 1e30f;
+    int bdi = 0, bper = 0, taken = 0;
+    for (int per = 0; per < 2; per++) {
+        for (int di : DIS) {
+            b->DI = di;
+            b->persistent = per;
+            b->stale = true;
+            // one untimed launch (table build, caches), then one timed
+            int rc = run_chain(b, one.data(), false);
+            if (rc) return rc;
+            rc = run_chain(b, one.data(), false, true);
+            if (rc) return rc;
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            taken += 2;
+            if (ms < best) {
+                best = ms;
+                bdi = di;
+                bper = per;
+            }
+        }
+    }
+    b->DI = bdi;
+    b->persistent = bper;
+    b->stale = true;
+    b->tuned = true;
+    if (steps_taken) *steps_taken = taken;
+    return 0;
+}

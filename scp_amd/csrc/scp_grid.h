@@ -1,0 +1,56 @@
+// scp_grid.h — internal: the device state behind the opaque scp_grid handle
+// and the helpers the C-ABI files share (scp_abi.hip, scp_batch.hip).
+#pragma once
+
+#include "scp_amd.h"
+#include "scp_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+struct scp_grid {
+    scp_params p;
+    scp::KParams kp;
+    int mode;        // storage mode (0/1/2), number of columns follows it
+    int step_mode;   // kernel mode: 2 only when gauged with g != 0
+    int NP;
+    hipStream_t stream;
+    int device;
+    int f32;          // SCP_PREC_F32C
+    int lo_kind;      // LoKind of the compensation words
+    double *P[2];     // FP64 state
+    double *W;
+    float *P32[2];    // FP32C state: hi words ...
+    float *W32;
+    void *Plo[2];     // ... and compensation words
+    void *Wlo;
+    double *xo, *vo;
+    double *aux;     // real-mode Cosserat/hardening fields (6), when needed
+    double *acc;     // forces output (lazily allocated)
+    double *partials;
+    double *dout;
+    unsigned long long *argmin;
+    int cur;
+    int nblocks_diag;
+    int64_t steps;
+    hipEvent_t ev0, ev1;
+    char kname[96];
+    struct scp_batch *batch;   // the batch this grid belongs to, if any
+};
+
+namespace scp_internal {
+int set_err(int code, const char *fmt, ...);
+int kerr(int rc, const char *what);
+// K_STEP runs the LDS-staged march for this grid
+bool use_march(const scp_grid *g, int kind);
+scp::Bufs<double> bufs_for(scp_grid *g, int kind);
+scp::Bufs32 bufs32_for(scp_grid *g, int kind);
+}  // namespace scp_internal
+
+// return the HIP error of x as status -2 with a message
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess)                                                                  \
+            return scp_internal::set_err(-2, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), \
+                                         __FILE__, __LINE__);                                  \
+    } while (0)

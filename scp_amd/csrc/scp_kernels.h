@@ -64,6 +64,22 @@ struct Bufs32 {
     const double *aux;
 };
 
+// Batched march (many independent grids of one mode and storage, one launch
+// per step): per-grid descriptor in device memory and a tile table of
+// (grid, local tile) pairs, local tile = tk + ntk * (tj + ntj * seg).
+template <class B>
+struct BatchGrid {
+    KParams kp;
+    B buf[2];          // the grid's buffers when its P parity is 0 / 1
+    int32_t DI;        // planes per segment
+    int32_t cur0;      // P parity at chain launch 0
+    int32_t nq;        // chain launches this grid takes part in; it emits on the last
+    uint32_t ntk, ntj; // tiles per plane along k and j
+};
+struct BatchTile {
+    uint32_t grid, tile;
+};
+
 // Launchers (scp_step.hip, scp_step32.hip). Return hipError_t as int.
 int launch_step_f64(int mode, int kind, bool emit, const KParams &kp, const Bufs<double> &b,
                     hipStream_t s);
@@ -77,6 +93,25 @@ int launch_march_f32c(int mode, bool emit, int lo_kind, const KParams &kp, const
 // the same with the tile width chosen: kw waves (64 sites each) per tile row
 int launch_march_f32c_kw(int kw, int mode, bool emit, int lo_kind, const KParams &kp, const Bufs32 &b, int DI,
                          hipStream_t s);
+// One batched launch, chain step q: nblocks blocks, block b runs tiles
+// [b*per_block, (b+1)*per_block) of the table (XCD-aware block order).
+int launch_march_batch_f64(int mode, const BatchGrid<Bufs<double>> *grids, const BatchTile *tiles,
+                           uint32_t ntiles, uint32_t nblocks, uint32_t per_block, int q, hipStream_t s);
+int launch_march_batch_f32c(int mode, int lo_kind, const BatchGrid<Bufs32> *grids, const BatchTile *tiles,
+                            uint32_t ntiles, uint32_t nblocks, uint32_t per_block, int q, hipStream_t s);
+// tiles per plane of the march at N (k, j), for building tile tables
+inline uint32_t march_tiles_k(int N) { return (uint32_t)(N + 63) / 64; }
+inline uint32_t march_tiles_j(int N) { return (uint32_t)(N + 3) / 4; }
+// default planes per block of the march (64*KW x 4 tiles): deep enough to
+// amortise the 3-plane prologue (3/DI extra plane reads), shallow enough for
+// >= ~16 blocks per CU (measured at 512^3: DI = 128, 4096 blocks, within 1 %
+// of the best over DI = 16..128)
+inline int march_di(int N, int KW) {
+    const uint32_t tiles = (uint32_t)((N + 64 * KW - 1) / (64 * KW)) * march_tiles_j(N);
+    int di = N;
+    while (di > 16 && (uint64_t)tiles * (uint64_t)((N + di - 1) / di) < 4096u / (uint32_t)KW) di /= 2;
+    return di;
+}
 int launch_real_aux(const KParams &kp, const double *pin, double *aux, hipStream_t s);
 int launch_real_aux_f32(const KParams &kp, const float *pin, double *aux, hipStream_t s);
 

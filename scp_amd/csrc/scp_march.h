@@ -57,20 +57,17 @@ struct MTile {
 
 __device__ inline int wrapc(int x, int N) { return x < 0 ? x + N : (x >= N ? x - N : x); }
 
+// tile (tk, tj) of segment seg of an N^3 grid; see MG for the tile size
 template <int KW>
-__device__ inline MTile mtile(int N, int DI) {
+__device__ inline MTile mtile(int N, int DI, uint32_t tk, uint32_t tj, uint32_t seg) {
     using G = MG<KW>;
     MTile t;
     t.N = N;
-    const uint32_t ntk = (N + G::TKM - 1) / G::TKM, ntj = (N + MTJ - 1) / MTJ;
-    uint32_t lb = logical_block(blockIdx.x, gridDim.x);
-    const uint32_t tk = lb % ntk; lb /= ntk;
-    const uint32_t tj = lb % ntj; lb /= ntj;
     t.k0 = tk * G::TKM;
     t.j0 = tj * MTJ;
     t.W = min(G::TKM, N - t.k0);
     t.H = min(MTJ, N - t.j0);
-    t.i0 = lb * DI;
+    t.i0 = seg * DI;
     t.i1 = min(N, t.i0 + DI);
     t.k = t.k0 + threadIdx.x;
     t.j = t.j0 + threadIdx.y;
@@ -282,15 +279,13 @@ struct S32 {
                                   float xh, Own s, bool damp, double dmpd) {
         const DF hdt = df_split(kp.hdt), dt = df_split(kp.dt), gad = df_split(kp.gad);
         float wh = s.wh, wl = s.wl, vh = 0.0f, vl = 0.0f;
-        if (EMIT || damp) {
-            vh = wh; vl = wl;
-            comp_add(vh, vl, hdt, a, 0.0f);                   // stage 4: v = W + dt/2 a
-            if (damp) df_scale_m(vh, vl, (float)dmpd);     // stage 5
-            wh = vh; wl = vl;
-            comp_add(wh, wl, hdt, a, 0.0f);                   // next stage 1
-        } else {
-            comp_add(wh, wl, dt, a, 0.0f);                   // both half kicks at once
-        }
+        // both half kicks, every step (also between emitted steps): the result then
+        // does not depend on how a run is split into calls or batch chains
+        vh = wh; vl = wl;
+        comp_add(vh, vl, hdt, a, 0.0f);                       // stage 4: v = W + dt/2 a
+        if (damp) df_scale_m(vh, vl, (float)dmpd);                 // stage 5
+        wh = vh; wl = vl;
+        comp_add(wh, wl, hdt, a, 0.0f);                       // next stage 1
         float xl = s.xl;
         const float xh0 = xh, xl0 = xl;
         if (MODE == 2 && f >= F_TH) {                       // link drift + wrap
@@ -317,18 +312,26 @@ struct S32 {
 // the kernel
 // ---------------------------------------------------------------------------
 
+template <int MODE>
+constexpr int march_nf() { return MODE == 2 ? 15 : 12; }   // staged fields (matter + links)
+template <class S, int KW, int MODE>
+constexpr int march_lds_elems() { return MNBUF * march_nf<MODE>() * MG<KW>::MPL; }
+
+// One tile of one grid: the march over planes [seg*DI, seg*DI + DI) of the
+// 64*KW x 4 column tile (tk, tj). lds: the block's staging area,
+// march_lds_elems() words. Ends with a barrier, so the caller may stage the
+// next tile into the same area.
 template <class S, int KW, int MODE, bool EMIT>
-__global__ void __launch_bounds__(MG<KW>::MNT, KW == 1 ? S::MINB : 1)
-k_march(KParams kp, typename S::B b, int DI) {
+__device__ __forceinline__ void march_tile(const KParams &kp, const typename S::B &b, int DI, uint32_t tk,
+                                           uint32_t tj, uint32_t seg, typename S::T *lds) {
     using T = typename S::T;
-    constexpr int NF = MODE == 2 ? 15 : 12;   // staged fields (matter + links)
+    constexpr int NF = march_nf<MODE>();
     constexpr int NM = 12;                    // matter fields (the i-1 plane is matter only)
     constexpr int MPL = MG<KW>::MPL;
     constexpr int RQ = Halo<NF, KW>::RQ;
-    __shared__ T lds[MNBUF * NF * MPL];
 
     const int N = kp.N;
-    const MTile t = mtile<KW>(N, DI);
+    const MTile t = mtile<KW>(N, DI, tk, tj, seg);
     if (t.i0 >= N) return;                    // whole block: uniform
     Halo<NF, KW> hs;
     hs.init(t);
@@ -468,16 +471,51 @@ k_march(KParams kp, typename S::B b, int DI) {
         hc = hcc;
         bb = nbuf;
     }
+    __syncthreads();
 }
 
-// planes per block: deep enough to amortise the 3-plane prologue (3/DI extra
-// plane reads), shallow enough for >= ~16 blocks per CU (measured at 512^3:
-// DI = 128, 4096 blocks, within 1 % of the best over DI = 16..128)
-inline int march_di(int N, int KW) {
-    const uint32_t tiles = ((N + TK * KW - 1) / (TK * KW)) * ((N + MTJ - 1) / MTJ);
-    int di = N;
-    while (di > 16 && (uint64_t)tiles * ((N + di - 1) / di) < 4096u / KW) di /= 2;
-    return di;
+// One grid: block b runs tile (tk, tj, seg) of the XCD-aware logical numbering.
+template <class S, int KW, int MODE, bool EMIT>
+__global__ void __launch_bounds__(MG<KW>::MNT, KW == 1 ? S::MINB : 1)
+k_march(KParams kp, typename S::B b, int DI) {
+    __shared__ typename S::T lds[march_lds_elems<S, KW, MODE>()];
+    const uint32_t ntk = (kp.N + MG<KW>::TKM - 1) / MG<KW>::TKM, ntj = (kp.N + MTJ - 1) / MTJ;
+    uint32_t lb = logical_block(blockIdx.x, gridDim.x);
+    const uint32_t tk = lb % ntk; lb /= ntk;
+    const uint32_t tj = lb % ntj; lb /= ntj;
+    march_tile<S, KW, MODE, EMIT>(kp, b, DI, tk, tj, lb, lds);
+}
+
+// Many grids: block b runs tiles [b*per_block, (b+1)*per_block) of the table
+// (b in XCD-aware order, so an XCD's blocks take adjacent tiles); a grid takes
+// part in chain launches q < nq and emits its FP64 state on the last.
+template <class S, int KW, int MODE>
+__global__ void __launch_bounds__(MG<KW>::MNT, KW == 1 ? S::MINB : 1)
+k_march_batch(const BatchGrid<typename S::B> *__restrict__ grids, const BatchTile *__restrict__ tiles,
+              uint32_t ntiles, uint32_t per_block, int q) {
+    __shared__ typename S::T lds[march_lds_elems<S, KW, MODE>()];
+    const uint32_t b = logical_block(blockIdx.x, gridDim.x);
+    const uint32_t t1 = min(ntiles, (b + 1) * per_block);
+    for (uint32_t t = b * per_block; t < t1; t++) {
+        const BatchTile bt = tiles[t];
+        const BatchGrid<typename S::B> &g = grids[bt.grid];
+        if (q >= g.nq) continue;
+        const uint32_t tk = bt.tile % g.ntk, r = bt.tile / g.ntk;
+        const uint32_t tj = r % g.ntj, seg = r / g.ntj;
+        const int par = (g.cur0 + q) & 1;
+        if (q == g.nq - 1)
+            march_tile<S, KW, MODE, true>(g.kp, g.buf[par], g.DI, tk, tj, seg, lds);
+        else
+            march_tile<S, KW, MODE, false>(g.kp, g.buf[par], g.DI, tk, tj, seg, lds);
+    }
+}
+
+template <class S, int MODE>
+inline int launch_march_batch_t(const BatchGrid<typename S::B> *grids, const BatchTile *tiles, uint32_t ntiles,
+                                uint32_t nblocks, uint32_t per_block, int q, hipStream_t s) {
+    hipLaunchKernelGGL((k_march_batch<S, 1, MODE>), dim3(nblocks), dim3(MG<1>::TKM, MTJ), 0, s, grids, tiles,
+                       ntiles, per_block, q);
+    return (int)hipGetLastError();
 }
 
 template <class S, int KW, int MODE>

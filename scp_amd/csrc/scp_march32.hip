@@ -32,4 +32,19 @@ int launch_march_f32c(int mode, bool emit, int lo_kind, const KParams &kp, const
     return launch_march_f32c_kw(1, mode, emit, lo_kind, kp, b, DI, s);
 }
 
+template <int LOK>
+static int batch_lo(int mode, const BatchGrid<Bufs32> *grids, const BatchTile *tiles, uint32_t ntiles,
+                    uint32_t nblocks, uint32_t per_block, int q, hipStream_t s) {
+    return mode == 2 ? launch_march_batch_t<S32<LOK>, 2>(grids, tiles, ntiles, nblocks, per_block, q, s)
+                     : launch_march_batch_t<S32<LOK>, 1>(grids, tiles, ntiles, nblocks, per_block, q, s);
+}
+
+int launch_march_batch_f32c(int mode, int lo_kind, const BatchGrid<Bufs32> *grids, const BatchTile *tiles,
+                            uint32_t ntiles, uint32_t nblocks, uint32_t per_block, int q, hipStream_t s) {
+    if (mode != 1 && mode != 2) return (int)hipErrorInvalidValue;
+    if (lo_kind == LO_F32) return batch_lo<LO_F32>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
+    if (lo_kind == LO_BF16) return batch_lo<LO_BF16>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
+    return batch_lo<LO_NONE>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
+}
+
 }  // namespace scp

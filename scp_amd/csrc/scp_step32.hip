@@ -7,8 +7,9 @@
 // covariant transport, plaquettes — is evaluated in FP32 on the hi words; the
 // leapfrog accumulations v += h a, x += dt v, E and link updates are
 // compensated FP32 additions (TwoSum with the increment formed by FMAs), so
-// increments far below ulp(x) accumulate in lo instead of being lost. Between
-// two emitted steps the two half kicks fuse into one W += dt a. FP64 appears
+// increments far below ulp(x) accumulate in lo instead of being lost. Both half
+// kicks are taken every step, so results do not depend on how a run is split
+// into calls. FP64 appears
 // only in INIT (from the FP64 host state) and in the emitted snapshot state
 // (hi + lo), which the FP64 diagnostics (energy, Q, Gauss residual) read.
 //
@@ -166,15 +167,13 @@ k_step32(KParams kp, Bufs32 b, int DI) {
                 const size_t o = (size_t)f * fs + ci;
                 float wh = __builtin_nontemporal_load(&b.w[o]), wl = LS::get(b.wlo, o);
                 float vh = 0.0f, vl = 0.0f;
-                if (EMIT || damp) {
-                    vh = wh; vl = wl;
-                    comp_add(vh, vl, hdt, acc[f], 0.0f);                // stage 4: v = W + dt/2 a
-                    if (damp) df_scale(vh, vl, dmp);               // stage 5
-                    wh = vh; wl = vl;
-                    comp_add(wh, wl, hdt, acc[f], 0.0f);                // next stage 1
-                } else {
-                    comp_add(wh, wl, dt, acc[f], 0.0f);                // both half kicks at once
-                }
+                // both half kicks, every step (also between emitted steps): the result then
+                // does not depend on how a run is split into calls or batch chains
+                vh = wh; vl = wl;
+                comp_add(vh, vl, hdt, acc[f], 0.0f);                       // stage 4: v = W + dt/2 a
+                if (damp) df_scale(vh, vl, dmp);                 // stage 5
+                wh = vh; wl = vl;
+                comp_add(wh, wl, hdt, acc[f], 0.0f);                       // next stage 1
                 float xh = c0[f], xl = LS::get(b.pinlo, o);
                 const float xh0 = xh, xl0 = xl;
                 if (MODE == 2 && f >= F_TH) {                       // link drift + wrap

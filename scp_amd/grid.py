@@ -35,7 +35,10 @@ ABI_SYMBOLS = ["scp_abi_version", "scp_last_error", "scp_grid_bytes", "scp_grid_
                "scp_grid_destroy", "scp_grid_upload", "scp_grid_download", "scp_grid_step",
                "scp_grid_diag", "scp_grid_forces", "scp_grid_gauss_project",
                "scp_grid_set_g_offset", "scp_grid_g_offset", "scp_grid_rescale_phi",
-               "scp_grid_steps_done", "scp_grid_time_steps", "scp_grid_step_kernel_name"]
+               "scp_grid_steps_done", "scp_grid_time_steps", "scp_grid_step_kernel_name",
+               "scp_batch_create", "scp_batch_destroy", "scp_batch_add", "scp_batch_remove",
+               "scp_batch_size", "scp_batch_step", "scp_batch_step_all", "scp_batch_time_steps",
+               "scp_batch_tune", "scp_batch_set_config", "scp_batch_config"]
 
 
 class ScpError(RuntimeError):
@@ -111,6 +114,18 @@ def lib():
         L.scp_grid_time_steps.argtypes = [VP, C.c_int, C.POINTER(C.c_float)]
         L.scp_grid_step_kernel_name.restype = C.c_char_p
         L.scp_grid_step_kernel_name.argtypes = [VP]
+        IP = C.POINTER(C.c_int)
+        L.scp_batch_create.argtypes = [C.POINTER(VP)]
+        L.scp_batch_destroy.argtypes = [VP]
+        L.scp_batch_add.argtypes = [VP, VP]
+        L.scp_batch_remove.argtypes = [VP, VP]
+        L.scp_batch_size.argtypes = [VP]
+        L.scp_batch_step.argtypes = [VP, IP, C.c_int]
+        L.scp_batch_step_all.argtypes = [VP, C.c_int]
+        L.scp_batch_time_steps.argtypes = [VP, C.c_int, C.POINTER(C.c_float)]
+        L.scp_batch_tune.argtypes = [VP, IP]
+        L.scp_batch_set_config.argtypes = [VP, C.c_int, C.c_int]
+        L.scp_batch_config.argtypes = [VP, IP, IP, IP]
         _typed = True
     return L
 
@@ -209,3 +224,72 @@ class Grid:
     @property
     def step_kernel_name(self):
         return lib().scp_grid_step_kernel_name(self._h).decode()
+
+
+class Batch:
+    """Many grids of one class advanced together (scp_batch_* of include/scp_amd.h).
+    Holds references to its member Grid objects (a closed grid leaves the batch)."""
+
+    def __init__(self, grids=()):
+        self._h = C.c_void_p()
+        self.members = []
+        _check(lib().scp_batch_create(C.byref(self._h)), "scp_batch_create")
+        for g in grids:
+            self.add(g)
+
+    def add(self, grid):
+        _check(lib().scp_batch_add(self._h, grid._h), "scp_batch_add")
+        self.members.append(grid)
+
+    def remove(self, grid):
+        _check(lib().scp_batch_remove(self._h, grid._h), "scp_batch_remove")
+        self.members.remove(grid)
+
+    def __len__(self):
+        return len(self.members)
+
+    def step(self, nsteps):
+        """nsteps: int (every member) or a sequence, one count per member."""
+        if isinstance(nsteps, int):
+            _check(lib().scp_batch_step_all(self._h, nsteps), "scp_batch_step_all")
+        else:
+            v = (C.c_int * len(nsteps))(*[int(n) for n in nsteps])
+            _check(lib().scp_batch_step(self._h, v, len(nsteps)), "scp_batch_step")
+
+    def time_steps(self, n):
+        ms = C.c_float()
+        _check(lib().scp_batch_time_steps(self._h, int(n), C.byref(ms)), "scp_batch_time_steps")
+        return ms.value
+
+    def tune(self):
+        """Autotune depth and launch form on the batch's own steps; returns steps taken."""
+        n = C.c_int()
+        _check(lib().scp_batch_tune(self._h, C.byref(n)), "scp_batch_tune")
+        return n.value
+
+    def set_config(self, di=0, persistent=False):
+        _check(lib().scp_batch_set_config(self._h, int(di), 1 if persistent else 0), "scp_batch_set_config")
+
+    @property
+    def config(self):
+        di, per, tuned = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().scp_batch_config(self._h, C.byref(di), C.byref(per), C.byref(tuned)), "scp_batch_config")
+        return {"di": di.value, "persistent": bool(per.value), "tuned": bool(tuned.value)}
+
+    def close(self):
+        if self._h:
+            lib().scp_batch_destroy(self._h)
+            self._h = C.c_void_p()
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

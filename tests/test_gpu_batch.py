@@ -1,0 +1,108 @@
+"""Batched stepping (scp_batch_*): many grids of different sizes in one launch
+per step, chains replayed from captured HIP graphs. Every result must be
+bitwise the result of stepping each grid alone (scp_grid_step), across graph
+replays with eager work between them, per-member step counts, the autotuner
+and every launch configuration."""
+import numpy as np
+import pytest
+
+from scp_amd import grid as G
+from scp_amd import seeds
+
+pytestmark = pytest.mark.gpu
+
+SIZES = (16, 24, 33, 40, 70)
+
+
+def ball(N, seed):
+    L = 0.25 * (N - 1)
+    dx = 2 * L / (N - 1)
+    rng = np.random.default_rng(seed)
+    b = [dict(x0=float(rng.uniform(-0.3, 0.3) * L), y0=float(rng.uniform(-0.3, 0.3) * L), z0=0.0, amp=0.8,
+              sigma=3 * dx, omega=float(rng.uniform(0.6, 0.95)), phase=float(rng.uniform(0, 6.28)), winding=1)]
+    return L, seeds.seed_balls(N, L, 2, b, theta_sigma=0.01, link_sigma=0.02, seed=seed)
+
+
+def make(N, seed, prec, lo):
+    L, st = ball(N, seed)
+    g = G.Grid(G.make_params(N, G.MODE_GAUGED, L, 0.2, g=0.1, precision=prec, lo_words=lo))
+    g.upload(st)
+    return g
+
+
+@pytest.mark.parametrize("prec,lo", [(G.PREC_F64, 0), (G.PREC_F32C, 1), (G.PREC_F32C, 0)])
+def test_batch_equals_each_grid_alone(prec, lo):
+    print(f"case prec={prec} lo={lo} sizes={SIZES}", flush=True)
+    alone = [make(N, 100 + i, prec, lo) for i, N in enumerate(SIZES)]
+    mixed = [make(N, 100 + i, prec, lo) for i, N in enumerate(SIZES)]
+    counts = [3, 5, 0, 7, 2]
+    with G.Batch(mixed) as b:
+        for rep in range(3):                       # the same chain replayed 3 times
+            for g in alone:
+                g.step(7)
+            b.step(7)
+            for g, h in zip(alone, mixed):        # eager work between the replays
+                h.diag()
+                assert np.array_equal(g.download(), h.download()), (rep, g.params.N)
+        for g, n in zip(alone, counts):
+            if n:
+                g.step(n)
+        b.step(counts)
+        for g, h, n in zip(alone, mixed, counts):
+            assert np.array_equal(g.download(), h.download()), ("counts", g.params.N)
+            assert h.steps_done == g.steps_done
+    for g in alone + mixed:
+        g.close()
+
+
+@pytest.mark.parametrize("di,persistent", [(16, False), (16, True), (128, True), (32, False)])
+def test_batch_launch_configs_agree(di, persistent):
+    ref = [make(N, 200 + i, G.PREC_F32C, 1) for i, N in enumerate(SIZES)]
+    got = [make(N, 200 + i, G.PREC_F32C, 1) for i, N in enumerate(SIZES)]
+    for g in ref:
+        g.step(6)
+    with G.Batch(got) as b:
+        b.set_config(di=di, persistent=persistent)
+        b.step(6)
+    for g, h in zip(ref, got):
+        assert np.array_equal(g.download(), h.download()), g.params.N
+        g.close()
+        h.close()
+
+
+def test_batch_autotune_keeps_results():
+    ref = [make(N, 300 + i, G.PREC_F32C, 1) for i, N in enumerate(SIZES)]
+    got = [make(N, 300 + i, G.PREC_F32C, 1) for i, N in enumerate(SIZES)]
+    with G.Batch(got) as b:
+        taken = b.tune()
+        cfg = b.config
+        b.step(4)
+    print(f"tuned: {cfg}, steps taken {taken}")
+    assert cfg["tuned"] and cfg["di"] in (16, 32, 64, 128)
+    for g, h in zip(ref, got):
+        g.step(taken + 4)
+        assert np.array_equal(g.download(), h.download()), g.params.N
+        g.close()
+        h.close()
+
+
+def test_batch_membership_rules():
+    a = make(16, 1, G.PREC_F32C, 1)
+    b64 = make(16, 2, G.PREC_F64, 0)
+    L = 0.25 * 15
+    real = G.Grid(G.make_params(16, G.MODE_REAL, L, 0.2))
+    with G.Batch([a]) as b:
+        with pytest.raises(G.ScpError):
+            b.add(b64)                 # other class
+        with pytest.raises(G.ScpError):
+            b.add(real)                # the real step is not batched
+        with pytest.raises(G.ScpError):
+            b.add(a)                   # already a member
+        b.remove(a)
+        assert len(b) == 0
+        b.add(a)
+        a.close()                      # a destroyed grid leaves its batch
+        b.members.remove(a)
+        assert G.lib().scp_batch_size(b._h) == 0
+    b64.close()
+    real.close()

@@ -57,10 +57,13 @@ def test_forces_match_oracle(name):
         assert np.array_equal(acc, ref)
 
 
-def test_split_steps_equal_one_call():
-    """Emitting on the last launch of a call must not perturb the state."""
+@pytest.mark.parametrize("prec", [G.PREC_F64, G.PREC_F32C])
+def test_split_steps_equal_one_call(prec):
+    """Emitting on the last launch of a call must not perturb the state (FP64
+    and FP32C: both take the two half kicks on every step)."""
     fx = fixtures.load("gauged_n8")
-    with G.Grid(gpu_params(fx)) as a, G.Grid(gpu_params(fx)) as b:
+    with G.Grid(gpu_params(fx, precision=prec, lo_words=1)) as a, \
+            G.Grid(gpu_params(fx, precision=prec, lo_words=1)) as b:
         a.upload(fx["frame0"])
         b.upload(fx["frame0"])
         a.step(7)
