@@ -200,6 +200,89 @@ def sweep_bench(a):
     print(json.dumps(out), flush=True)
 
 
+def serve_bench(a):
+    """Serving: Poisson job arrivals (rate a.rate per second for a.duration
+    seconds); each job a gauged periodic grid of edge log-uniform in
+    [48, 256], steps uniform in [steps_lo, steps_hi], 1-4 seeded Q-balls;
+    admitted by an HBM budget, device-resident, continuously batched. Reports
+    p50/p99 completion latency (arrival to completion) and aggregate
+    site-updates/s; jobs still running when the horizon ends are counted."""
+    from scp_amd import grid as G
+    rng = np.random.default_rng(a.seed)
+    arrivals, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / a.rate)
+        if t > a.duration:
+            break
+        arrivals.append(t)
+    specs = []
+    for _ in arrivals:
+        N = int(round(np.exp(rng.uniform(np.log(48), np.log(256)))))
+        L = 0.25 * (N - 1)
+        dx = 0.5
+        nb = int(rng.integers(1, 5))
+        balls = []
+        for b in range(nb):
+            sg = 1 if b % 2 == 0 else -1
+            balls.append(dict(x0=float(rng.uniform(-0.3, 0.3) * L), y0=float(rng.uniform(-0.3, 0.3) * L),
+                              z0=float(rng.uniform(-0.3, 0.3) * L), amp=float(rng.uniform(0.6, 0.9)),
+                              sigma=float(rng.uniform(2.0, 4.0)) * dx, omega=sg * float(rng.uniform(0.6, 0.95)),
+                              phase=float(rng.uniform(0, 2 * np.pi)), winding=sg))
+        specs.append(dict(N=N, L=L, balls=balls, steps=int(rng.integers(a.steps_lo, a.steps_hi + 1))))
+    offered = sum(s_["N"] ** 3 * s_["steps"] for s_ in specs)
+    log(f"{len(specs)} jobs over {a.duration:.0f}s, offered {offered / 1e9:.1f} G site-updates")
+    lat, done_sites = [], 0
+    with G.Server(int(a.budget_gb * 1e9), quantum=a.quantum) as srv:
+        pending, i, next_log = {}, 0, 10.0
+        while True:
+            now = srv.now()
+            if now >= next_log:
+                st = srv.stats()
+                log(f"t={now:6.1f}s submitted {i} queued {st['queued']} active {st['active']} "
+                    f"completed {len(lat)} in use {st['bytes_in_use'] / 1e9:.1f} GB")
+                next_log = now + 10.0
+            while i < len(specs) and arrivals[i] <= now:
+                sp = specs[i]
+                p = G.make_params(sp["N"], G.MODE_GAUGED, sp["L"], 0.2, g=0.05, precision=G.PREC_F32C,
+                                  lo_words=a.lo_words)
+                pending[srv.submit(p, sp["balls"], sp["steps"], link_sigma=0.0, seed=a.seed + i)] = i
+                i += 1
+            if now > a.duration + a.horizon:
+                break
+            st = srv.stats()
+            if st["active"] == 0 and st["queued"] == 0 and st["done"] == 0:
+                if i == len(specs):
+                    break
+                time.sleep(max(0.0, min(arrivals[i] - now, 1.0)))
+                continue
+            srv.round()
+            for jid in list(pending):
+                info = srv.job(jid)
+                if info["state"] == G.JOB_DONE:
+                    k = pending.pop(jid)
+                    srv.take(jid, state=False)
+                    lat.append(info["t_done"] - arrivals[k])
+                    done_sites += specs[k]["N"] ** 3 * specs[k]["steps"]
+        st = srv.stats()
+    elapsed = st["now"]
+    completed = len(lat)
+    lat = np.array(lat) if lat else np.array([float("nan")])
+    out = {"metric": "serving: p50/p99 job completion latency and aggregate Gsite-updates/s",
+           "value": st["site_updates"] / elapsed / 1e9, "unit": "Gsite-updates/s", "n_gpus": 1,
+           "higher_is_better": True, "vs_baseline": None,
+           "dtype": f"f32c ({['f32', 'bf16', 'no'][a.lo_words]} compensation words)",
+           "data": "synthetic (Poisson arrivals, seeded Q-ball jobs)",
+           "config": {"workload": "gauged jobs, edge log-uniform [48,256], 1-4 Q-balls", "rate_per_s": a.rate,
+                      "duration_s": a.duration, "horizon_s": a.horizon, "steps": [a.steps_lo, a.steps_hi],
+                      "hbm_budget_gb": a.budget_gb, "quantum": a.quantum, "seed": a.seed},
+           "jobs": {"submitted": i, "completed": completed, "unfinished": i - completed},
+           "latency_s": {"p50": float(np.nanpercentile(lat, 50)), "p99": float(np.nanpercentile(lat, 99)),
+                         "max": float(np.nanmax(lat))},
+           "offered_gsite_updates_per_s": offered / a.duration / 1e9,
+           "busy_fraction": st["busy_seconds"] / elapsed, "elapsed_s": elapsed}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,8 +295,16 @@ def main():
                     help="FP32C compensation words: 0 FP32, 1 bfloat16 (default), 2 none (plain FP32)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="step", choices=["step", "sweep"],
-                    help="step: the 512^3 gauged step (default); sweep: batched independent grids")
+    ap.add_argument("--workload", default="step", choices=["step", "sweep", "serve"],
+                    help="step: the 512^3 gauged step (default); sweep: batched independent grids; "
+                         "serve: Poisson job arrivals through the request loop")
+    ap.add_argument("--rate", type=float, default=4.0, help="serve: job arrivals per second")
+    ap.add_argument("--duration", type=float, default=600.0, help="serve: arrival window (s)")
+    ap.add_argument("--horizon", type=float, default=60.0, help="serve: time after the window to finish jobs (s)")
+    ap.add_argument("--steps-lo", type=int, default=1000)
+    ap.add_argument("--steps-hi", type=int, default=20000)
+    ap.add_argument("--budget-gb", type=float, default=250.0, help="serve: HBM budget")
+    ap.add_argument("--quantum", type=int, default=200, help="serve: steps per scheduling round")
     ap.add_argument("--grids", type=int, default=256, help="sweep: number of grids")
     ap.add_argument("--seed", type=int, default=3, help="sweep: seed of sizes and balls")
     ap.add_argument("--copy-calib", type=float, default=0.0,
@@ -229,6 +320,9 @@ def main():
         sys.exit(2)
     if a.workload == "sweep":
         sweep_bench(a)
+        return
+    if a.workload == "serve":
+        serve_bench(a)
         return
     from scp_amd import grid as G
 

@@ -114,6 +114,25 @@ int  scp_grid_time_steps(scp_grid *g, int nsteps, float *ms_total);
 /* Name of the step kernel as rocprof reports it (for profile matching). */
 const char *scp_grid_step_kernel_name(const scp_grid *g);
 
+/* One Gaussian Q-ball of an initial state (physical units, minimum image in
+ * the periodic box): Phi_a = amp exp(-r^2 / (2 sigma^2)) e^{i(phase + winding
+ * 2 pi a / 3)}, dPhi_a/dt = i omega Phi_a. */
+typedef struct {
+    double x0, y0, z0, amp, sigma, omega, phase;
+    int32_t winding;      /* +1 / -1 */
+    int32_t reserved;     /* 0 */
+} scp_qball;
+
+/* Seed the grid on the device (complex or gauged mode): a superposition of
+ * Q-balls, Theta ~ N(0, theta_sigma^2) per component (tu and tv), zero Theta
+ * velocities, links ~ N(0, link_sigma^2), E = 0, from counter-based random
+ * numbers keyed by seed; then the initial force evaluation and first half kick,
+ * as scp_grid_upload. The reference seeds such states with separate generator
+ * programs that write an SFA file for init=sfa (scp_init.h:489-499); this
+ * needs neither the host state nor its copy. */
+int  scp_grid_seed_balls(scp_grid *g, const scp_qball *balls, int nballs, double theta_sigma,
+                         double link_sigma, uint64_t seed);
+
 /* Batches: many grids advanced together. No single-call counterpart in the
  * reference, where every job is its own scp_sim process started by the Go
  * runner (LocalExecutor.executeRun, sfa/runner/local.go:239-330) with at most
@@ -140,9 +159,63 @@ int  scp_batch_time_steps(scp_batch *b, int nsteps, float *ms_total);
  * tile, or persistent resident blocks over contiguous tile ranges), each
  * timed on the batch's own steps; every member advances *steps_taken steps. */
 int  scp_batch_tune(scp_batch *b, int *steps_taken);
-/* di = 0: per-grid default depth; persistent: 0 / 1. */
+/* di = 0: default depth (the deepest of 128, 64, 32, 16 planes that gives
+ * >= 4096 blocks over all members); persistent: 0 / 1. */
 int  scp_batch_set_config(scp_batch *b, int di, int persistent);
 int  scp_batch_config(const scp_batch *b, int *di, int *persistent, int *tuned);
+
+/* Serving: a request loop over device-resident grids. Counterpart of the Go
+ * runner's job model (sfa/runner/local.go: Run :213 queues a config, a
+ * concurrency slot is acquired :253, executeRun :239 execs one scp_sim process
+ * per job, RunStatus :483, Download :514), rebuilt as one process whose jobs
+ * share the GPU: jobs are (parameters, Q-balls, seed, steps); each round
+ * admits queued jobs in arrival order while their scp_grid_bytes fit the HBM
+ * budget (seeded on the device, Gauss-projected if asked), advances all active
+ * jobs by up to `quantum` steps in one batched chain per class, and retires
+ * finished jobs, whose grids stay resident until scp_server_take. */
+typedef struct scp_server scp_server;
+
+typedef struct {
+    scp_params params;         /* complex or gauged; step_kernel 0 */
+    const scp_qball *balls;    /* copied at submission */
+    int32_t nballs;
+    int32_t nsteps;            /* >= 1 */
+    double theta_sigma, link_sigma;
+    uint64_t seed;
+    int32_t gauss_project;     /* 1: Gauss projection after seeding (gauged, g != 0) */
+    int32_t reserved;          /* 0 */
+} scp_job;
+
+enum { SCP_JOB_FAILED = -1, SCP_JOB_QUEUED = 0, SCP_JOB_ACTIVE = 1, SCP_JOB_DONE = 2 };
+
+typedef struct {
+    int32_t state;             /* SCP_JOB_* */
+    int32_t nsteps;
+    int64_t steps_done, bytes;
+    double t_submit, t_admit, t_done;   /* seconds on the server's clock (scp_server_now) */
+} scp_job_info;
+
+typedef struct {
+    int64_t queued, active, done, failed;   /* done: finished, result not taken yet */
+    int64_t bytes_in_use;                   /* device bytes of admitted and finished jobs */
+    int64_t site_updates;                   /* site-steps advanced so far */
+    double busy_seconds;                    /* time spent in batched chains */
+    double now;
+} scp_server_stats;
+
+/* hbm_budget: device bytes jobs may hold at once; quantum: steps per round;
+ * max_active: 0 = no limit besides memory. */
+int  scp_server_create(int64_t hbm_budget, int quantum, int max_active, scp_server **out);
+int  scp_server_destroy(scp_server *s);
+int  scp_server_submit(scp_server *s, const scp_job *job, int64_t *id);
+/* One scheduling round (admit, step, retire); *finished = jobs retired. */
+int  scp_server_round(scp_server *s, int *finished);
+int  scp_server_job(const scp_server *s, int64_t id, scp_job_info *info);
+/* A finished job's state (cols may be NULL) and diagnostics row (diag may be
+ * NULL); frees its device memory. A failed job reports its failure here. */
+int  scp_server_take(scp_server *s, int64_t id, double *const *cols, int ncols, double *diag, int ndiag);
+int  scp_server_stats_get(const scp_server *s, scp_server_stats *st);
+double scp_server_now(const scp_server *s);
 
 #ifdef __cplusplus
 }

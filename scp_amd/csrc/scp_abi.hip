@@ -268,7 +268,7 @@ static int launch_kind(scp_grid *g, int kind, bool emit) {
 }
 
 // initial force + first half kick + first drift, from xo/vo
-static int run_init(scp_grid *g) {
+int scp_internal::run_init(scp_grid *g) {
     int rc = run_aux(g, g->xo);
     if (rc) return rc;
     rc = launch_kind(g, K_INIT, false);
@@ -288,6 +288,25 @@ extern "C" int scp_grid_upload(scp_grid *g, const double *const *cols, int ncols
         HIPCHK(hipMemcpyAsync(g->vo + (size_t)f * g->kp.fs, vp, nb, hipMemcpyHostToDevice, g->stream));
     }
     int rc = run_init(g);
+    if (rc) return rc;
+    g->steps = 0;
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+extern "C" int scp_grid_seed_balls(scp_grid *g, const scp_qball *balls, int nballs, double theta_sigma,
+                                   double link_sigma, uint64_t seed) {
+    if (!g) return set_err(-1, "null grid");
+    if (g->mode == 0) return set_err(-1, "seeding Q-balls needs the complex or gauged mode");
+    if (nballs < 0 || (nballs > 0 && !balls)) return set_err(-1, "bad ball list");
+    for (int b = 0; b < nballs; b++)
+        if (!(balls[b].sigma > 0) || (balls[b].winding != 1 && balls[b].winding != -1))
+            return set_err(-1, "ball %d: sigma must be > 0 and winding +-1", b);
+    HIPCHK(hipStreamSynchronize(g->stream));
+    int rc = seed_balls_device(g->p.N, g->p.L, g->p.dx, g->kp.fs, g->mode == 2, balls, nballs, theta_sigma,
+                               link_sigma, seed, g->xo, g->vo, g->stream);
+    if (rc) return set_err(-2, "device seeding failed: %s", hipGetErrorString((hipError_t)rc));
+    rc = run_init(g);
     if (rc) return rc;
     g->steps = 0;
     HIPCHK(hipStreamSynchronize(g->stream));

@@ -37,7 +37,8 @@ struct scp_batch {
     size_t tiles_cap = 0;
     uint32_t ntiles = 0;
     bool stale = true;          // member set or configuration changed since the tables were built
-    int DI = 0;                 // 0: per-grid default (march_di)
+    int DI = 0;                 // 0: batch default (batch_default_di)
+    int auto_di = 64;
     int persistent = 0;
     bool tuned = false;
     std::map<int, hipGraphExec_t> graphs;   // chain length -> replayable chain (valid for the current tables)
@@ -53,13 +54,28 @@ static size_t desc_size(const scp_batch *b) {
 }
 
 static int grid_di(const scp_batch *b, const scp_grid *g) {
-    int di = b->DI > 0 ? b->DI : march_di(g->p.N, 1);
-    return std::max(1, std::min(di, g->p.N));
+    return std::max(1, std::min(b->DI > 0 ? b->DI : b->auto_di, g->p.N));
+}
+
+// default depth for the whole batch: the deepest segments (fewest prologue
+// re-reads) that still give >= 4096 blocks over all members
+static int batch_default_di(const scp_batch *b) {
+    int di = 128;
+    for (; di > 16; di /= 2) {
+        uint64_t tiles = 0;
+        for (const scp_grid *g : b->members) {
+            const int N = g->p.N, d = std::min(di, N);
+            tiles += (uint64_t)march_tiles_k(N) * march_tiles_j(N) * (uint64_t)((N + d - 1) / d);
+        }
+        if (tiles >= 4096) break;
+    }
+    return di;
 }
 
 // rebuild descriptor and tile tables for the current members (device copies)
 static int build_tables(scp_batch *b) {
     const size_t n = b->members.size();
+    b->auto_di = batch_default_di(b);
     std::vector<BatchTile> tiles;
     for (size_t m = 0; m < n; m++) {
         const scp_grid *g = b->members[m];
@@ -286,7 +302,7 @@ extern "C" int scp_batch_set_config(scp_batch *b, int di, int persistent) {
 
 extern "C" int scp_batch_config(const scp_batch *b, int *di, int *persistent, int *tuned) {
     if (!b) return set_err(-1, "null batch");
-    if (di) *di = b->DI;
+    if (di) *di = b->DI > 0 ? b->DI : b->auto_di;
     if (persistent) *persistent = b->persistent;
     if (tuned) *tuned = b->tuned ? 1 : 0;
     return 0;

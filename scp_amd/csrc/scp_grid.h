@@ -37,6 +37,14 @@ struct scp_grid {
     struct scp_batch *batch;   // the batch this grid belongs to, if any
 };
 
+namespace scp {
+// device seeding (scp_seed.hip): fills xo/vo (device field order) of an N^3
+// grid with field stride fs; synchronous on s; returns a hipError_t
+int seed_balls_device(int N, double L, double dx, uint32_t fs, int gauged, const scp_qball *balls, int nballs,
+                      double theta_sigma, double link_sigma, uint64_t seed, double *xo, double *vo,
+                      hipStream_t s);
+}  // namespace scp
+
 namespace scp_internal {
 int set_err(int code, const char *fmt, ...);
 int kerr(int rc, const char *what);
@@ -44,6 +52,8 @@ int kerr(int rc, const char *what);
 bool use_march(const scp_grid *g, int kind);
 scp::Bufs<double> bufs_for(scp_grid *g, int kind);
 scp::Bufs32 bufs32_for(scp_grid *g, int kind);
+// initial force + first half kick + drift from xo/vo (upload and device seeding)
+int run_init(scp_grid *g);
 }  // namespace scp_internal
 
 // return the HIP error of x as status -2 with a message

@@ -38,7 +38,9 @@ ABI_SYMBOLS = ["scp_abi_version", "scp_last_error", "scp_grid_bytes", "scp_grid_
                "scp_grid_steps_done", "scp_grid_time_steps", "scp_grid_step_kernel_name",
                "scp_batch_create", "scp_batch_destroy", "scp_batch_add", "scp_batch_remove",
                "scp_batch_size", "scp_batch_step", "scp_batch_step_all", "scp_batch_time_steps",
-               "scp_batch_tune", "scp_batch_set_config", "scp_batch_config"]
+               "scp_batch_tune", "scp_batch_set_config", "scp_batch_config", "scp_grid_seed_balls",
+               "scp_server_create", "scp_server_destroy", "scp_server_submit", "scp_server_round",
+               "scp_server_job", "scp_server_take", "scp_server_stats_get", "scp_server_now"]
 
 
 class ScpError(RuntimeError):
@@ -58,6 +60,41 @@ class Params(C.Structure):
                 ("damp_width", C.c_double), ("damp_rate", C.c_double), ("G_offset", C.c_double),
                 ("qdiag_radius", C.c_double), ("qdiag_probe1", C.c_double),
                 ("qdiag_probe2", C.c_double)]
+
+
+class QBall(C.Structure):
+    """scp_qball of include/scp_amd.h."""
+    _fields_ = [("x0", C.c_double), ("y0", C.c_double), ("z0", C.c_double), ("amp", C.c_double),
+                ("sigma", C.c_double), ("omega", C.c_double), ("phase", C.c_double),
+                ("winding", C.c_int32), ("reserved", C.c_int32)]
+
+
+def qballs(balls):
+    """ctypes array of QBall from dicts with x0,y0,z0,amp,sigma,omega,phase,winding."""
+    arr = (QBall * max(1, len(balls)))()
+    for i, b in enumerate(balls):
+        arr[i] = QBall(**{k: b[k] for k in ("x0", "y0", "z0", "amp", "sigma", "omega", "phase", "winding")})
+    return arr
+
+
+class Job(C.Structure):
+    _fields_ = [("params", Params), ("balls", C.POINTER(QBall)), ("nballs", C.c_int32), ("nsteps", C.c_int32),
+                ("theta_sigma", C.c_double), ("link_sigma", C.c_double), ("seed", C.c_uint64),
+                ("gauss_project", C.c_int32), ("reserved", C.c_int32)]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("nsteps", C.c_int32), ("steps_done", C.c_int64), ("bytes", C.c_int64),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("queued", C.c_int64), ("active", C.c_int64), ("done", C.c_int64), ("failed", C.c_int64),
+                ("bytes_in_use", C.c_int64), ("site_updates", C.c_int64), ("busy_seconds", C.c_double),
+                ("now", C.c_double)]
+
+
+JOB_FAILED, JOB_QUEUED, JOB_ACTIVE, JOB_DONE = -1, 0, 1, 2
 
 
 def make_params(N, mode, L, dt_factor, precision=PREC_F64, **kw):
@@ -126,6 +163,16 @@ def lib():
         L.scp_batch_tune.argtypes = [VP, IP]
         L.scp_batch_set_config.argtypes = [VP, C.c_int, C.c_int]
         L.scp_batch_config.argtypes = [VP, IP, IP, IP]
+        L.scp_grid_seed_balls.argtypes = [VP, C.POINTER(QBall), C.c_int, C.c_double, C.c_double, C.c_uint64]
+        L.scp_server_create.argtypes = [C.c_int64, C.c_int, C.c_int, C.POINTER(VP)]
+        L.scp_server_destroy.argtypes = [VP]
+        L.scp_server_submit.argtypes = [VP, C.POINTER(Job), C.POINTER(C.c_int64)]
+        L.scp_server_round.argtypes = [VP, IP]
+        L.scp_server_job.argtypes = [VP, C.c_int64, C.POINTER(JobInfo)]
+        L.scp_server_take.argtypes = [VP, C.c_int64, DPP, C.c_int, C.POINTER(C.c_double), C.c_int]
+        L.scp_server_stats_get.argtypes = [VP, C.POINTER(ServerStats)]
+        L.scp_server_now.restype = C.c_double
+        L.scp_server_now.argtypes = [VP]
         _typed = True
     return L
 
@@ -177,6 +224,12 @@ class Grid:
     def upload(self, state):
         st = np.ascontiguousarray(np.asarray(state, dtype=np.float64).reshape(ncols(self.mode), self.n3))
         _check(lib().scp_grid_upload(self._h, _col_ptrs(st), st.shape[0]), "scp_grid_upload")
+
+    def seed_balls(self, balls, theta_sigma=0.01, link_sigma=0.0, seed=0):
+        """Seed on the device (scp_grid_seed_balls); balls: list of dicts."""
+        arr = qballs(balls)
+        _check(lib().scp_grid_seed_balls(self._h, arr, len(balls), float(theta_sigma), float(link_sigma),
+                                         int(seed)), "scp_grid_seed_balls")
 
     def download(self):
         st = np.empty((ncols(self.mode), self.n3))
@@ -281,6 +334,71 @@ class Batch:
             lib().scp_batch_destroy(self._h)
             self._h = C.c_void_p()
             self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Server:
+    """Request loop over device-resident grids (scp_server_* of include/scp_amd.h)."""
+
+    def __init__(self, hbm_budget, quantum=200, max_active=0):
+        self._h = C.c_void_p()
+        _check(lib().scp_server_create(int(hbm_budget), int(quantum), int(max_active), C.byref(self._h)),
+               "scp_server_create")
+
+    def submit(self, params, balls, nsteps, theta_sigma=0.01, link_sigma=0.0, seed=0, gauss_project=True):
+        arr = qballs(balls)
+        job = Job(params=params, balls=C.cast(arr, C.POINTER(QBall)), nballs=len(balls), nsteps=int(nsteps),
+                  theta_sigma=float(theta_sigma), link_sigma=float(link_sigma), seed=int(seed),
+                  gauss_project=1 if gauss_project else 0, reserved=0)
+        jid = C.c_int64()
+        _check(lib().scp_server_submit(self._h, C.byref(job), C.byref(jid)), "scp_server_submit")
+        return jid.value
+
+    def round(self):
+        n = C.c_int()
+        _check(lib().scp_server_round(self._h, C.byref(n)), "scp_server_round")
+        return n.value
+
+    def job(self, jid):
+        info = JobInfo()
+        _check(lib().scp_server_job(self._h, int(jid), C.byref(info)), "scp_server_job")
+        return {k: getattr(info, k) for k, _ in JobInfo._fields_}
+
+    def take(self, jid, mode=MODE_GAUGED, n3=None, state=True):
+        """(state or None, diag dict) of a finished job; frees its device memory."""
+        out = None
+        cols = None
+        if state:
+            out = np.empty((ncols(mode), n3))
+            cols = _col_ptrs(out)
+        d = (C.c_double * len(DIAG_NAMES))()
+        _check(lib().scp_server_take(self._h, int(jid), cols, out.shape[0] if state else 0, d, len(DIAG_NAMES)),
+               "scp_server_take")
+        return out, dict(zip(DIAG_NAMES, list(d)))
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().scp_server_stats_get(self._h, C.byref(st)), "scp_server_stats_get")
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def now(self):
+        return lib().scp_server_now(self._h)
+
+    def close(self):
+        if self._h:
+            lib().scp_server_destroy(self._h)
+            self._h = C.c_void_p()
 
     def __del__(self):
         try:
