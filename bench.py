@@ -153,7 +153,10 @@ def sweep_bench(a):
     from scp_amd import grid as G
     from scp_amd import seeds
     rng = np.random.default_rng(a.seed)
-    sizes = rng.choice([32, 48, 64, 80, 96], size=a.grids)
+    if a.sizes == "serve":       # the serving mix: edge log-uniform in [48, 256]
+        sizes = np.round(np.exp(rng.uniform(np.log(48), np.log(256), size=a.grids))).astype(int)
+    else:
+        sizes = rng.choice([32, 48, 64, 80, 96], size=a.grids)
     grids, sites = [], 0
     t0 = time.time()
     for i, N in enumerate(sizes):
@@ -171,6 +174,9 @@ def sweep_bench(a):
         sites += N ** 3
     log(f"{len(grids)} grids, {sites / 1e6:.1f} M sites, set up in {time.time() - t0:.1f}s")
     with G.Batch(grids) as b:
+        b.step(2)
+        ms_default = b.time_steps(a.steps) / a.steps
+        log(f"default config {b.config}: {ms_default:.3f} ms/step")
         taken = b.tune()
         cfg = b.config
         log(f"autotune: {cfg} ({taken} steps)")
@@ -190,12 +196,15 @@ def sweep_bench(a):
            "steps": a.steps, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": f"f32c ({['f32', 'bf16', 'no'][a.lo_words]} compensation words)",
            "data": "synthetic (seeded single Q-ball per grid)",
-           "config": {"workload": f"{len(grids)} gauged grids, edges from {{32,48,64,80,96}}, one Q-ball each",
+           "config": {"workload": f"{len(grids)} gauged grids, edges " +
+                                  ("from {32,48,64,80,96}" if a.sizes == "sweep" else "log-uniform in [48,256]") +
+                                  ", one Q-ball each",
                       "grids": len(grids), "sites": sites, "seed": a.seed, "tuned": cfg},
            "roofline": {"bound": "hbm", "achieved": bytes_per_site("f32c", a.lo_words) * sites / (ms_step * 1e-3) / 1e9,
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": bytes_per_site("f32c", a.lo_words) * sites / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
            "unbatched": {"value": sites / (ms_alone_step * 1e-3) / 1e9, "ms_per_step": ms_alone_step},
+           "untuned_ms_per_step": ms_default,
            "batch_speedup": ms_alone_step / ms_step}
     print(json.dumps(out), flush=True)
 
@@ -306,6 +315,8 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=250.0, help="serve: HBM budget")
     ap.add_argument("--quantum", type=int, default=200, help="serve: steps per scheduling round")
     ap.add_argument("--grids", type=int, default=256, help="sweep: number of grids")
+    ap.add_argument("--sizes", default="sweep", choices=["sweep", "serve"],
+                    help="sweep: edges from {32,48,64,80,96}; serve: edges log-uniform in [48,256]")
     ap.add_argument("--seed", type=int, default=3, help="sweep: seed of sizes and balls")
     ap.add_argument("--copy-calib", type=float, default=0.0,
                     help="instead of the step: time a plain device copy of this many GiB (HBM counter calibration)")

@@ -155,9 +155,10 @@ int  scp_batch_step(scp_batch *b, const int *nsteps, int n);
 int  scp_batch_step_all(scp_batch *b, int nsteps);
 /* scp_batch_step_all timed with HIP events around the graph replay. */
 int  scp_batch_time_steps(scp_batch *b, int nsteps, float *ms_total);
-/* Autotuner: planes per block (16, 32, 64, 128) x launch form (one block per
- * tile, or persistent resident blocks over contiguous tile ranges), each
- * timed on the batch's own steps; every member advances *steps_taken steps. */
+/* Autotuner: planes per block (default, 16, 32, 64, 128) x launch form (one
+ * block per tile, or persistent resident blocks over contiguous tile ranges),
+ * each timed on a 3-step chain of the batch's own steps; every member advances
+ * *steps_taken steps (results do not depend on the configuration). */
 int  scp_batch_tune(scp_batch *b, int *steps_taken);
 /* di = 0: default depth (the deepest of 128, 64, 32, 16 planes that gives
  * >= 4096 blocks over all members); persistent: 0 / 1. */

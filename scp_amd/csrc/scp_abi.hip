@@ -23,10 +23,10 @@ namespace scp_internal {
 
 // K_STEP of the complex and gauged steps runs the LDS-staged march unless the
 // caller asked for the direct-neighbour kernel; the march addresses a field
-// with 32-bit byte offsets, so one field must stay below 4 GiB (N <= 812 in
+// with 32-bit byte offsets, so one field must stay below 4 GiB (N <= ~800 in
 // FP64, N <= 1023 in FP32C); larger grids run the direct-neighbour kernel.
 bool use_march(const scp_grid *g, int kind) {
-    const uint64_t field_bytes = (uint64_t)g->kp.fs * (g->f32 ? 4u : 8u);
+    const uint64_t field_bytes = (uint64_t)g->kp.fsh * (g->f32 ? 4u : 8u);
     return kind == K_STEP && g->p.step_kernel == 0 && (g->step_mode == 1 || g->step_mode == 2) &&
            field_bytes < (1ull << 32);
 }
@@ -79,6 +79,12 @@ static int acc_col_of(int mode, int f) {
 }
 
 static uint32_t field_stride(int64_t n3) { return (uint32_t)((n3 + 63) / 64 * 64); }
+// row pitch of the step state: 32-element rows (128 B in FP32) for the
+// complex and gauged steps; the real step (direct kernel only) keeps N
+static uint32_t step_pitch(int N, int mode) { return mode == 0 ? (uint32_t)N : (uint32_t)((N + 31) / 32 * 32); }
+static uint32_t step_field_stride(int N, int mode) {
+    return field_stride((int64_t)N * N * step_pitch(N, mode));
+}
 
 static int check_params(const scp_params *p) {
     if (!p) return set_err(-1, "null params");
@@ -102,9 +108,10 @@ extern "C" int64_t scp_grid_bytes(const scp_params *p) {
     int64_t n3 = (int64_t)p->N * p->N * p->N;
     int64_t fs = field_stride(n3);
     int NP = np_of_mode(p->mode);
+    const int64_t fh = step_field_stride(p->N, p->mode);
     int64_t b = 2LL * NP * fs * 8;                       // xo, vo
-    if (p->precision == SCP_PREC_F64) b += 3LL * NP * fs * 8;           // P[2], W
-    else b += 3LL * NP * fs * (4 + (p->lo_words == 0 ? 4 : p->lo_words == 1 ? 2 : 0));
+    if (p->precision == SCP_PREC_F64) b += 3LL * NP * fh * 8;           // P[2], W
+    else b += 3LL * NP * fh * (4 + (p->lo_words == 0 ? 4 : p->lo_words == 1 ? 2 : 0));
     if (p->mode == 0 && (p->alpha_cs != 0 || p->beta_h != 0)) b += 6LL * fs * 8;
     b += 1024LL * 32 * 8 + 4096;
     return b;
@@ -118,6 +125,9 @@ static KParams make_kparams(const scp_params &p, int step_mode) {
     int64_t n3 = (int64_t)p.N * p.N * p.N;
     k.n3 = (uint32_t)n3;
     k.fs = field_stride(n3);
+    k.pitch = step_pitch(p.N, p.mode);
+    k.plane = (uint32_t)p.N * k.pitch;
+    k.fsh = step_field_stride(p.N, p.mode);
     const double dx = p.dx, dt = p.dt;
     k.hdt = 0.5 * dt;                        // scp_sim.c:1584
     k.dt = dt;
@@ -165,18 +175,19 @@ extern "C" int scp_grid_create(const scp_params *p, scp_grid **out) {
     g->kp = make_kparams(*p, g->step_mode);
     hipGetDevice(&g->device);
     const size_t fsb = (size_t)g->kp.fs * sizeof(double);
-    const size_t grp = fsb * g->NP;
+    const size_t grp = fsb * g->NP;                                      // xo, vo
+    const size_t grph = (size_t)g->kp.fsh * sizeof(double) * g->NP;      // P, W (padded rows)
     hipError_t e = hipSuccess;
     e = e ? e : hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
     g->f32 = p->precision == SCP_PREC_F32C;
     g->lo_kind = p->lo_words;
     if (!g->f32) {
-        e = e ? e : hipMalloc(&g->P[0], grp);
-        e = e ? e : hipMalloc(&g->P[1], grp);
-        e = e ? e : hipMalloc(&g->W, grp);
+        e = e ? e : hipMalloc(&g->P[0], grph);
+        e = e ? e : hipMalloc(&g->P[1], grph);
+        e = e ? e : hipMalloc(&g->W, grph);
     } else {
-        const size_t g32 = (size_t)g->kp.fs * 4 * g->NP;
-        const size_t glo = (size_t)g->kp.fs * g->NP * (g->lo_kind == LO_F32 ? 4 : 2);
+        const size_t g32 = (size_t)g->kp.fsh * 4 * g->NP;
+        const size_t glo = (size_t)g->kp.fsh * g->NP * (g->lo_kind == LO_F32 ? 4 : 2);
         e = e ? e : hipMalloc(&g->P32[0], g32);
         e = e ? e : hipMalloc(&g->P32[1], g32);
         e = e ? e : hipMalloc(&g->W32, g32);

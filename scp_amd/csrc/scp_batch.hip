@@ -312,8 +312,12 @@ extern "C" int scp_batch_tune(scp_batch *b, int *steps_taken) {
     if (!b) return set_err(-1, "null batch");
     if (steps_taken) *steps_taken = 0;
     if (b->members.empty()) return set_err(-1, "empty batch");
-    static const int DIS[] = {16, 32, 64, 128};
-    std::vector<int> one(b->members.size(), 1);
+    // candidates: the default depth and 16..128 planes, each as one block per
+    // tile and as persistent blocks; each timed on a 3-step chain after one
+    // untimed step (tables, caches)
+    static const int DIS[] = {0, 16, 32, 64, 128};
+    const int REPS = 3;
+    std::vector<int> one(b->members.size(), 1), reps(b->members.size(), REPS);
     float best = 1e30f;
     int bdi = 0, bper = 0, taken = 0;
     for (int per = 0; per < 2; per++) {
@@ -321,14 +325,13 @@ extern "C" int scp_batch_tune(scp_batch *b, int *steps_taken) {
             b->DI = di;
             b->persistent = per;
             b->stale = true;
-            // one untimed launch (table build, caches), then one timed
             int rc = run_chain(b, one.data(), false);
             if (rc) return rc;
-            rc = run_chain(b, one.data(), false, true);
+            rc = run_chain(b, reps.data(), false, true);
             if (rc) return rc;
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-            taken += 2;
+            taken += 1 + REPS;
             if (ms < best) {
                 best = ms;
                 bdi = di;

@@ -30,6 +30,14 @@ struct KParams {
     double L, dx, Rd, DW, DR;  // sponge: R_damp = L - damp_width
     int32_t damp;              // 1 when the sponge is active (bc 0 with width, rate > 0)
     int32_t pad;
+    // Layout of the step state P, W and their compensation words (the arrays
+    // the step streams): rows padded to 32 elements (128 B in FP32), so every
+    // 64-site row segment of a field starts on a cache-line boundary whatever
+    // N is. The emitted state xo/vo and the diagnostics arrays keep the
+    // unpadded layout i*N*N + j*N + k with field stride fs.
+    uint32_t pitch;            // row pitch (elements) of P/W: N rounded up to 32 (N in the real mode)
+    uint32_t plane;            // N * pitch
+    uint32_t fsh;              // field stride of P/W: N * plane rounded up to 64
 };
 
 // Device buffers. Each group is one allocation of nfields * fs elements.

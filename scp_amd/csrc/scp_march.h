@@ -109,18 +109,18 @@ struct Halo {
     static constexpr int NSEG = NF * 2 * KW;              // row segments per plane
     static constexpr int RQ = (NSEG + G::NW - 1) / G::NW; // row segments per wave
     int wave, lane, W;
-    uint32_t rtop, rbot;      // in-plane element offsets of this lane's top/bottom halo cell (half 0)
+    uint32_t rtop, rbot;      // in-plane element offsets (padded rows) of this lane's top/bottom halo cell (half 0)
     int lrow_top, lrow_bot;   // LDS offsets (field 0, half 0) of this lane's halo row cells
     int cf;                   // column cell: field, or -1
     uint32_t cg;              // column cell: in-plane element offset
     int cl;                   // column cell: LDS offset
-    __device__ void init(const MTile &t) {
+    __device__ void init(const MTile &t, uint32_t pitch) {
         const int N = t.N, tid = threadIdx.y * G::TKM + threadIdx.x;
         wave = __builtin_amdgcn_readfirstlane(tid / 64);
         lane = tid & 63;
         W = t.W;
-        rtop = (uint32_t)wrapc(t.j0 - 1, N) * N + t.k0 + lane;
-        rbot = (uint32_t)wrapc(t.j0 + t.H, N) * N + t.k0 + lane;
+        rtop = (uint32_t)wrapc(t.j0 - 1, N) * pitch + t.k0 + lane;
+        rbot = (uint32_t)wrapc(t.j0 + t.H, N) * pitch + t.k0 + lane;
         lrow_top = lane + 1;
         lrow_bot = (t.H + 1) * G::MC + lane + 1;
         cf = -1; cg = 0; cl = 0;
@@ -130,7 +130,7 @@ struct Halo {
                 cf = f;
                 const int gj = wrapc(t.j0 - 1 + r, N);
                 const int ck = side ? wrapc(t.k0 + t.W, N) : wrapc(t.k0 - 1, N);
-                cg = (uint32_t)gj * N + ck;
+                cg = (uint32_t)gj * pitch + ck;
                 cl = f * G::MPL + r * G::MC + (side ? t.W + 1 : 0);
             }
         }
@@ -212,12 +212,13 @@ struct S64 {
     using B = Bufs<double>;
     struct Own { double w; };
     __device__ static const double *pin(const B &b) { return b.pin; }
-    __device__ static Own own_load(const B &b, int f, uint32_t fs, uint32_t e) {
-        return {ldo_nt(fbase(b.w, f, fs), e)};
+    __device__ static Own own_load(const B &b, int f, uint32_t fh, uint32_t e) {
+        return {ldo_nt(fbase(b.w, f, fh), e)};
     }
+    // fh, e: field stride and index of the step state; fs, ec: of the emitted state
     template <int MODE, bool EMIT>
-    __device__ static void update(const KParams &kp, const B &b, int f, uint32_t fs, uint32_t e, double a,
-                                  double x, Own s, bool damp, double dmp) {
+    __device__ static void update(const KParams &kp, const B &b, int f, uint32_t fh, uint32_t e, uint32_t fs,
+                                  uint32_t ec, double a, double x, Own s, bool damp, double dmp) {
         double v = s.w;
         v += kp.hdt * a;                               // stage 4 (C:1680-1696)
         if (damp) v *= dmp;                            // stage 5 (C:1454-1460)
@@ -229,11 +230,11 @@ struct S64 {
         } else {
             xn = x + kp.dt * w;                        // next stage 2 (C:1629-1638)
         }
-        sto_nt(fbase(b.w, f, fs), e, w);
-        sto_nt(fbase(b.pout, f, fs), e, xn);
+        sto_nt(fbase(b.w, f, fh), e, w);
+        sto_nt(fbase(b.pout, f, fh), e, xn);
         if constexpr (EMIT) {
-            __builtin_nontemporal_store(x, &b.xo[(size_t)f * fs + e]);
-            __builtin_nontemporal_store(v, &b.vo[(size_t)f * fs + e]);
+            __builtin_nontemporal_store(x, &b.xo[(size_t)f * fs + ec]);
+            __builtin_nontemporal_store(v, &b.vo[(size_t)f * fs + ec]);
         }
     }
 };
@@ -271,12 +272,13 @@ struct S32 {
     using B = Bufs32;
     struct Own { float wh, wl, xl; };
     __device__ static const float *pin(const B &b) { return b.pin; }
-    __device__ static Own own_load(const B &b, int f, uint32_t fs, uint32_t e) {
-        return {ldo_nt(fbase(b.w, f, fs), e), LoIO<LOK>::get(b.wlo, f, fs, e), LoIO<LOK>::get(b.pinlo, f, fs, e)};
+    __device__ static Own own_load(const B &b, int f, uint32_t fh, uint32_t e) {
+        return {ldo_nt(fbase(b.w, f, fh), e), LoIO<LOK>::get(b.wlo, f, fh, e), LoIO<LOK>::get(b.pinlo, f, fh, e)};
     }
+    // fh, e: field stride and index of the step state; fs, ec: of the emitted state
     template <int MODE, bool EMIT>
-    __device__ static void update(const KParams &kp, const B &b, int f, uint32_t fs, uint32_t e, float a,
-                                  float xh, Own s, bool damp, double dmpd) {
+    __device__ static void update(const KParams &kp, const B &b, int f, uint32_t fh, uint32_t e, uint32_t fs,
+                                  uint32_t ec, float a, float xh, Own s, bool damp, double dmpd) {
         const DF hdt = df_split(kp.hdt), dt = df_split(kp.dt), gad = df_split(kp.gad);
         float wh = s.wh, wl = s.wl, vh = 0.0f, vl = 0.0f;
         // both half kicks, every step (also between emitted steps): the result then
@@ -297,13 +299,13 @@ struct S32 {
         } else {
             comp_add(xh, xl, dt, wh, wl);                   // next stage 2
         }
-        sto_nt(fbase(b.w, f, fs), e, wh);
-        LoIO<LOK>::put(b.wlo, f, fs, e, wl);
-        sto_nt(fbase(b.pout, f, fs), e, xh);
-        LoIO<LOK>::put(b.poutlo, f, fs, e, xl);
+        sto_nt(fbase(b.w, f, fh), e, wh);
+        LoIO<LOK>::put(b.wlo, f, fh, e, wl);
+        sto_nt(fbase(b.pout, f, fh), e, xh);
+        LoIO<LOK>::put(b.poutlo, f, fh, e, xl);
         if constexpr (EMIT) {
-            __builtin_nontemporal_store((double)xh0 + (double)xl0, &b.xo[(size_t)f * fs + e]);
-            __builtin_nontemporal_store((double)vh + (double)vl, &b.vo[(size_t)f * fs + e]);
+            __builtin_nontemporal_store((double)xh0 + (double)xl0, &b.xo[(size_t)f * fs + ec]);
+            __builtin_nontemporal_store((double)vh + (double)vl, &b.vo[(size_t)f * fs + ec]);
         }
     }
 };
@@ -334,10 +336,13 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
     const MTile t = mtile<KW>(N, DI, tk, tj, seg);
     if (t.i0 >= N) return;                    // whole block: uniform
     Halo<NF, KW> hs;
-    hs.init(t);
-    const uint32_t fs = kp.fs, NN = (uint32_t)N * (uint32_t)N;
+    hs.init(t, kp.pitch);
+    // step state: padded rows (kp.pitch), plane stride kp.plane, field stride kp.fsh;
+    // emitted state: unpadded (N, N*N, kp.fs)
+    const uint32_t fs = kp.fsh, NN = kp.plane;
     const T *__restrict__ pin = S::pin(b);
-    const uint32_t col = t.valid ? (uint32_t)t.j * (uint32_t)N + (uint32_t)t.k : 0u;
+    const uint32_t col = t.valid ? (uint32_t)t.j * kp.pitch + (uint32_t)t.k : 0u;
+    const uint32_t colc = t.valid ? (uint32_t)t.j * (uint32_t)N + (uint32_t)t.k : 0u;
     auto ld = [&](int f, uint32_t e) -> T { return ldo(fbase(pin, f, fs), e); };
     // halo loads of one plane: row pairs (wave-uniform field) and this thread's column cell
     auto halo_rows = [&](uint32_t pe, T *hr) {
@@ -379,8 +384,8 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
 #pragma unroll
             for (int f = 0; f < NF; f++) { c0[f] = ld(f, e0); cp[f] = ld(f, ep); }
             if constexpr (MODE == 2) {
-                const uint32_t jpo = (uint32_t)im0 * NN + (uint32_t)wp(t.j, N) * N + t.k;
-                const uint32_t kpo = (uint32_t)im0 * NN + (uint32_t)t.j * N + wp(t.k, N);
+                const uint32_t jpo = (uint32_t)im0 * NN + (uint32_t)wp(t.j, N) * kp.pitch + t.k;
+                const uint32_t kpo = (uint32_t)im0 * NN + (uint32_t)t.j * kp.pitch + wp(t.k, N);
                 gauge_prologue(ld(F_TH + 0, em), ld(F_TH + 1, em), ld(F_TH + 2, em), c0[F_TH + 1], c0[F_TH + 2],
                                ld(F_TH + 0, jpo), ld(F_TH + 0, kpo), cy);
             }
@@ -407,6 +412,7 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
         const int nbuf = (bb + 1 == MNBUF) ? 0 : bb + 1;
         const uint32_t p2e = (uint32_t)wp(wp(i, N), N) * NN;
         const uint32_t e = (uint32_t)i * NN + col;
+        const uint32_t ec = (uint32_t)i * (uint32_t)N * (uint32_t)N + colc;
         // prefetch plane i+2 (own column + halo) and this plane's own-site velocity words
         T cpp[NF], hrr[RQ];
         typename S::Own sw[NF];
@@ -439,17 +445,17 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
                 if (kp.damp && sponge(kp, i, t.j, t.k, dmp)) {
 #pragma unroll
                     for (int f = 0; f < NF; f++)
-                        S::template update<MODE, EMIT>(kp, b, f, fs, e, acc[f], c0[f], sw[f], true, dmp);
+                        S::template update<MODE, EMIT>(kp, b, f, fs, e, kp.fs, ec, acc[f], c0[f], sw[f], true, dmp);
                 } else {
 #pragma unroll
                     for (int f = 0; f < NF; f++)
-                        S::template update<MODE, EMIT>(kp, b, f, fs, e, acc[f], c0[f], sw[f], false, 1.0);
+                        S::template update<MODE, EMIT>(kp, b, f, fs, e, kp.fs, ec, acc[f], c0[f], sw[f], false, 1.0);
                 }
             } else {
                 const bool damp = kp.damp && sponge(kp, i, t.j, t.k, dmp);
 #pragma unroll
                 for (int f = 0; f < NF; f++)
-                    S::template update<MODE, EMIT>(kp, b, f, fs, e, acc[f], c0[f], sw[f], damp, dmp);
+                    S::template update<MODE, EMIT>(kp, b, f, fs, e, kp.fs, ec, acc[f], c0[f], sw[f], damp, dmp);
             }
         };
         if constexpr (S::REGQ) {

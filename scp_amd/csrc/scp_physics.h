@@ -107,13 +107,15 @@ struct GlobalNB {
     LD X;
     int N, i, j, k;
     uint32_t jp, jm, kp, km;
+    uint32_t rs, ps;   // row and plane strides of the source array
+    __device__ uint32_t at(int a, int b, int c) const { return (uint32_t)a * ps + (uint32_t)b * rs + (uint32_t)c; }
     __device__ R face(int f, int d, int sg) const {
         return X(f, d == 1 ? (sg > 0 ? jp : jm) : (sg > 0 ? kp : km));
     }
-    __device__ R th1_jm_kp() const { return X(F_TH + 1, sidx(N, i, wm(j, N), wp(k, N))); }
-    __device__ R th2_jp_km() const { return X(F_TH + 2, sidx(N, i, wp(j, N), wm(k, N))); }
-    __device__ R th1_ip_jm() const { return X(F_TH + 1, sidx(N, wp(i, N), wm(j, N), k)); }
-    __device__ R th2_ip_km() const { return X(F_TH + 2, sidx(N, wp(i, N), j, wm(k, N))); }
+    __device__ R th1_jm_kp() const { return X(F_TH + 1, at(i, wm(j, N), wp(k, N))); }
+    __device__ R th2_jp_km() const { return X(F_TH + 2, at(i, wp(j, N), wm(k, N))); }
+    __device__ R th1_ip_jm() const { return X(F_TH + 1, at(wp(i, N), wm(j, N), k)); }
+    __device__ R th2_ip_km() const { return X(F_TH + 2, at(wp(i, N), j, wm(k, N))); }
 };
 
 // Complex-mode accelerations at one site (compute_forces_complex, C:931-994).

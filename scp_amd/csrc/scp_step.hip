@@ -32,17 +32,22 @@ k_step64(KParams kp, Bufs<double> b, int DI) {
     const int N = kp.N;
     const Geo g = geometry(N, DI);
     if (!g.valid || g.i0 >= N) return;
-    const uint32_t fs = kp.fs;
-    const double *__restrict__ src = (KIND == K_STEP) ? b.pin : b.xo;
-    auto X = [&](int f, uint32_t idx) -> double { return src[(size_t)f * fs + idx]; };
-    auto AUX = [&](int f, uint32_t idx) -> double { return b.aux[(size_t)f * fs + idx]; };
+    // the stencil source: P (padded rows, kp.pitch) when stepping, the emitted
+    // state xo (unpadded) at INIT / FORCES; outputs P', W hot, xo/vo/acc cold
+    const uint32_t fs = kp.fs, fh = kp.fsh;
+    const bool hot = (KIND == K_STEP);
+    const uint32_t RS = hot ? kp.pitch : (uint32_t)N, PS = hot ? kp.plane : (uint32_t)N * N, SF = hot ? fh : fs;
+    const double *__restrict__ src = hot ? b.pin : b.xo;
+    auto X = [&](int f, uint32_t idx) -> double { return src[(size_t)f * SF + idx]; };
+    auto IX = [&](int a, int bb, int c) -> uint32_t { return (uint32_t)a * PS + (uint32_t)bb * RS + (uint32_t)c; };
+    auto AUX = [&](int f, uint32_t idx) -> double { return b.aux[(size_t)f * fs + idx]; };   // real mode: hot == cold
     const int j = g.j, k = g.k;
     const int jp1 = wp(j, N), jm1 = wm(j, N), kp1 = wp(k, N), km1 = wm(k, N);
 
     double cm[NP], c0[NP], cp[NP];
     {
         const int im1 = wm(g.i0, N);
-        const uint32_t a = sidx(N, im1, j, k), c = sidx(N, g.i0, j, k);
+        const uint32_t a = IX(im1, j, k), c = IX(g.i0, j, k);
 #pragma unroll
         for (int f = 0; f < NP; f++) { cm[f] = X(f, a); c0[f] = X(f, c); }
     }
@@ -50,41 +55,43 @@ k_step64(KParams kp, Bufs<double> b, int DI) {
     if constexpr (MODE == 2) {
         const int im0 = wm(g.i0, N);
         gauge_prologue(cm[F_TH + 0], cm[F_TH + 1], cm[F_TH + 2], c0[F_TH + 1], c0[F_TH + 2],
-                       X(F_TH + 0, sidx(N, im0, jp1, k)), X(F_TH + 0, sidx(N, im0, j, kp1)), cy);
+                       X(F_TH + 0, IX(im0, jp1, k)), X(F_TH + 0, IX(im0, j, kp1)), cy);
     }
 
     for (int i = g.i0; i < g.i1; i++) {
         const int ip1 = wp(i, N), im1 = wm(i, N);
-        const uint32_t ci = sidx(N, i, j, k);
+        const uint32_t ci = IX(i, j, k);                          // source index
+        const uint32_t cc = sidx(N, i, j, k);                     // cold (xo, vo, acc)
+        const uint32_t chh = (uint32_t)i * kp.plane + (uint32_t)j * kp.pitch + (uint32_t)k;   // hot (P', W)
         {
-            const uint32_t a = sidx(N, ip1, j, k);
+            const uint32_t a = IX(ip1, j, k);
 #pragma unroll
             for (int f = 0; f < NP; f++) cp[f] = X(f, a);
         }
-        const uint32_t jp = sidx(N, i, jp1, k), jm = sidx(N, i, jm1, k);
-        const uint32_t kpx = sidx(N, i, j, kp1), kmx = sidx(N, i, j, km1);
+        const uint32_t jp = IX(i, jp1, k), jm = IX(i, jm1, k);
+        const uint32_t kpx = IX(i, j, kp1), kmx = IX(i, j, km1);
         double acc[NP];
-        const GlobalNB<double, decltype(X)> nb{X, N, i, j, k, jp, jm, kpx, kmx};
+        const GlobalNB<double, decltype(X)> nb{X, N, i, j, k, jp, jm, kpx, kmx, RS, PS};
         if constexpr (MODE == 2)
             forces_gauge(kp, c0, cp, cm, nb, cy, acc);
         else if constexpr (MODE == 1)
             forces_complex(kp, c0, cp, cm, nb, acc);
         else
-            forces_real(kp, c0, cp, cm, X, AUX, ci, sidx(N, ip1, j, k), sidx(N, im1, j, k), jp, jm,
+            forces_real(kp, c0, cp, cm, X, AUX, ci, IX(ip1, j, k), IX(im1, j, k), jp, jm,
                         kpx, kmx, acc);
 
         if constexpr (KIND == K_FORCES) {
 #pragma unroll
-            for (int f = 0; f < NP; f++) b.acc[(size_t)f * fs + ci] = acc[f];
+            for (int f = 0; f < NP; f++) b.acc[(size_t)f * fs + cc] = acc[f];
         } else {
             double dmp = 1.0;
             const bool damp = (KIND == K_STEP) && kp.damp && sponge(kp, i, j, k, dmp);
 #pragma unroll
             for (int f = 0; f < NP; f++) {
-                const size_t o = (size_t)f * fs + ci;
+                const size_t o = (size_t)f * fh + chh, oc = (size_t)f * fs + cc;
                 double v;
                 if constexpr (KIND == K_INIT) {
-                    v = b.vo[o];
+                    v = b.vo[oc];
                 } else {
                     v = __builtin_nontemporal_load(&b.w[o]);
                     v += kp.hdt * acc[f];                  // stage 4 (C:1680-1696)
@@ -102,8 +109,8 @@ k_step64(KParams kp, Bufs<double> b, int DI) {
                 __builtin_nontemporal_store(w, &b.w[o]);
                 __builtin_nontemporal_store(xn, &b.pout[o]);
                 if constexpr (EMIT && KIND == K_STEP) {
-                    __builtin_nontemporal_store(x, &b.xo[o]);
-                    __builtin_nontemporal_store(v, &b.vo[o]);
+                    __builtin_nontemporal_store(x, &b.xo[oc]);
+                    __builtin_nontemporal_store(v, &b.vo[oc]);
                 }
             }
         }

@@ -88,19 +88,24 @@ k_step32(KParams kp, Bufs32 b, int DI) {
     const int N = kp.N;
     const Geo g = geometry(N, DI);
     if (!g.valid || g.i0 >= N) return;
-    const uint32_t fs = kp.fs;
-    // stencil source: FP32 hi words, or the FP64 host state rounded (INIT / FORCES)
+    // stencil source: FP32 hi words of P (padded rows, kp.pitch), or the FP64
+    // emitted state rounded (INIT / FORCES, unpadded); outputs P', W and their
+    // words hot, xo/vo/acc cold
+    const uint32_t fs = kp.fs, fh = kp.fsh;
+    constexpr bool hot = (KIND == K_STEP);
+    const uint32_t RS = hot ? kp.pitch : (uint32_t)N, PS = hot ? kp.plane : (uint32_t)N * N;
     auto X = [&](int f, uint32_t idx) -> float {
-        if constexpr (KIND == K_STEP) return b.pin[(size_t)f * fs + idx];
+        if constexpr (KIND == K_STEP) return b.pin[(size_t)f * fh + idx];
         else return (float)b.xo[(size_t)f * fs + idx];
     };
-    auto AUX = [&](int f, uint32_t idx) -> float { return (float)b.aux[(size_t)f * fs + idx]; };
+    auto IX = [&](int a, int bb, int c) -> uint32_t { return (uint32_t)a * PS + (uint32_t)bb * RS + (uint32_t)c; };
+    auto AUX = [&](int f, uint32_t idx) -> float { return (float)b.aux[(size_t)f * fs + idx]; };   // real mode: hot == cold
     const int j = g.j, k = g.k;
     const int jp1 = wp(j, N), jm1 = wm(j, N), kp1 = wp(k, N), km1 = wm(k, N);
 
     float cm[NP], c0[NP], cp[NP];
     {
-        const uint32_t a = sidx(N, wm(g.i0, N), j, k), c = sidx(N, g.i0, j, k);
+        const uint32_t a = IX(wm(g.i0, N), j, k), c = IX(g.i0, j, k);
 #pragma unroll
         for (int f = 0; f < NP; f++) { cm[f] = X(f, a); c0[f] = X(f, c); }
     }
@@ -108,40 +113,42 @@ k_step32(KParams kp, Bufs32 b, int DI) {
     if constexpr (MODE == 2) {
         const int im0 = wm(g.i0, N);
         gauge_prologue(cm[F_TH + 0], cm[F_TH + 1], cm[F_TH + 2], c0[F_TH + 1], c0[F_TH + 2],
-                       X(F_TH + 0, sidx(N, im0, jp1, k)), X(F_TH + 0, sidx(N, im0, j, kp1)), cy);
+                       X(F_TH + 0, IX(im0, jp1, k)), X(F_TH + 0, IX(im0, j, kp1)), cy);
     }
     const DF hdt = df_split(kp.hdt), dt = df_split(kp.dt), gad = df_split(kp.gad);
 
     for (int i = g.i0; i < g.i1; i++) {
         const int ip1 = wp(i, N), im1 = wm(i, N);
-        const uint32_t ci = sidx(N, i, j, k);
+        const uint32_t ci = IX(i, j, k);                          // source index
+        const uint32_t cc = sidx(N, i, j, k);                     // cold (xo, vo, acc)
+        const uint32_t chh = (uint32_t)i * kp.plane + (uint32_t)j * kp.pitch + (uint32_t)k;   // hot (P', W, words)
         {
-            const uint32_t a = sidx(N, ip1, j, k);
+            const uint32_t a = IX(ip1, j, k);
 #pragma unroll
             for (int f = 0; f < NP; f++) cp[f] = X(f, a);
         }
-        const uint32_t jp = sidx(N, i, jp1, k), jm = sidx(N, i, jm1, k);
-        const uint32_t kpx = sidx(N, i, j, kp1), kmx = sidx(N, i, j, km1);
+        const uint32_t jp = IX(i, jp1, k), jm = IX(i, jm1, k);
+        const uint32_t kpx = IX(i, j, kp1), kmx = IX(i, j, km1);
         float acc[NP];
-        const GlobalNB<float, decltype(X)> nb{X, N, i, j, k, jp, jm, kpx, kmx};
+        const GlobalNB<float, decltype(X)> nb{X, N, i, j, k, jp, jm, kpx, kmx, RS, PS};
         if constexpr (MODE == 2)
             forces_gauge(kp, c0, cp, cm, nb, cy, acc);
         else if constexpr (MODE == 1)
             forces_complex(kp, c0, cp, cm, nb, acc);
         else
-            forces_real(kp, c0, cp, cm, X, AUX, ci, sidx(N, ip1, j, k), sidx(N, im1, j, k), jp, jm,
+            forces_real(kp, c0, cp, cm, X, AUX, ci, IX(ip1, j, k), IX(im1, j, k), jp, jm,
                         kpx, kmx, acc);
 
         if constexpr (KIND == K_FORCES) {
 #pragma unroll
-            for (int f = 0; f < NP; f++) b.acc[(size_t)f * fs + ci] = (double)acc[f];
+            for (int f = 0; f < NP; f++) b.acc[(size_t)f * fs + cc] = (double)acc[f];
         } else if constexpr (KIND == K_INIT) {
             // first half kick and drift in FP64 from the FP64 state, then split
 #pragma unroll
             for (int f = 0; f < NP; f++) {
-                const size_t o = (size_t)f * fs + ci;
-                double w = b.vo[o] + kp.hdt * (double)acc[f];
-                double x = b.xo[o], xn;
+                const size_t o = (size_t)f * fh + chh, oc = (size_t)f * fs + cc;
+                double w = b.vo[oc] + kp.hdt * (double)acc[f];
+                double x = b.xo[oc], xn;
                 if (MODE == 2 && f >= F_TH) {
                     double t = x + kp.gad * w;
                     xn = t - 2.0 * SCP_PI * rint(t / (2.0 * SCP_PI));
@@ -164,7 +171,7 @@ k_step32(KParams kp, Bufs32 b, int DI) {
             }
 #pragma unroll
             for (int f = 0; f < NP; f++) {
-                const size_t o = (size_t)f * fs + ci;
+                const size_t o = (size_t)f * fh + chh, oc = (size_t)f * fs + cc;
                 float wh = __builtin_nontemporal_load(&b.w[o]), wl = LS::get(b.wlo, o);
                 float vh = 0.0f, vl = 0.0f;
                 // both half kicks, every step (also between emitted steps): the result then
@@ -190,8 +197,8 @@ k_step32(KParams kp, Bufs32 b, int DI) {
                 __builtin_nontemporal_store(xh, &b.pout[o]);
                 LS::put(b.poutlo, o, xl);
                 if constexpr (EMIT) {
-                    __builtin_nontemporal_store((double)xh0 + (double)xl0, &b.xo[o]);
-                    __builtin_nontemporal_store((double)vh + (double)vl, &b.vo[o]);
+                    __builtin_nontemporal_store((double)xh0 + (double)xl0, &b.xo[oc]);
+                    __builtin_nontemporal_store((double)vh + (double)vl, &b.vo[oc]);
                 }
             }
         }
