@@ -241,7 +241,7 @@ def serve_bench(a):
     offered = sum(s_["N"] ** 3 * s_["steps"] for s_ in specs)
     log(f"{len(specs)} jobs over {a.duration:.0f}s, offered {offered / 1e9:.1f} G site-updates")
     lat, done_sites = [], 0
-    with G.Server(int(a.budget_gb * 1e9), quantum=a.quantum) as srv:
+    with G.Server(int(a.budget_gb * 1e9), quantum=a.quantum, max_active=a.max_active) as srv:
         pending, i, next_log = {}, 0, 10.0
         while True:
             now = srv.now()
@@ -283,7 +283,8 @@ def serve_bench(a):
            "data": "synthetic (Poisson arrivals, seeded Q-ball jobs)",
            "config": {"workload": "gauged jobs, edge log-uniform [48,256], 1-4 Q-balls", "rate_per_s": a.rate,
                       "duration_s": a.duration, "horizon_s": a.horizon, "steps": [a.steps_lo, a.steps_hi],
-                      "hbm_budget_gb": a.budget_gb, "quantum": a.quantum, "seed": a.seed},
+                      "hbm_budget_gb": a.budget_gb, "quantum": a.quantum, "max_active": a.max_active,
+                      "seed": a.seed},
            "jobs": {"submitted": i, "completed": completed, "unfinished": i - completed},
            "latency_s": {"p50": float(np.nanpercentile(lat, 50)), "p99": float(np.nanpercentile(lat, 99)),
                          "max": float(np.nanmax(lat))},
@@ -314,6 +315,7 @@ def main():
     ap.add_argument("--steps-hi", type=int, default=20000)
     ap.add_argument("--budget-gb", type=float, default=250.0, help="serve: HBM budget")
     ap.add_argument("--quantum", type=int, default=200, help="serve: steps per scheduling round")
+    ap.add_argument("--max-active", type=int, default=0, help="serve: concurrent jobs limit (0: memory only)")
     ap.add_argument("--grids", type=int, default=256, help="sweep: number of grids")
     ap.add_argument("--sizes", default="sweep", choices=["sweep", "serve"],
                     help="sweep: edges from {32,48,64,80,96}; serve: edges log-uniform in [48,256]")
