@@ -134,14 +134,33 @@ def run_one(G, N, L, st, prec, lo_words, steps, warmup):
         wall = time.time() - w0
         d = g.diag()
         kernel = g.step_kernel_name
+        diag_ms = diag_step_timing(g)
     ms_step = ms / steps
     sites = float(N) ** 3
     bps = bytes_per_site(prec, lo_words)
     achieved = bps * sites / (ms_step * 1e-3) / 1e9
     log(f"{prec}: {steps} steps {ms:.2f} ms (wall {wall * 1e3:.1f} ms) -> {ms_step:.3f} ms/step; "
         f"E_total={d['E_total']:.9e} Q_total={d['Q_total']:.6e}")
+    log(f"{prec}: diag step (wall, ms): step alone {diag_ms['step']:.2f}, step + stand-alone diagnostics "
+        f"{diag_ms['separate']:.2f}, diagnostics fused into the step {diag_ms['fused']:.2f}")
     return {"ms_per_step": ms_step, "value": sites / (ms_step * 1e-3) / 1e9, "kernel": kernel,
-            "bytes_per_site": bps, "achieved": achieved, "traffic": measured_traffic(kernel, N)}
+            "bytes_per_site": bps, "achieved": achieved, "traffic": measured_traffic(kernel, N),
+            "diag_step_ms": diag_ms}
+
+
+def diag_step_timing(g, reps=3):
+    """Wall time (synchronised, median of reps) of one step, of one step followed
+    by the stand-alone diagnostics pass, and of one step with the diagnostics
+    fused into its launch (scp_grid_step_diag): the cost of a diag.tsv row."""
+    def med(fn):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return float(np.median(ts))
+    return {"step": med(lambda: g.step(1)), "separate": med(lambda: (g.step(1), g.diag())),
+            "fused": med(lambda: g.step_diag(1))}
 
 
 def sweep_bench(a):
@@ -363,11 +382,12 @@ def main():
                      "frac": head["achieved"] / HBM_PEAK_GBS, "traffic": head["traffic"],
                      "kernel": head["kernel"], "bytes_per_site": head["bytes_per_site"]},
         "cpu_baseline": cpu,
+        "diag_step_ms": head["diag_step_ms"],
     }
     if cmp64:
         out["fp64"] = {"value": cmp64["value"], "ms_per_step": cmp64["ms_per_step"],
                        "roofline_frac": cmp64["achieved"] / HBM_PEAK_GBS, "kernel": cmp64["kernel"],
-                       "traffic": cmp64["traffic"]}
+                       "traffic": cmp64["traffic"], "diag_step_ms": cmp64["diag_step_ms"]}
         out["speedup_vs_fp64"] = head["value"] / cmp64["value"]
     print(json.dumps(out), flush=True)
 

@@ -95,6 +95,15 @@ int  scp_grid_download(scp_grid *g, double *const *cols, int ncols);
 /* Advance nsteps leapfrog steps (nsteps >= 1). */
 int  scp_grid_step(scp_grid *g, int nsteps);
 int  scp_grid_diag(scp_grid *g, double *out, int nout);
+/* scp_grid_step followed by scp_grid_diag of the resulting state, with the
+ * diagnostics fused into the step's last launch (complex and gauged modes on
+ * the LDS-staged march: every energy, charge and moment sum is reduced by the
+ * step kernel itself; the Gauss-law statistics take one light pass over E).
+ * Replaces the main loop's verlet_step + diagnostics pair on a diag step
+ * (scp_sim.c:3082 and 2993-3065; scp_sim.cu:5492, 3697-4041). Other grids
+ * run the step and the stand-alone pass. The first call on a gauged grid
+ * allocates N^3 doubles of charge density. */
+int  scp_grid_step_diag(scp_grid *g, int nsteps, double *out, int nout);
 /* Accelerations of the current positions: 6/12/15 columns in the order
  * phi_acc, theta_acc, [phi_im_acc, theta_im_acc], [E_acc]. */
 int  scp_grid_forces(scp_grid *g, double *const *acc, int nacc);

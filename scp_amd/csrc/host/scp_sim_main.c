@@ -345,11 +345,15 @@ int main(int argc, char **argv) {
         const int do_diag = step % diag_every == 0;
         pending++;
         if (!do_snap && !do_diag && step != n_steps) continue;
-        if (scp_grid_step(x.g, pending)) die_abi("step");
+        /* a diag step takes the diagnostics fused into its last launch */
+        if (do_diag) {
+            if (scp_grid_step_diag(x.g, pending, d, SCP_D_COUNT)) die_abi("step");
+        } else if (scp_grid_step(x.g, pending)) {
+            die_abi("step");
+        }
         pending = 0;
         if (do_snap) snap(&x, t);
         if (do_diag) {
-            if (scp_grid_diag(x.g, d, SCP_D_COUNT)) die_abi("diag");
             ROW(t);
             fflush(fp);
             if (step % major == 0) {

@@ -153,6 +153,7 @@ static void free_grid(scp_grid *g) {
     hipFree(g->P32[0]); hipFree(g->P32[1]); hipFree(g->W32);
     hipFree(g->Plo[0]); hipFree(g->Plo[1]); hipFree(g->Wlo);
     hipFree(g->aux); hipFree(g->acc); hipFree(g->partials); hipFree(g->dout); hipFree(g->argmin);
+    hipFree(g->rho); hipFree(g->fpart);
     if (g->ev0) hipEventDestroy(g->ev0);
     if (g->ev1) hipEventDestroy(g->ev1);
     if (g->stream) hipStreamDestroy(g->stream);
@@ -338,7 +339,9 @@ extern "C" int scp_grid_download(scp_grid *g, double *const *cols, int ncols) {
     return 0;
 }
 
-static int enqueue_steps(scp_grid *g, int nsteps) {
+// nsteps launches, the last emitting; da: that launch also accumulates the
+// diagnostics (march kernels of the complex and gauged modes only)
+static int enqueue_steps(scp_grid *g, int nsteps, const DiagArgs *da = nullptr) {
     for (int s = 0; s < nsteps; s++) {
         int rc = 0;
         if (g->aux) {
@@ -347,7 +350,13 @@ static int enqueue_steps(scp_grid *g, int nsteps) {
             if (rc) return rc;
         }
         const bool emit = (s == nsteps - 1);
-        rc = launch_kind(g, K_STEP, emit);
+        if (emit && da)
+            rc = g->f32 ? kerr(launch_march_f32c_diag(g->step_mode, g->lo_kind, g->kp, bufs32_for(g, K_STEP), 0, *da,
+                                                      g->stream), "march32 diag")
+                        : kerr(launch_march_f64_diag(g->step_mode, g->kp, bufs_for(g, K_STEP), 0, *da, g->stream),
+                               "march64 diag");
+        else
+            rc = launch_kind(g, K_STEP, emit);
         if (rc) return rc;
         g->cur ^= 1;
     }
@@ -394,14 +403,9 @@ static int fetch(scp_grid *g, double *host, const double *dev, size_t n) {
     return 0;
 }
 
-extern "C" int scp_grid_diag(scp_grid *g, double *out, int nout) {
-    if (!g || !out) return set_err(-1, "null argument");
-    if (nout < SCP_D_COUNT) return set_err(-1, "nout=%d < %d", nout, SCP_D_COUNT);
-    for (int q = 0; q < nout; q++) out[q] = 0.0;
+static DiagParams diag_params(const scp_grid *g) {
     const scp_params &p = g->p;
-    const KParams &kp = g->kp;
     const double dx = p.dx, dV = dx * dx * dx;
-    const int64_t n3 = kp.n3;
     DiagParams dp;
     memset(&dp, 0, sizeof dp);
     dp.R2core = p.qdiag_radius * p.qdiag_radius;
@@ -413,14 +417,20 @@ extern "C" int scp_grid_diag(scp_grid *g, double *out, int nout) {
     dp.has_gauge = g->mode == 2;
     dp.BPRE = dp.gauged ? 1.0 / (p.g * p.g * dx * dx * dx * dx) : 0.0;
     dp.dV = dV;
+    return dp;
+}
+
+// out[] from the reduced pass-1 sums S and maxima M of the emitted state: the
+// totals, then the core moments and argmax voxel (second pass) and the probes
+static int diag_finish(scp_grid *g, const DiagParams &dp, const double *S, const double *M, double *out,
+                       int nout) {
+    for (int q = 0; q < nout; q++) out[q] = 0.0;
+    const scp_params &p = g->p;
+    const KParams &kp = g->kp;
+    const double dx = p.dx, dV = dx * dx * dx;
+    const int64_t n3 = kp.n3;
     const int nb = g->nblocks_diag;
-    int rc = kerr(launch_diag_pass1(g->mode, kp, dp, g->xo, g->vo, g->partials, nb, g->stream), "diag1");
-    if (rc) return rc;
-    rc = kerr(launch_reduce(g->partials, nb, DS_NSUM, DM_NMAX, g->dout, g->stream), "reduce");
-    if (rc) return rc;
-    double r[DS_NSUM + DM_NMAX];
-    if ((rc = fetch(g, r, g->dout, DS_NSUM + DM_NMAX))) return rc;
-    const double *S = r, *M = r + DS_NSUM;
+    int rc = 0;
     out[SCP_D_E_PHI_KIN] = S[DS_EPK]; out[SCP_D_E_THETA_KIN] = S[DS_ETK];
     out[SCP_D_E_GRAD] = S[DS_EG]; out[SCP_D_E_MASS] = S[DS_EM]; out[SCP_D_E_POT] = S[DS_EP];
     out[SCP_D_E_TGRAD] = S[DS_ETG]; out[SCP_D_E_TMASS] = S[DS_ETM]; out[SCP_D_E_COUPLING] = S[DS_EC];
@@ -489,6 +499,61 @@ extern "C" int scp_grid_diag(scp_grid *g, double *out, int nout) {
         out[SCP_D_Q_FLUX] = (p.g != 0) ? (S[DS_FSUM] - p.G_offset * S[DS_NC] * dV) / p.g : 0.0;
     }
     return 0;
+}
+
+extern "C" int scp_grid_diag(scp_grid *g, double *out, int nout) {
+    if (!g || !out) return set_err(-1, "null argument");
+    if (nout < SCP_D_COUNT) return set_err(-1, "nout=%d < %d", nout, SCP_D_COUNT);
+    const DiagParams dp = diag_params(g);
+    const int nb = g->nblocks_diag;
+    int rc = kerr(launch_diag_pass1(g->mode, g->kp, dp, g->xo, g->vo, g->partials, nb, g->stream), "diag1");
+    if (rc) return rc;
+    rc = kerr(launch_reduce(g->partials, nb, DS_NSUM, DM_NMAX, g->dout, g->stream), "reduce");
+    if (rc) return rc;
+    double r[DS_NSUM + DM_NMAX];
+    if ((rc = fetch(g, r, g->dout, DS_NSUM + DM_NMAX))) return rc;
+    return diag_finish(g, dp, r, r + DS_NSUM, out, nout);
+}
+
+// Steps with the diagnostics of the final state fused into the emitting
+// launch: the march accumulates every pass-1 sum and maximum of the emitted
+// state per block (and the gauged charge density), a light pass adds the
+// Gauss-law statistics, which need E at neighbours other blocks compute. Grids
+// whose step does not run the march (real mode, direct-neighbour kernel,
+// gauged storage with g = 0) take the step and the stand-alone pass.
+extern "C" int scp_grid_step_diag(scp_grid *g, int nsteps, double *out, int nout) {
+    if (!g || !out) return set_err(-1, "null argument");
+    if (nsteps < 1) return set_err(-1, "nsteps=%d must be >= 1", nsteps);
+    if (nout < SCP_D_COUNT) return set_err(-1, "nout=%d < %d", nout, SCP_D_COUNT);
+    if (!use_march(g, K_STEP) || g->step_mode != g->mode) {
+        int rc = scp_grid_step(g, nsteps);
+        return rc ? rc : scp_grid_diag(g, out, nout);
+    }
+    constexpr int NQ = DS_NSUM + DM_NMAX;
+    const int nbm = march_blocks(g->kp.N, 0), nbt = g->mode == 2 ? g->nblocks_diag : 0;
+    if (g->fpart_rows < nbm + nbt) {
+        hipFree(g->fpart);
+        g->fpart = nullptr;
+        g->fpart_rows = 0;
+        HIPCHK(hipMalloc(&g->fpart, (size_t)(nbm + nbt) * NQ * sizeof(double)));
+        g->fpart_rows = nbm + nbt;
+    }
+    if (g->mode == 2 && !g->rho) HIPCHK(hipMalloc(&g->rho, (size_t)g->kp.n3 * sizeof(double)));
+    DiagArgs da;
+    da.dp = diag_params(g);
+    da.rho = g->rho;
+    da.partials = g->fpart;
+    int rc = enqueue_steps(g, nsteps, &da);
+    if (rc) return rc;
+    if (nbt)
+        rc = kerr(launch_gauss_tail(g->kp, da.dp, g->vo, g->rho, g->fpart + (size_t)nbm * NQ, nbt, g->stream),
+                  "gauss tail");
+    if (rc) return rc;
+    rc = kerr(launch_reduce_wide(g->fpart, nbm + nbt, DS_NSUM, DM_NMAX, g->dout, g->stream), "reduce");
+    if (rc) return rc;
+    double r[NQ];
+    if ((rc = fetch(g, r, g->dout, NQ))) return rc;
+    return diag_finish(g, da.dp, r, r + DS_NSUM, out, nout);
 }
 
 extern "C" int scp_grid_forces(scp_grid *g, double *const *acc, int nacc) {

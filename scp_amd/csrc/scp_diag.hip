@@ -6,6 +6,7 @@
 // block with a fixed butterfly, and a single-block kernel sums the block
 // partials in block order. The result is bitwise reproducible run to run
 // (the reference's OpenMP/atomicAdd sums are not; they agree to a few ulps).
+#include "scp_diag_site.h"
 #include "scp_kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -62,7 +63,7 @@ k_diag1(KParams kp, DiagParams dp, const double *__restrict__ xo, const double *
     for (int q = 0; q < DS_NSUM; q++) S[q] = 0.0;
 #pragma unroll
     for (int q = 0; q < DM_NMAX; q++) M[q] = 0.0;
-    const int C1[3] = {1, 2, 0}, C2[3] = {2, 0, 1};
+    constexpr int NFD = MODE == 2 ? 15 : 12;
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n3; c += gridDim.x * blockDim.x) {
         const int k = c % N, j = (c / N) % N, i = c / ((uint32_t)N * N);
         uint32_t np[3] = {sidx3(N, wpd(i, N), j, k), sidx3(N, i, wpd(j, N), k), sidx3(N, i, j, wpd(k, N))};
@@ -122,103 +123,14 @@ k_diag1(KParams kp, DiagParams dp, const double *__restrict__ xo, const double *
             for (int a = 0; a < 3; a++) S[DS_EC] -= kp.ETA * X(a, c) * ct[a] * dV;
             S[DS_PINT] += fabs(p0 * p1 * p2) * dV;
         } else {
-            const bool gauged = (MODE == 2) && dp.gauged;
-            double fu[6], fv[6];
-            for (int a = 0; a < 3; a++) {
-                fu[a] = X(F_U + a, c); fv[a] = X(F_V + a, c);
-                fu[3 + a] = X(F_TU + a, c); fv[3 + a] = X(F_TV + a, c);
-            }
-            double DU[3][6], DV[3][6];
-            for (int d = 0; d < 3; d++) {
-                double cP = 1, sP = 0, cM = 1, sM = 0;
-                if (gauged) { sincos(X(F_TH + d, c), &sP, &cP); sincos(X(F_TH + d, nm[d]), &sM, &cM); }
-                for (int a = 0; a < 3; a++)
-                    for (int h = 0; h < 2; h++) {
-                        const int fu_ = h ? F_TU + a : F_U + a, fv_ = h ? F_TV + a : F_V + a;
-                        double up = X(fu_, np[d]), vp = X(fv_, np[d]), um = X(fu_, nm[d]), vm = X(fv_, nm[d]);
-                        if (gauged) {
-                            DU[d][h * 3 + a] = ((cP * up - sP * vp) - (cM * um + sM * vm)) * h1;
-                            DV[d][h * 3 + a] = ((cP * vp + sP * up) - (cM * vm - sM * um)) * h1;
-                        } else {
-                            DU[d][h * 3 + a] = (up - um) * h1;
-                            DV[d][h * 3 + a] = (vp - vm) * h1;
-                        }
-                    }
-            }
-            double s2[3], rho2 = 0, s = 1.0, qpv = 0, qtv = 0;
-            for (int a = 0; a < 3; a++) {
-                s2[a] = fu[a] * fu[a] + fv[a] * fv[a];
-                double ud = V(F_U + a, c), vd = V(F_V + a, c), tud = V(F_TU + a, c), tvd = V(F_TV + a, c);
-                S[DS_EPK] += 0.5 * (ud * ud + vd * vd) * dV;
-                S[DS_ETK] += 0.5 * (tud * tud + tvd * tvd) * dV;
-                S[DS_EG] += 0.5 * (DU[0][a] * DU[0][a] + DU[1][a] * DU[1][a] + DU[2][a] * DU[2][a]
-                                 + DV[0][a] * DV[0][a] + DV[1][a] * DV[1][a] + DV[2][a] * DV[2][a]) * dV;
-                if (gauged) S[DS_EM] += 0.5 * kp.MASS2 * s2[a] * dV;
-                else S[DS_EM] += 0.5 * kp.MASS2 * (fu[a] * fu[a] + fv[a] * fv[a]) * dV;
-                S[DS_ETG] += 0.5 * (DU[0][3 + a] * DU[0][3 + a] + DU[1][3 + a] * DU[1][3 + a] + DU[2][3 + a] * DU[2][3 + a]
-                                  + DV[0][3 + a] * DV[0][3 + a] + DV[1][3 + a] * DV[1][3 + a] + DV[2][3 + a] * DV[2][3 + a]) * dV;
-                S[DS_ETM] += 0.5 * kp.MT2 * (fu[3 + a] * fu[3 + a] + fv[3 + a] * fv[3 + a]) * dV;
-                M[DM_PHI] = fmax(M[DM_PHI], sqrt(s2[a]));
-                double reDxT, imDxT;
-                if (gauged) {
-                    reDxT = DU[C1[a]][3 + C2[a]] - DU[C2[a]][3 + C1[a]];
-                    imDxT = DV[C1[a]][3 + C2[a]] - DV[C2[a]][3 + C1[a]];
-                } else {   // ungauged curls read the raw neighbours (C:1868-1869)
-                    const int b1 = C1[a], b2 = C2[a];
-                    reDxT = (X(F_TU + b2, np[b1]) - X(F_TU + b2, nm[b1]) - X(F_TU + b1, np[b2]) + X(F_TU + b1, nm[b2])) * h1;
-                    imDxT = (X(F_TV + b2, np[b1]) - X(F_TV + b2, nm[b1]) - X(F_TV + b1, np[b2]) + X(F_TV + b1, nm[b2])) * h1;
-                }
-                S[DS_EC] -= kp.ETA * (fu[a] * reDxT + fv[a] * imDxT) * dV;
-                S[DS_TH2] += fu[3 + a] * fu[3 + a] + fv[3 + a] * fv[3 + a];
-                // charges (C:2091-2103)
-                double q = fu[a] * vd - fv[a] * ud;
-                qpv += q;
-                S[DS_QP0 + a] += q;
-                qtv += fu[3 + a] * tvd - fv[3 + a] * tud;
-                rho2 += s2[a];
-                s *= s2[a];
-            }
-            double sp = s2[0] * s2[1] * s2[2];
-            S[DS_EP] += (kp.MU / 2.0) * sp / (1.0 + kp.KAPPA * sp) * dV;
-            M[DM_P] = fmax(M[DM_P], sqrt(sp));
-            S[DS_PINT] += sqrt(sp) * dV;
-            S[DS_QP] += qpv; S[DS_QT] += qtv;
-            if (x * x + y * y + z * z <= dp.R2core) S[DS_QC] += qpv + qtv;
-            M[DM_S] = fmax(M[DM_S], s);
-            S[DS_M0] += rho2; S[DS_CX] += x * rho2; S[DS_CY] += y * rho2; S[DS_CZ] += z * rho2;
+            const auto xn = [&](int f, int d, int sg) -> double { return X(f, sg > 0 ? np[d] : nm[d]); };
+            double xc[15] = {}, vc[15] = {};
+#pragma unroll
+            for (int f = 0; f < NFD; f++) { xc[f] = X(f, c); vc[f] = V(f, c); }
+            const double rho = diag_site<MODE>(kp, dp, xc, vc, xn, i, j, k, S, M);
             if (MODE == 2) {
-                double E0 = V(F_TH + 0, c), E1 = V(F_TH + 1, c), E2 = V(F_TH + 2, c);
-                double ee = 0.5 * (E0 * E0 + E1 * E1 + E2 * E2);
-                double bb = 0.0;
-                if (gauged) {
-                    const int PA[3] = {0, 1, 2}, PB[3] = {1, 2, 0};
-                    for (int p = 0; p < 3; p++) {
-                        int a = PA[p], b = PB[p];
-                        double ang = X(F_TH + a, c) + X(F_TH + b, np[a]) - X(F_TH + a, np[b]) - X(F_TH + b, c);
-                        bb += 1.0 - cos(ang);
-                    }
-                    S[DS_EEM_E] += (ee + dp.BPRE * bb) * dV;
-                }
-                // Gauss residual (C:2176-2181, 2210-2219)
-                double divE = (E0 - V(F_TH + 0, nm[0]) + E1 - V(F_TH + 1, nm[1]) + E2 - V(F_TH + 2, nm[2])) / dx;
-                double rho = 0;
-                for (int a = 0; a < 3; a++)
-                    rho += fu[a] * V(F_V + a, c) - fv[a] * V(F_U + a, c)
-                         + fu[3 + a] * V(F_TV + a, c) - fv[3 + a] * V(F_TU + a, c);
-                double G = divE - kp.GG * rho;
-                if (x * x + y * y + z * z < dp.Rint2) {
-                    double dG = fabs(G - dp.G_offset);
-                    M[DM_GAUSS] = fmax(M[DM_GAUSS], dG);
-                    S[DS_GL2] += (G - dp.G_offset) * (G - dp.G_offset);
-                    S[DS_NOMEGA] += 1.0;
-                }
-                double eg = ee;
-                if (kp.GG != 0) eg += bb / (kp.GG * kp.GG * dx * dx * dx * dx);
-                S[DS_EEM_G] += eg * dV;
-                if (fabs(x) <= dp.Rcube && fabs(y) <= dp.Rcube && fabs(z) <= dp.Rcube) {
-                    S[DS_FSUM] += divE * dV;
-                    S[DS_NC] += 1.0;
-                }
+                const double Em[3] = {V(F_TH + 0, nm[0]), V(F_TH + 1, nm[1]), V(F_TH + 2, nm[2])};
+                gauss_site(kp, dp, vc + F_TH, Em, rho, i, j, k, S, M);
             }
         }
     }
@@ -258,6 +170,59 @@ k_diag2(KParams kp, const double *__restrict__ xo, double thr, double cx, double
     block_reduce<1, false>(v, partials + (size_t)blockIdx.x);
 }
 
+// Gauss statistics of the state emitted by a fused-diagnostics launch: E from
+// the emitted velocities, rho from the step (compute_gauss, C:2186-2243). The
+// row's other slots are zero, so it reduces with the step's partials.
+__global__ void __launch_bounds__(DIAG_BLOCK)
+k_gauss_tail(KParams kp, DiagParams dp, const double *__restrict__ vo, const double *__restrict__ rho,
+             double *partials) {
+    const int N = kp.N;
+    const uint32_t fs = kp.fs, n3 = kp.n3;
+    double S[DS_NSUM], M[DM_NMAX];
+#pragma unroll
+    for (int q = 0; q < DS_NSUM; q++) S[q] = 0.0;
+#pragma unroll
+    for (int q = 0; q < DM_NMAX; q++) M[q] = 0.0;
+    const double *E0 = vo + (size_t)F_TH * fs, *E1 = E0 + fs, *E2 = E1 + fs;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n3; c += gridDim.x * blockDim.x) {
+        const int k = c % N, j = (c / N) % N, i = c / ((uint32_t)N * N);
+        const double E[3] = {E0[c], E1[c], E2[c]};
+        const double Em[3] = {E0[sidx3(N, wmd(i, N), j, k)], E1[sidx3(N, i, wmd(j, N), k)],
+                              E2[sidx3(N, i, j, wmd(k, N))]};
+        gauss_site(kp, dp, E, Em, rho[c], i, j, k, S, M);
+    }
+    double *out = partials + (size_t)blockIdx.x * (DS_NSUM + DM_NMAX);
+    block_reduce<DS_NSUM, false>(S, out);
+    block_reduce<DM_NMAX, true>(M, out + DS_NSUM);
+}
+
+// Wide form of k_reduce for many partial rows: block q reduces slot q, thread t
+// rows t, t + 256, ... in order, then a fixed butterfly; deterministic. The
+// maxima are of non-negative values (0 is their identity).
+__global__ void __launch_bounds__(256)
+k_reduce_wide(const double *partials, int nblocks, int nsum, int nmax, double *out) {
+    __shared__ double red[4];
+    const int q = blockIdx.x, stride = nsum + nmax, t = threadIdx.x;
+    const bool isum = q < nsum;
+    double x = 0.0;
+    for (int b = t; b < nblocks; b += 256) {
+        const double y = partials[(size_t)b * stride + q];
+        x = isum ? x + y : fmax(x, y);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double y = __shfl_xor(x, o, 64);
+        x = isum ? x + y : fmax(x, y);
+    }
+    if ((t & 63) == 0) red[t >> 6] = x;
+    __syncthreads();
+    if (t == 0) {
+        double v = red[0];
+        for (int w = 1; w < 4; w++) v = isum ? v + red[w] : fmax(v, red[w]);
+        out[q] = v;
+    }
+}
+
 // sums (first nsum) and maxima (next nmax) of the per-block partials, in block order
 __global__ void k_reduce(const double *partials, int nblocks, int nsum, int nmax, double *out) {
     int q = threadIdx.x;
@@ -283,6 +248,17 @@ int launch_diag_pass2(const KParams &kp, const double *xo, double thr, double cx
                       double cz, double *partials, unsigned long long *argmin, int nblocks,
                       hipStream_t s) {
     hipLaunchKernelGGL(k_diag2, dim3(nblocks), dim3(DIAG_BLOCK), 0, s, kp, xo, thr, cx, cy, cz, partials, argmin);
+    return (int)hipGetLastError();
+}
+
+int launch_gauss_tail(const KParams &kp, const DiagParams &dp, const double *vo, const double *rho,
+                      double *partials, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_gauss_tail, dim3(nblocks), dim3(DIAG_BLOCK), 0, s, kp, dp, vo, rho, partials);
+    return (int)hipGetLastError();
+}
+
+int launch_reduce_wide(const double *partials, int nblocks, int nsum, int nmax, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_wide, dim3(nsum + nmax), dim3(256), 0, s, partials, nblocks, nsum, nmax, out);
     return (int)hipGetLastError();
 }
 

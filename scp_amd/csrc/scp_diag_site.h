@@ -1,0 +1,147 @@
+// scp_diag_site.h — per-site diagnostics of the complex and gauged modes,
+// shared by the stand-alone diagnostics pass (scp_diag.hip, neighbours from the
+// emitted FP64 state) and the diagnostics fused into the step's emitting
+// launch (scp_march.h, neighbours from the staged planes).
+//
+// Energy terms (compute_energy_complex, scp_sim.c:1823-1880, and
+// compute_energy_complex_gauge, C:1886-2036), Noether charges and the core
+// moments (compute_charges, C:2074-2169), P_int and theta_rms (C:2039-2059) of
+// one site, accumulated into S[] and M[] (the DS_* / DM_* slots). Everything
+// but the Gauss residual, which needs E at the backward neighbours: the
+// function returns the site's charge density rho (C:2176-2181) for it.
+#pragma once
+
+#include "scp_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+namespace scp {
+
+// xc, vc: the site's fields and velocities (device field order, FP64);
+// xn(f, d, sg): field f at the neighbour one step along d (0 = i, 1 = j,
+// 2 = k) in direction sg = +1 / -1, as a double.
+template <int MODE, class XN>
+__device__ inline double diag_site(const KParams &kp, const DiagParams &dp, const double *xc, const double *vc,
+                                   const XN &xn, int i, int j, int k, double *S, double *M) {
+#pragma clang fp contract(off)
+    const double dx = kp.dx, dV = dp.dV, h1 = kp.h1, L = kp.L;
+    const double x = -L + i * dx, y = -L + j * dx, z = -L + k * dx;
+    const bool gauged = (MODE == 2) && dp.gauged;
+    const int C1[3] = {1, 2, 0}, C2[3] = {2, 0, 1};
+    double fu[6], fv[6];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        fu[a] = xc[F_U + a]; fv[a] = xc[F_V + a];
+        fu[3 + a] = xc[F_TU + a]; fv[3 + a] = xc[F_TV + a];
+    }
+    // covariant (gauged) or plain central differences, C:1838-1849, 1915-1931
+    double DU[3][6], DV[3][6];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double cP = 1, sP = 0, cM = 1, sM = 0;
+        if (gauged) { sincos(xc[F_TH + d], &sP, &cP); sincos(xn(F_TH + d, d, -1), &sM, &cM); }
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int fu_ = h ? F_TU + a : F_U + a, fv_ = h ? F_TV + a : F_V + a;
+                const double up = xn(fu_, d, 1), vp = xn(fv_, d, 1), um = xn(fu_, d, -1), vm = xn(fv_, d, -1);
+                if (gauged) {
+                    DU[d][h * 3 + a] = ((cP * up - sP * vp) - (cM * um + sM * vm)) * h1;
+                    DV[d][h * 3 + a] = ((cP * vp + sP * up) - (cM * vm - sM * um)) * h1;
+                } else {
+                    DU[d][h * 3 + a] = (up - um) * h1;
+                    DV[d][h * 3 + a] = (vp - vm) * h1;
+                }
+            }
+    }
+    double s2[3], rho2 = 0, s = 1.0, qpv = 0, qtv = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        s2[a] = fu[a] * fu[a] + fv[a] * fv[a];
+        const double ud = vc[F_U + a], vd = vc[F_V + a], tud = vc[F_TU + a], tvd = vc[F_TV + a];
+        S[DS_EPK] += 0.5 * (ud * ud + vd * vd) * dV;
+        S[DS_ETK] += 0.5 * (tud * tud + tvd * tvd) * dV;
+        S[DS_EG] += 0.5 * (DU[0][a] * DU[0][a] + DU[1][a] * DU[1][a] + DU[2][a] * DU[2][a]
+                         + DV[0][a] * DV[0][a] + DV[1][a] * DV[1][a] + DV[2][a] * DV[2][a]) * dV;
+        if (gauged) S[DS_EM] += 0.5 * kp.MASS2 * s2[a] * dV;
+        else S[DS_EM] += 0.5 * kp.MASS2 * (fu[a] * fu[a] + fv[a] * fv[a]) * dV;
+        S[DS_ETG] += 0.5 * (DU[0][3 + a] * DU[0][3 + a] + DU[1][3 + a] * DU[1][3 + a] + DU[2][3 + a] * DU[2][3 + a]
+                          + DV[0][3 + a] * DV[0][3 + a] + DV[1][3 + a] * DV[1][3 + a] + DV[2][3 + a] * DV[2][3 + a]) * dV;
+        S[DS_ETM] += 0.5 * kp.MT2 * (fu[3 + a] * fu[3 + a] + fv[3 + a] * fv[3 + a]) * dV;
+        M[DM_PHI] = fmax(M[DM_PHI], sqrt(s2[a]));
+        double reDxT, imDxT;
+        if (gauged) {
+            reDxT = DU[C1[a]][3 + C2[a]] - DU[C2[a]][3 + C1[a]];
+            imDxT = DV[C1[a]][3 + C2[a]] - DV[C2[a]][3 + C1[a]];
+        } else {   // ungauged curls read the raw neighbours (C:1868-1869)
+            const int b1 = C1[a], b2 = C2[a];
+            reDxT = (xn(F_TU + b2, b1, 1) - xn(F_TU + b2, b1, -1) - xn(F_TU + b1, b2, 1) + xn(F_TU + b1, b2, -1)) * h1;
+            imDxT = (xn(F_TV + b2, b1, 1) - xn(F_TV + b2, b1, -1) - xn(F_TV + b1, b2, 1) + xn(F_TV + b1, b2, -1)) * h1;
+        }
+        S[DS_EC] -= kp.ETA * (fu[a] * reDxT + fv[a] * imDxT) * dV;
+        S[DS_TH2] += fu[3 + a] * fu[3 + a] + fv[3 + a] * fv[3 + a];
+        // charges (C:2091-2103)
+        const double q = fu[a] * vd - fv[a] * ud;
+        qpv += q;
+        S[DS_QP0 + a] += q;
+        qtv += fu[3 + a] * tvd - fv[3 + a] * tud;
+        rho2 += s2[a];
+        s *= s2[a];
+    }
+    const double sp = s2[0] * s2[1] * s2[2];
+    S[DS_EP] += (kp.MU / 2.0) * sp / (1.0 + kp.KAPPA * sp) * dV;
+    M[DM_P] = fmax(M[DM_P], sqrt(sp));
+    S[DS_PINT] += sqrt(sp) * dV;
+    S[DS_QP] += qpv; S[DS_QT] += qtv;
+    if (x * x + y * y + z * z <= dp.R2core) S[DS_QC] += qpv + qtv;
+    M[DM_S] = fmax(M[DM_S], s);
+    S[DS_M0] += rho2; S[DS_CX] += x * rho2; S[DS_CY] += y * rho2; S[DS_CZ] += z * rho2;
+    double rho = 0;
+    if (MODE == 2) {
+        const double E0 = vc[F_TH + 0], E1 = vc[F_TH + 1], E2 = vc[F_TH + 2];
+        const double ee = 0.5 * (E0 * E0 + E1 * E1 + E2 * E2);
+        double bb = 0.0;
+        if (gauged) {   // plaquette energy (C:1957-1968)
+            const int PA[3] = {0, 1, 2}, PB[3] = {1, 2, 0};
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+                const int a = PA[p], b = PB[p];
+                const double ang = xc[F_TH + a] + xn(F_TH + b, a, 1) - xn(F_TH + a, b, 1) - xc[F_TH + b];
+                bb += 1.0 - cos(ang);
+            }
+            S[DS_EEM_E] += (ee + dp.BPRE * bb) * dV;
+        }
+        double eg = ee;
+        if (kp.GG != 0) eg += bb / (kp.GG * kp.GG * dx * dx * dx * dx);
+        S[DS_EEM_G] += eg * dV;
+        // charge density of the Gauss law (C:2176-2181)
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            rho += fu[a] * vc[F_V + a] - fv[a] * vc[F_U + a] + fu[3 + a] * vc[F_TV + a] - fv[3 + a] * vc[F_TU + a];
+    }
+    return rho;
+}
+
+// Gauss-law statistics of one site (compute_gauss, C:2186-2243): G from the
+// site's E, the backward neighbours' E (Em[d] = E_d(x - e_d)) and rho.
+__device__ inline void gauss_site(const KParams &kp, const DiagParams &dp, const double *E, const double *Em,
+                                  double rho, int i, int j, int k, double *S, double *M) {
+#pragma clang fp contract(off)
+    const double dx = kp.dx, dV = dp.dV, L = kp.L;
+    const double x = -L + i * dx, y = -L + j * dx, z = -L + k * dx;
+    const double divE = (E[0] - Em[0] + E[1] - Em[1] + E[2] - Em[2]) / dx;
+    const double G = divE - kp.GG * rho;
+    if (x * x + y * y + z * z < dp.Rint2) {
+        const double dG = fabs(G - dp.G_offset);
+        M[DM_GAUSS] = fmax(M[DM_GAUSS], dG);
+        S[DS_GL2] += (G - dp.G_offset) * (G - dp.G_offset);
+        S[DS_NOMEGA] += 1.0;
+    }
+    if (fabs(x) <= dp.Rcube && fabs(y) <= dp.Rcube && fabs(z) <= dp.Rcube) {
+        S[DS_FSUM] += divE * dV;
+        S[DS_NC] += 1.0;
+    }
+}
+
+}  // namespace scp

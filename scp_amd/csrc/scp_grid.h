@@ -28,6 +28,9 @@ struct scp_grid {
     double *acc;     // forces output (lazily allocated)
     double *partials;
     double *dout;
+    double *rho;      // fused diagnostics: Gauss charge density (N^3, gauged), lazily allocated
+    double *fpart;    // fused diagnostics: partial rows of the step and the Gauss tail, lazily allocated
+    int fpart_rows;
     unsigned long long *argmin;
     int cur;
     int nblocks_diag;

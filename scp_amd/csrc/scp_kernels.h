@@ -142,6 +142,30 @@ struct DiagParams {
     int32_t gauged;     // g != 0
     int32_t has_gauge;  // gauge sector present (complex_gauge)
 };
+// Diagnostics fused into the emitting launch of the march (complex and gauged
+// modes): every block writes DS_NSUM sums and DM_NMAX maxima of its sites to
+// partials (one row of DS_NSUM + DM_NMAX per block, the pass-1 layout), and
+// the gauged step writes each site's Gauss-law charge density to rho (N^3,
+// unpadded) for the Gauss tail pass, which needs E at the backward neighbours
+// that other blocks compute.
+struct DiagArgs {
+    DiagParams dp;
+    double *rho;
+    double *partials;
+};
+int launch_march_f64_diag(int mode, const KParams &kp, const Bufs<double> &b, int DI, const DiagArgs &da,
+                          hipStream_t s);
+int launch_march_f32c_diag(int mode, int lo_kind, const KParams &kp, const Bufs32 &b, int DI, const DiagArgs &da,
+                           hipStream_t s);
+// blocks of one march launch at N (64 x 4 tiles, DI planes; DI <= 0: the default)
+inline int march_blocks(int N, int DI) {
+    if (DI <= 0) DI = march_di(N, 1);
+    return (int)(march_tiles_k(N) * march_tiles_j(N) * (uint32_t)((N + DI - 1) / DI));
+}
+// Gauss statistics (DS_GL2, DS_NOMEGA, DS_FSUM, DS_NC, DM_GAUSS; other slots
+// zero) from the emitted E and rho, per block in the pass-1 layout
+int launch_gauss_tail(const KParams &kp, const DiagParams &dp, const double *vo, const double *rho,
+                      double *partials, int nblocks, hipStream_t s);
 int launch_diag_pass1(int mode, const KParams &kp, const DiagParams &dp, const double *xo,
                       const double *vo, double *partials, int nblocks, hipStream_t s);
 int launch_diag_pass2(const KParams &kp, const double *xo, double thr, double cx, double cy,
@@ -149,6 +173,9 @@ int launch_diag_pass2(const KParams &kp, const double *xo, double thr, double cx
                       hipStream_t s);
 int launch_reduce(const double *partials, int nblocks, int nsum, int nmax, double *out,
                   hipStream_t s);
+// the same, one block per slot (for thousands of rows); maxima of non-negative values
+int launch_reduce_wide(const double *partials, int nblocks, int nsum, int nmax, double *out,
+                       hipStream_t s);
 
 // Gauss-law helpers (FP64, device CG).
 int launch_gauss_residual(const KParams &kp, const double *xo, const double *vo, double *b,

@@ -29,6 +29,7 @@
 // at the same relative LDS offset.
 #pragma once
 
+#include "scp_diag_site.h"
 #include "scp_physics.h"
 
 #include <hip/hip_runtime.h>
@@ -215,10 +216,12 @@ struct S64 {
     __device__ static Own own_load(const B &b, int f, uint32_t fh, uint32_t e) {
         return {ldo_nt(fbase(b.w, f, fh), e)};
     }
-    // fh, e: field stride and index of the step state; fs, ec: of the emitted state
+    // fh, e: field stride and index of the step state; fs, ec: of the emitted state;
+    // xe, ve: the emitted position and velocity (x_n, v_n in FP64)
     template <int MODE, bool EMIT>
     __device__ static void update(const KParams &kp, const B &b, int f, uint32_t fh, uint32_t e, uint32_t fs,
-                                  uint32_t ec, double a, double x, Own s, bool damp, double dmp) {
+                                  uint32_t ec, double a, double x, Own s, bool damp, double dmp, double &xe,
+                                  double &ve) {
         double v = s.w;
         v += kp.hdt * a;                               // stage 4 (C:1680-1696)
         if (damp) v *= dmp;                            // stage 5 (C:1454-1460)
@@ -232,6 +235,8 @@ struct S64 {
         }
         sto_nt(fbase(b.w, f, fh), e, w);
         sto_nt(fbase(b.pout, f, fh), e, xn);
+        xe = x;
+        ve = v;
         if constexpr (EMIT) {
             __builtin_nontemporal_store(x, &b.xo[(size_t)f * fs + ec]);
             __builtin_nontemporal_store(v, &b.vo[(size_t)f * fs + ec]);
@@ -275,10 +280,12 @@ struct S32 {
     __device__ static Own own_load(const B &b, int f, uint32_t fh, uint32_t e) {
         return {ldo_nt(fbase(b.w, f, fh), e), LoIO<LOK>::get(b.wlo, f, fh, e), LoIO<LOK>::get(b.pinlo, f, fh, e)};
     }
-    // fh, e: field stride and index of the step state; fs, ec: of the emitted state
+    // fh, e: field stride and index of the step state; fs, ec: of the emitted state;
+    // xe, ve: the emitted position and velocity (hi + lo in FP64)
     template <int MODE, bool EMIT>
     __device__ static void update(const KParams &kp, const B &b, int f, uint32_t fh, uint32_t e, uint32_t fs,
-                                  uint32_t ec, float a, float xh, Own s, bool damp, double dmpd) {
+                                  uint32_t ec, float a, float xh, Own s, bool damp, double dmpd, double &xe,
+                                  double &ve) {
         const DF hdt = df_split(kp.hdt), dt = df_split(kp.dt), gad = df_split(kp.gad);
         float wh = s.wh, wl = s.wl, vh = 0.0f, vl = 0.0f;
         // both half kicks, every step (also between emitted steps): the result then
@@ -303,9 +310,11 @@ struct S32 {
         LoIO<LOK>::put(b.wlo, f, fh, e, wl);
         sto_nt(fbase(b.pout, f, fh), e, xh);
         LoIO<LOK>::put(b.poutlo, f, fh, e, xl);
+        xe = (double)xh0 + (double)xl0;
+        ve = (double)vh + (double)vl;
         if constexpr (EMIT) {
-            __builtin_nontemporal_store((double)xh0 + (double)xl0, &b.xo[(size_t)f * fs + ec]);
-            __builtin_nontemporal_store((double)vh + (double)vl, &b.vo[(size_t)f * fs + ec]);
+            __builtin_nontemporal_store(xe, &b.xo[(size_t)f * fs + ec]);
+            __builtin_nontemporal_store(ve, &b.vo[(size_t)f * fs + ec]);
         }
     }
 };
@@ -319,13 +328,21 @@ constexpr int march_nf() { return MODE == 2 ? 15 : 12; }   // staged fields (mat
 template <class S, int KW, int MODE>
 constexpr int march_lds_elems() { return MNBUF * march_nf<MODE>() * MG<KW>::MPL; }
 
+// Per-thread diagnostics accumulators of the DIAG instance (DS_* sums, DM_* maxima).
+struct DiagState {
+    double S[DS_NSUM], M[DM_NMAX];
+};
+
 // One tile of one grid: the march over planes [seg*DI, seg*DI + DI) of the
 // 64*KW x 4 column tile (tk, tj). lds: the block's staging area,
 // march_lds_elems() words. Ends with a barrier, so the caller may stage the
-// next tile into the same area.
-template <class S, int KW, int MODE, bool EMIT>
+// next tile into the same area. DIAG (with EMIT): the emitted state's per-site
+// diagnostics are accumulated into ds, and the gauged charge density is
+// written to da->rho.
+template <class S, int KW, int MODE, bool EMIT, bool DIAG = false>
 __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::B &b, int DI, uint32_t tk,
-                                           uint32_t tj, uint32_t seg, typename S::T *lds) {
+                                           uint32_t tj, uint32_t seg, typename S::T *lds,
+                                           const DiagArgs *da = nullptr, DiagState *ds = nullptr) {
     using T = typename S::T;
     constexpr int NF = march_nf<MODE>();
     constexpr int NM = 12;                    // matter fields (the i-1 plane is matter only)
@@ -375,6 +392,7 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
     T cm[NM], cp[NF], hr[RQ], hc;
     T cq[S::REGQ ? NF : 1];     // register copy of plane i (register-queue policy)
     GaugeCarry<T> cy{};
+    T th0m = T(0);              // DIAG, gauged: th_0 at plane i-1 (backward transport along i)
     {
         const uint32_t em = (uint32_t)im0 * NN + col, e0 = (uint32_t)t.i0 * NN + col, ep = (uint32_t)ip0 * NN + col;
         T c0[NF];
@@ -383,6 +401,7 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
             for (int f = 0; f < NM; f++) cm[f] = ld(f, em);
 #pragma unroll
             for (int f = 0; f < NF; f++) { c0[f] = ld(f, e0); cp[f] = ld(f, ep); }
+            if constexpr (DIAG && MODE == 2) th0m = ld(F_TH + 0, em);
             if constexpr (MODE == 2) {
                 const uint32_t jpo = (uint32_t)im0 * NN + (uint32_t)wp(t.j, N) * kp.pitch + t.k;
                 const uint32_t kpo = (uint32_t)im0 * NN + (uint32_t)t.j * kp.pitch + wp(t.k, N);
@@ -440,28 +459,46 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
             else
                 forces_complex(kp, c0, c1, cm, nb, acc);
             double dmp = 1.0;
+            double xe[NF], ve[NF];   // the emitted state of this site (used by DIAG)
             if constexpr (S::SPLIT_DAMP) {
                 // separate damped / undamped paths: a select between them would keep both live
                 if (kp.damp && sponge(kp, i, t.j, t.k, dmp)) {
 #pragma unroll
                     for (int f = 0; f < NF; f++)
-                        S::template update<MODE, EMIT>(kp, b, f, fs, e, kp.fs, ec, acc[f], c0[f], sw[f], true, dmp);
+                        S::template update<MODE, EMIT>(kp, b, f, fs, e, kp.fs, ec, acc[f], c0[f], sw[f], true, dmp,
+                                                       xe[f], ve[f]);
                 } else {
 #pragma unroll
                     for (int f = 0; f < NF; f++)
-                        S::template update<MODE, EMIT>(kp, b, f, fs, e, kp.fs, ec, acc[f], c0[f], sw[f], false, 1.0);
+                        S::template update<MODE, EMIT>(kp, b, f, fs, e, kp.fs, ec, acc[f], c0[f], sw[f], false, 1.0,
+                                                       xe[f], ve[f]);
                 }
             } else {
                 const bool damp = kp.damp && sponge(kp, i, t.j, t.k, dmp);
 #pragma unroll
                 for (int f = 0; f < NF; f++)
-                    S::template update<MODE, EMIT>(kp, b, f, fs, e, kp.fs, ec, acc[f], c0[f], sw[f], damp, dmp);
+                    S::template update<MODE, EMIT>(kp, b, f, fs, e, kp.fs, ec, acc[f], c0[f], sw[f], damp, dmp,
+                                                   xe[f], ve[f]);
+            }
+            if constexpr (DIAG) {
+                // neighbours at the stencil level of this launch: plane i+1 (own column),
+                // plane i-1 (matter column and th_0), in-plane from the staged tile
+                const auto xn = [&](int f, int d, int sg) -> double {
+                    if (d == 0) {
+                        if (sg > 0) return (double)c1[f];
+                        return f < NM ? (double)cm[f] : (double)th0m;
+                    }
+                    return (double)nb.face(f, d, sg);
+                };
+                const double rho = diag_site<MODE>(kp, da->dp, xe, ve, xn, i, t.j, t.k, ds->S, ds->M);
+                if constexpr (MODE == 2) __builtin_nontemporal_store(rho, &da->rho[ec]);
             }
         };
         if constexpr (S::REGQ) {
             if (t.valid) compute(cq, cp);
 #pragma unroll
             for (int f = 0; f < NM; f++) cm[f] = cq[f];
+            if constexpr (DIAG && MODE == 2) th0m = cq[F_TH];
 #pragma unroll
             for (int f = 0; f < NF; f++) { cq[f] = cp[f]; cp[f] = cpp[f]; }
         } else {
@@ -469,6 +506,7 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
             if (t.valid) compute(c0, c1);
 #pragma unroll
             for (int f = 0; f < NM; f++) cm[f] = c0[f];
+            if constexpr (DIAG && MODE == 2) th0m = c0[F_TH];
 #pragma unroll
             for (int f = 0; f < NF; f++) cp[f] = cpp[f];
         }
@@ -490,6 +528,55 @@ k_march(KParams kp, typename S::B b, int DI) {
     const uint32_t tk = lb % ntk; lb /= ntk;
     const uint32_t tj = lb % ntj; lb /= ntj;
     march_tile<S, KW, MODE, EMIT>(kp, b, DI, tk, tj, lb, lds);
+}
+
+// The emitting launch with the diagnostics fused (single grid, KW = 1): each
+// block reduces its threads' accumulators (wave butterfly, then its waves in
+// order: deterministic) into its row of da.partials, also when it has no
+// planes (the row is then zero).
+template <class S, int MODE>
+__global__ void __launch_bounds__(MG<1>::MNT, 1)
+k_march_diag(KParams kp, typename S::B b, int DI, DiagArgs da) {
+    __shared__ typename S::T lds[march_lds_elems<S, 1, MODE>()];
+    constexpr int NQ = DS_NSUM + DM_NMAX, NW = MG<1>::NW;
+    const uint32_t ntk = (kp.N + MG<1>::TKM - 1) / MG<1>::TKM, ntj = (kp.N + MTJ - 1) / MTJ;
+    uint32_t lb = logical_block(blockIdx.x, gridDim.x);
+    const uint32_t tk = lb % ntk; lb /= ntk;
+    const uint32_t tj = lb % ntj; lb /= ntj;
+    DiagState ds;
+#pragma unroll
+    for (int q = 0; q < DS_NSUM; q++) ds.S[q] = 0.0;
+#pragma unroll
+    for (int q = 0; q < DM_NMAX; q++) ds.M[q] = 0.0;
+    march_tile<S, 1, MODE, true, true>(kp, b, DI, tk, tj, lb, lds, &da, &ds);
+    // march_tile ended with a barrier, or the whole block returned before staging: lds is free
+    double *red = reinterpret_cast<double *>(lds);
+    const int tid = threadIdx.y * MG<1>::TKM + threadIdx.x, lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        double v = q < DS_NSUM ? ds.S[q] : ds.M[q - DS_NSUM];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double y = __shfl_xor(v, o, 64);
+            v = q < DS_NSUM ? v + y : fmax(v, y);
+        }
+        if (lane == 0) red[wv * NQ + q] = v;
+    }
+    __syncthreads();
+    if (tid < NQ) {
+        double v = red[tid];
+        for (int w = 1; w < NW; w++) v = tid < DS_NSUM ? v + red[w * NQ + tid] : fmax(v, red[w * NQ + tid]);
+        da.partials[(size_t)blockIdx.x * NQ + tid] = v;
+    }
+}
+
+template <class S, int MODE>
+inline int launch_march_diag_t(const KParams &kp, const typename S::B &b, int DI, const DiagArgs &da,
+                               hipStream_t s) {
+    if (DI <= 0) DI = march_di(kp.N, 1);
+    const dim3 grid((uint32_t)march_blocks(kp.N, DI)), block(MG<1>::TKM, MTJ);
+    hipLaunchKernelGGL((k_march_diag<S, MODE>), grid, block, 0, s, kp, b, DI, da);
+    return (int)hipGetLastError();
 }
 
 // Many grids: block b runs tiles [b*per_block, (b+1)*per_block) of the table

@@ -11,6 +11,13 @@ int launch_march_f64(int mode, bool emit, const KParams &kp, const Bufs<double> 
     return (int)hipErrorInvalidValue;
 }
 
+int launch_march_f64_diag(int mode, const KParams &kp, const Bufs<double> &b, int DI, const DiagArgs &da,
+                          hipStream_t s) {
+    if (mode == 2) return launch_march_diag_t<S64, 2>(kp, b, DI, da, s);
+    if (mode == 1) return launch_march_diag_t<S64, 1>(kp, b, DI, da, s);
+    return (int)hipErrorInvalidValue;
+}
+
 int launch_march_batch_f64(int mode, const BatchGrid<Bufs<double>> *grids, const BatchTile *tiles,
                            uint32_t ntiles, uint32_t nblocks, uint32_t per_block, int q, hipStream_t s) {
     if (mode == 2) return launch_march_batch_t<S64, 2>(grids, tiles, ntiles, nblocks, per_block, q, s);

@@ -33,7 +33,7 @@ DIAG_NAMES = ["E_phi_kin", "E_theta_kin", "E_grad", "E_mass", "E_pot", "E_tgrad"
 # every entry point declared in include/scp_amd.h
 ABI_SYMBOLS = ["scp_abi_version", "scp_last_error", "scp_grid_bytes", "scp_grid_create",
                "scp_grid_destroy", "scp_grid_upload", "scp_grid_download", "scp_grid_step",
-               "scp_grid_diag", "scp_grid_forces", "scp_grid_gauss_project",
+               "scp_grid_diag", "scp_grid_step_diag", "scp_grid_forces", "scp_grid_gauss_project",
                "scp_grid_set_g_offset", "scp_grid_g_offset", "scp_grid_rescale_phi",
                "scp_grid_steps_done", "scp_grid_time_steps", "scp_grid_step_kernel_name",
                "scp_batch_create", "scp_batch_destroy", "scp_batch_add", "scp_batch_remove",
@@ -140,6 +140,7 @@ def lib():
         L.scp_grid_download.argtypes = [VP, DPP, C.c_int]
         L.scp_grid_step.argtypes = [VP, C.c_int]
         L.scp_grid_diag.argtypes = [VP, C.POINTER(C.c_double), C.c_int]
+        L.scp_grid_step_diag.argtypes = [VP, C.c_int, C.POINTER(C.c_double), C.c_int]
         L.scp_grid_forces.argtypes = [VP, DPP, C.c_int]
         L.scp_grid_gauss_project.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.scp_grid_set_g_offset.argtypes = [VP, C.c_double]
@@ -247,6 +248,13 @@ class Grid:
     def diag(self):
         out = (C.c_double * len(DIAG_NAMES))()
         _check(lib().scp_grid_diag(self._h, out, len(DIAG_NAMES)), "scp_grid_diag")
+        return dict(zip(DIAG_NAMES, list(out)))
+
+    def step_diag(self, n=1):
+        """n steps, then the diagnostics of the new state, fused into the last
+        step launch where the grid runs the march (see scp_grid_step_diag)."""
+        out = (C.c_double * len(DIAG_NAMES))()
+        _check(lib().scp_grid_step_diag(self._h, int(n), out, len(DIAG_NAMES)), "scp_grid_step_diag")
         return dict(zip(DIAG_NAMES, list(out)))
 
     def forces(self):

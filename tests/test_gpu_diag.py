@@ -1,0 +1,127 @@
+"""Diagnostics fused into the step (scp_grid_step_diag) through the C-ABI.
+
+The emitting launch of the LDS-staged march accumulates the energy, charge and
+moment sums of the state it emits; a light pass adds the Gauss-law statistics.
+Bars:
+  * the state is bitwise the state of scp_grid_step;
+  * FP64: every diagnostic equals the stand-alone pass (scp_grid_diag) on the
+    same state to summation-order rounding (1e-13 relative), maxima, argmax
+    voxel quantities and probes exactly; and matches the oracle restatement
+    of the reference to 1e-12 (golden fixtures);
+  * FP32C: the per-site terms read the own site as hi + lo and the staged
+    neighbours as FP32 hi words, so the gradient and plaquette sums differ
+    from the pass over the emitted hi + lo state by the FP32 rounding of the
+    neighbour values: 2e-7 relative on the energy terms. The FP32C kernels are
+    built with FMA contraction, so own-site quantities (charges, kinetic and
+    potential terms, maxima) agree to 1e-13 relative rather than bitwise, and
+    the Gauss statistics (a residual of ~1e-7 in FP32C) to 1e-9 relative; the
+    argmax-voxel quantity omega_core and the probes read the emitted state
+    and agree exactly.
+"""
+import numpy as np
+import pytest
+
+from oracle import core as oracle
+from scp_amd import grid as G
+from tests import fixtures
+from tests.test_gpu_parity import gpu_params, seeded
+
+pytestmark = pytest.mark.gpu
+
+EXACT = ("phi_max", "P_max", "s_max", "gauss_max", "omega_core", "thp1u", "thp1v", "thp2u", "thp2v")
+OWN_SITE = ("E_phi_kin", "E_theta_kin", "E_mass", "E_pot", "E_tmass", "P_int", "theta_rms", "Q_phi",
+            "Q_theta", "Q_total", "Q_core", "Q_p0", "Q_p1", "Q_p2", "r_core", "gauss_l2", "Q_flux")
+
+
+def _pair(params, st, steps):
+    """(fused diagnostics, state) and (stand-alone diagnostics, state) after `steps`."""
+    with G.Grid(params) as g:
+        g.upload(st)
+        d_f = g.step_diag(steps)
+        s_f = g.download()
+        n_f = g.steps_done
+    with G.Grid(params) as g:
+        g.upload(st)
+        g.step(steps)
+        d_s = g.diag()
+        s_s = g.download()
+    assert n_f == steps
+    return d_f, s_f, d_s, s_s
+
+
+def _case(mode, N, bc):
+    L = 0.25 * (N - 1)
+    st = seeded(N, mode, L, link_sigma=0.05)[: 24 if mode == 1 else 30]
+    kw = dict(g=0.1) if mode == 2 else {}
+    if bc == 0:
+        kw.update(bc_type=0, damp_width=0.2 * L, damp_rate=0.05)
+    return L, st, kw
+
+
+@pytest.mark.parametrize("mode,N,bc", [(1, 10, 2), (2, 10, 2), (1, 37, 2), (2, 70, 2), (2, 33, 0), (1, 24, 0)])
+def test_fused_diag_equals_standalone_f64(mode, N, bc):
+    L, st, kw = _case(mode, N, bc)
+    print(f"case f64 mode={mode} N={N} bc={bc}", flush=True)
+    d_f, s_f, d_s, s_s = _pair(G.make_params(N, mode, L, 0.1, **kw), st, 9)
+    assert np.array_equal(s_f, s_s)
+    for k in G.DIAG_NAMES:
+        if k in EXACT:
+            assert d_f[k] == d_s[k], f"{k}: fused {d_f[k]!r} stand-alone {d_s[k]!r}"
+        else:
+            tol = 1e-13 * max(1.0, abs(d_s[k]))
+            assert abs(d_f[k] - d_s[k]) <= tol, f"{k}: fused {d_f[k]!r} stand-alone {d_s[k]!r}"
+
+
+@pytest.mark.parametrize("mode,N,bc", [(2, 33, 2), (1, 37, 2), (2, 48, 0)])
+def test_fused_diag_f32c(mode, N, bc):
+    L, st, kw = _case(mode, N, bc)
+    print(f"case f32c mode={mode} N={N} bc={bc}", flush=True)
+    d_f, s_f, d_s, s_s = _pair(G.make_params(N, mode, L, 0.1, precision=G.PREC_F32C, lo_words=1, **kw), st, 9)
+    assert np.array_equal(s_f, s_s)
+    worst = {}
+    for k in G.DIAG_NAMES:
+        err = abs(d_f[k] - d_s[k])
+        if k in ("omega_core", "thp1u", "thp1v", "thp2u", "thp2v"):
+            assert d_f[k] == d_s[k], f"{k}: fused {d_f[k]!r} stand-alone {d_s[k]!r}"
+        elif k in ("gauss_max", "gauss_l2"):
+            assert err <= 1e-9 * abs(d_s[k]) + 1e-300, f"{k}: fused {d_f[k]!r} stand-alone {d_s[k]!r}"
+        elif k in OWN_SITE or k in EXACT:
+            assert err <= 1e-13 * max(1.0, abs(d_s[k])), f"{k}: fused {d_f[k]!r} stand-alone {d_s[k]!r}"
+        else:
+            worst[k] = err / max(abs(d_s[k]), 1e-300)
+            assert err <= 2e-7 * max(abs(d_s[k]), 1e-3 * abs(d_s["E_total"])), \
+                f"{k}: fused {d_f[k]!r} stand-alone {d_s[k]!r}"
+    print("relative differences of the neighbour terms:", {k: f"{v:.1e}" for k, v in worst.items()})
+
+
+@pytest.mark.parametrize("name", ["complex_n8", "complex_damp_n10", "gauged_n8", "gauged_damp_n10"])
+def test_fused_diag_matches_oracle(name):
+    """The fused diagnostics against the oracle restatement of compute_energy_complex*,
+    compute_charges and compute_gauss on the state the step emits."""
+    fx = fixtures.load(name)
+    n, p = fixtures.nsteps(fx)
+    with G.Grid(gpu_params(fx)) as g:
+        g.upload(fx["frame0"])
+        d = g.step_diag(5)
+        st = g.download()
+    ref = oracle.diag(p, st)
+    for k in G.DIAG_NAMES:
+        if k in ("gauss_max", "gauss_l2"):      # rounding-noise level quantities (~1e-14)
+            assert abs(d[k] - ref[k]) <= 1e-12, k
+            continue
+        tol = 1e-12 * max(1.0, abs(ref[k]))
+        assert abs(d[k] - ref[k]) <= tol, f"{k}: gpu {d[k]!r} oracle {ref[k]!r}"
+
+
+def test_fused_diag_other_grids_take_the_standalone_pass():
+    """Real mode and the direct-neighbour kernel: step_diag = step + diag."""
+    for mode, sk in ((0, 0), (2, 1)):
+        N = 12
+        L = 0.25 * (N - 1)
+        st = seeded(N, max(mode, 1), L, link_sigma=0.05)
+        st = st[:12] if mode == 0 else st
+        kw = dict(g=0.1) if mode == 2 else {}
+        d_f, s_f, d_s, s_s = _pair(G.make_params(N, mode, L, 0.1, step_kernel=sk, **kw), st, 4)
+        assert np.array_equal(s_f, s_s)
+        for k in G.DIAG_NAMES:
+            assert d_f[k] == d_s[k], k
