@@ -34,26 +34,45 @@ __device__ inline double diag_site(const KParams &kp, const DiagParams &dp, cons
         fu[a] = xc[F_U + a]; fv[a] = xc[F_V + a];
         fu[3 + a] = xc[F_TU + a]; fv[3 + a] = xc[F_TV + a];
     }
-    // covariant (gauged) or plain central differences, C:1838-1849, 1915-1931
-    double DU[3][6], DV[3][6];
+    // covariant (gauged) or plain central differences, C:1838-1849, 1915-1931,
+    // formed per field as consumed (only the six Theta differences the
+    // covariant curls need are kept); every sum keeps the reference's order
+    double cP[3], sP[3], cM[3], sM[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        double cP = 1, sP = 0, cM = 1, sM = 0;
-        if (gauged) { sincos(xc[F_TH + d], &sP, &cP); sincos(xn(F_TH + d, d, -1), &sM, &cM); }
+        cP[d] = 1; sP[d] = 0; cM[d] = 1; sM[d] = 0;
+        if (gauged) { sincos(xc[F_TH + d], &sP[d], &cP[d]); sincos(xn(F_TH + d, d, -1), &sM[d], &cM[d]); }
+    }
+    // D of field pair (fu_, fv_) along d: (re, im)
+    auto D = [&](int fu_, int fv_, int d, double &re, double &im) {
+        const double up = xn(fu_, d, 1), vp = xn(fv_, d, 1), um = xn(fu_, d, -1), vm = xn(fv_, d, -1);
+        if (gauged) {
+            re = ((cP[d] * up - sP[d] * vp) - (cM[d] * um + sM[d] * vm)) * h1;
+            im = ((cP[d] * vp + sP[d] * up) - (cM[d] * vm - sM[d] * um)) * h1;
+        } else {
+            re = (up - um) * h1;
+            im = (vp - vm) * h1;
+        }
+    };
 #pragma unroll
-        for (int a = 0; a < 3; a++)
+    for (int a = 0; a < 3; a++) {          // gradient energy of Phi_a
+        double u0, v0, u1, v1, u2, v2;
+        D(F_U + a, F_V + a, 0, u0, v0);
+        D(F_U + a, F_V + a, 1, u1, v1);
+        D(F_U + a, F_V + a, 2, u2, v2);
+        S[DS_EG] += 0.5 * (u0 * u0 + u1 * u1 + u2 * u2 + v0 * v0 + v1 * v1 + v2 * v2) * dV;
+    }
+    double TU[3][3], TV[3][3];             // Theta differences [d][b], d != b (curls)
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int fu_ = h ? F_TU + a : F_U + a, fv_ = h ? F_TV + a : F_V + a;
-                const double up = xn(fu_, d, 1), vp = xn(fv_, d, 1), um = xn(fu_, d, -1), vm = xn(fv_, d, -1);
-                if (gauged) {
-                    DU[d][h * 3 + a] = ((cP * up - sP * vp) - (cM * um + sM * vm)) * h1;
-                    DV[d][h * 3 + a] = ((cP * vp + sP * up) - (cM * vm - sM * um)) * h1;
-                } else {
-                    DU[d][h * 3 + a] = (up - um) * h1;
-                    DV[d][h * 3 + a] = (vp - vm) * h1;
-                }
-            }
+    for (int b = 0; b < 3; b++) {          // gradient energy of Theta_b
+        double u[3], v[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            D(F_TU + b, F_TV + b, d, u[d], v[d]);
+            TU[d][b] = u[d];
+            TV[d][b] = v[d];
+        }
+        S[DS_ETG] += 0.5 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2] + v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * dV;
     }
     double s2[3], rho2 = 0, s = 1.0, qpv = 0, qtv = 0;
 #pragma unroll
@@ -62,18 +81,14 @@ __device__ inline double diag_site(const KParams &kp, const DiagParams &dp, cons
         const double ud = vc[F_U + a], vd = vc[F_V + a], tud = vc[F_TU + a], tvd = vc[F_TV + a];
         S[DS_EPK] += 0.5 * (ud * ud + vd * vd) * dV;
         S[DS_ETK] += 0.5 * (tud * tud + tvd * tvd) * dV;
-        S[DS_EG] += 0.5 * (DU[0][a] * DU[0][a] + DU[1][a] * DU[1][a] + DU[2][a] * DU[2][a]
-                         + DV[0][a] * DV[0][a] + DV[1][a] * DV[1][a] + DV[2][a] * DV[2][a]) * dV;
         if (gauged) S[DS_EM] += 0.5 * kp.MASS2 * s2[a] * dV;
         else S[DS_EM] += 0.5 * kp.MASS2 * (fu[a] * fu[a] + fv[a] * fv[a]) * dV;
-        S[DS_ETG] += 0.5 * (DU[0][3 + a] * DU[0][3 + a] + DU[1][3 + a] * DU[1][3 + a] + DU[2][3 + a] * DU[2][3 + a]
-                          + DV[0][3 + a] * DV[0][3 + a] + DV[1][3 + a] * DV[1][3 + a] + DV[2][3 + a] * DV[2][3 + a]) * dV;
         S[DS_ETM] += 0.5 * kp.MT2 * (fu[3 + a] * fu[3 + a] + fv[3 + a] * fv[3 + a]) * dV;
         M[DM_PHI] = fmax(M[DM_PHI], sqrt(s2[a]));
         double reDxT, imDxT;
         if (gauged) {
-            reDxT = DU[C1[a]][3 + C2[a]] - DU[C2[a]][3 + C1[a]];
-            imDxT = DV[C1[a]][3 + C2[a]] - DV[C2[a]][3 + C1[a]];
+            reDxT = TU[C1[a]][C2[a]] - TU[C2[a]][C1[a]];
+            imDxT = TV[C1[a]][C2[a]] - TV[C2[a]][C1[a]];
         } else {   // ungauged curls read the raw neighbours (C:1868-1869)
             const int b1 = C1[a], b2 = C2[a];
             reDxT = (xn(F_TU + b2, b1, 1) - xn(F_TU + b2, b1, -1) - xn(F_TU + b1, b2, 1) + xn(F_TU + b1, b2, -1)) * h1;

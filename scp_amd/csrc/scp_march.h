@@ -244,6 +244,14 @@ struct S64 {
     }
 };
 
+// FP64 policy of the fused-diagnostics launch: the own column is read back
+// from the staged planes instead of held in registers (same values, so the
+// same results), which leaves the registers to the diagnostics.
+struct S64D : S64 {
+    static constexpr bool REGQ = false;
+    static constexpr bool SPLIT_DAMP = true;
+};
+
 template <int LOK> struct LoIO;
 template <> struct LoIO<LO_F32> {
     __device__ static float get(const void *p, int f, uint32_t fs, uint32_t e) {
@@ -534,6 +542,8 @@ k_march(KParams kp, typename S::B b, int DI) {
 // block reduces its threads' accumulators (wave butterfly, then its waves in
 // order: deterministic) into its row of da.partials, also when it has no
 // planes (the row is then zero).
+// One block per CU: the diagnostics need the registers (the FP32C instance
+// capped for two blocks per CU spills and ran 1.65x slower at 512^3).
 template <class S, int MODE>
 __global__ void __launch_bounds__(MG<1>::MNT, 1)
 k_march_diag(KParams kp, typename S::B b, int DI, DiagArgs da) {

@@ -13,8 +13,8 @@ int launch_march_f64(int mode, bool emit, const KParams &kp, const Bufs<double> 
 
 int launch_march_f64_diag(int mode, const KParams &kp, const Bufs<double> &b, int DI, const DiagArgs &da,
                           hipStream_t s) {
-    if (mode == 2) return launch_march_diag_t<S64, 2>(kp, b, DI, da, s);
-    if (mode == 1) return launch_march_diag_t<S64, 1>(kp, b, DI, da, s);
+    if (mode == 2) return launch_march_diag_t<S64D, 2>(kp, b, DI, da, s);
+    if (mode == 1) return launch_march_diag_t<S64D, 1>(kp, b, DI, da, s);
     return (int)hipErrorInvalidValue;
 }
 
