@@ -49,13 +49,15 @@ typedef struct {
     int32_t precision;    /* SCP_PREC_F64: FP64 state and stencil (parity path);
                              SCP_PREC_F32C: FP32 state + FP32 stencil with compensated
                              (two-sum) position/velocity accumulation */
-    int32_t bc_type;      /* 0 absorbing sphere, 2 periodic (1 unsupported: returns error) */
+    int32_t bc_type;      /* 0 absorbing sphere, 1 gradient pinned (real and complex modes,
+                             FP64), 2 periodic */
     int32_t rmode;        /* real-mode mass coupling: 0 const, 1 inverse, 3 density-kappa */
     int32_t lo_words;     /* SCP_PREC_F32C compensation-word storage: 0 FP32 (8 B/scalar),
                              1 bfloat16 (6 B/scalar), 2 none (plain FP32, 4 B/scalar) */
     int32_t step_kernel;  /* 0: LDS-staged march (default; complex and gauged steps),
                              1: direct-neighbour march (every mode; the real mode always uses it) */
-    int32_t reserved;     /* must be 0 */
+    int32_t gradient_margin;  /* bc_type 1: x-slabs pinned on each face (gradient_margin,
+                                 scp_config.h:110); must be 0 otherwise */
     double L, dx, dt;     /* dx = 2L/(N-1), dt = dt_factor*dx (scp_sim.c:204-205) */
     double m2, mtheta2, eta, mu, kappa, g;
     double kappa_h, alpha_cs, beta_h, inv_alpha, inv_beta, kappa_gamma;

@@ -27,7 +27,8 @@ class OrParams(C.Structure):
                 ("inv_alpha", C.c_double), ("inv_beta", C.c_double), ("kappa_gamma", C.c_double),
                 ("damp_width", C.c_double), ("damp_rate", C.c_double), ("G_offset", C.c_double),
                 ("qdiag_radius", C.c_double), ("qdiag_probe1", C.c_double),
-                ("qdiag_probe2", C.c_double)]
+                ("qdiag_probe2", C.c_double), ("gradient_margin", C.c_int),
+                ("pin", C.POINTER(C.c_double))]
 
 
 def make_params(N, mode, L, dt_factor, **kw):
@@ -88,12 +89,20 @@ def forces(p, state):
     return acc
 
 
-def step(p, state, nsteps, acc=None):
+def step(p, state, nsteps, acc=None, pin=None):
     """Run nsteps of the reference KDK step; returns (state, acc). The initial
-    acceleration is F(state) unless given (the reference's initial force call)."""
+    acceleration is F(state) unless given (the reference's initial force call).
+    bc 1: pin is the pinned state (default: the state given, which is the
+    reference's rule at init, scp_sim.c:2845-2849)."""
     st = np.array(state, dtype=np.float64).reshape(ncols(p.mode), -1).copy()
     a = forces(p, st) if acc is None else np.array(acc, dtype=np.float64)
+    keep = None
+    if p.bc_type == 1:
+        keep = np.array(st if pin is None else pin, dtype=np.float64).reshape(ncols(p.mode), -1).copy()
+        p.pin = _ptr(keep)
     lib().or_step(C.byref(p), _ptr(st), _ptr(a), nsteps)
+    if keep is not None:
+        p.pin = None
     return st, a
 
 

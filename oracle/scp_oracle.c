@@ -352,6 +352,31 @@ static int vel_to_pos(int vc) {
     return vc - 3;       /* E_d (27+d) drives th_d (24+d) */
 }
 
+/* apply_gradient_bc, scp_sim.c:1480-1555 (stage 5 of bc 1, real and complex
+ * modes): the x-slabs i < M and i >= N-M return to the pinned positions and
+ * velocities with zero acceleration; then, for i in [M, N-M), the rows j = 0
+ * and j = N-1 of every matter position and velocity column are extrapolated
+ * linearly from the two rows inside. */
+static void gradient_bc(const OrParams *p, double *state, double *acc) {
+    const int N = p->N, M = p->gradient_margin;
+    const long NN = (long)N * N, n3 = NN * N;
+    const int ncol = p->mode >= 1 ? 24 : 12, nacc = p->mode >= 1 ? 12 : 6;
+    for (long c = 0; c < n3; c++) {
+        const int i = (int)(c / NN);
+        if (i < M || i >= N - M) {
+            for (int col = 0; col < ncol; col++) state[col * n3 + c] = p->pin[col * n3 + c];
+            for (int b = 0; b < nacc; b++) acc[b * n3 + c] = 0;
+        }
+    }
+    for (int i = M; i < N - M; i++)
+        for (int k = 0; k < N; k++)
+            for (int col = 0; col < ncol; col++) {
+                double *f = state + col * n3 + (long)i * NN + k;
+                f[0] = 2 * f[N] - f[2L * N];
+                f[(long)(N - 1) * N] = 2 * f[(long)(N - 2) * N] - f[(long)(N - 3) * N];
+            }
+}
+
 /* verlet_step, scp_sim.c:1583-1716: KDK with stored accelerations. */
 void or_step(const OrParams *p, double *state, double *acc, int nsteps) {
     const int N = p->N; const long n3 = (long)N * N * N;
@@ -389,7 +414,8 @@ void or_step(const OrParams *p, double *state, double *acc, int nsteps) {
             #pragma omp parallel for schedule(static)
             for (long c = 0; c < n3; c++) v[c] += hdt * a[c];
         }
-        if (p->bc_type == 0) damp(p, state);                  /* stage 5: C:1712-1713 */
+        if (p->bc_type == 0) damp(p, state);                  /* stage 5: C:1712-1715 */
+        else if (p->bc_type == 1) gradient_bc(p, state, acc);
     }
 }
 

@@ -32,7 +32,7 @@ extern "C" {
 typedef struct {
     int    N;
     int    mode;          /* 0 real, 1 complex, 2 gauged */
-    int    bc_type;       /* 0 absorbing sphere, 1 gradient pinned (unsupported here), 2 periodic */
+    int    bc_type;       /* 0 absorbing sphere, 1 gradient pinned, 2 periodic */
     int    rmode;         /* real-mode mass coupling: 0 const, 1 inverse, 3 density-kappa */
     double L, dx, dt;
     double m2, mtheta2, eta, mu, kappa, g;
@@ -41,6 +41,8 @@ typedef struct {
     double damp_width, damp_rate;
     double G_offset;      /* frozen jellium offset from the Gauss projection */
     double qdiag_radius, qdiag_probe1, qdiag_probe2;
+    int    gradient_margin;   /* bc 1: pinned x-slabs per face */
+    const double *pin;        /* bc 1: the pinned state (same layout as the state) */
 } OrParams;
 
 /* Diagnostics vector: every column of the reference diag.tsv row for one time

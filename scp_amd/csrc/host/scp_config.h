@@ -56,7 +56,7 @@ void scp_cfg_print(const scp_config *c);
 int  scp_cfg_validate(const scp_config *c, char *err, int errlen);
 /* Sectors of the reference that this implementation does not run (multi-
  * fabric, Higgs, second gauge, locks, GPU-only real-mode experiments, vector
- * frames, self-tune, bc_type=1 in complex/gauged...). Returns -1 with a
+ * frames, self-tune...). Returns -1 with a
  * message naming the key when the config needs one of them. */
 int  scp_cfg_unsupported(const scp_config *c, char *err, int errlen);
 /* KVMD set 0 (25/27/29 keys, scp_init.h:505-556). Returns the pair count;

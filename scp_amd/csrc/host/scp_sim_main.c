@@ -216,6 +216,7 @@ int main(int argc, char **argv) {
     memset(&p, 0, sizeof p);
     p.N = c.N; p.mode = mode; p.precision = c.step_precision ? SCP_PREC_F32C : SCP_PREC_F64;
     p.bc_type = c.bc_type; p.rmode = c.mode;
+    p.gradient_margin = c.bc_type == 1 ? c.gradient_margin : 0;
     p.L = c.L; p.dx = dx; p.dt = dt;
     p.m2 = c.m2; p.mtheta2 = c.mtheta2; p.eta = c.eta; p.mu = c.mu; p.kappa = c.kappa; p.g = c.g_gauge;
     p.kappa_h = c.kappa_h; p.alpha_cs = c.alpha_cs; p.beta_h = c.beta_h;
@@ -261,6 +262,9 @@ int main(int argc, char **argv) {
         if (!gauged) die("ERROR: test_gauge_xform=1 requires complex_gauge=1 and g_gauge != 0");
         test_gauge_xform(&x, &c, dx);
     }
+    if (c.bc_type == 1)   /* the uploaded state is the pinned state (scp_sim.c:2845-2849) */
+        printf("Gradient BC: pinned %d slabs on each x-face (A_high=%.3f, A_low=%.3f)\n\n",
+               c.gradient_margin, c.gradient_A_high, c.gradient_A_low);
 
     int sfa_dtype = c.precision == 0 ? SFAIO_F16 : (c.precision == 1 ? SFAIO_F32 : SFAIO_F64);
     x.out = sfaio_create(c.output, c.N, c.N, c.N, c.L, c.L, c.L, dt);

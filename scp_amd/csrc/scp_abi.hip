@@ -28,7 +28,7 @@ namespace scp_internal {
 bool use_march(const scp_grid *g, int kind) {
     const uint64_t field_bytes = (uint64_t)g->kp.fsh * (g->f32 ? 4u : 8u);
     return kind == K_STEP && g->p.step_kernel == 0 && (g->step_mode == 1 || g->step_mode == 2) &&
-           field_bytes < (1ull << 32);
+           g->p.bc_type != 1 && field_bytes < (1ull << 32);
 }
 
 static thread_local char g_err[512];
@@ -95,9 +95,15 @@ static int check_params(const scp_params *p) {
     if (p->precision == SCP_PREC_F32C && (p->lo_words < 0 || p->lo_words > 2))
         return set_err(-1, "lo_words=%d invalid", p->lo_words);
     if (p->step_kernel < 0 || p->step_kernel > 1) return set_err(-1, "step_kernel=%d invalid", p->step_kernel);
-    if (p->reserved != 0) return set_err(-1, "reserved must be 0");
-    if (p->bc_type == 1) return set_err(-3, "bc_type=1 (gradient pinned) is not implemented");
-    if (p->bc_type != 0 && p->bc_type != 2) return set_err(-1, "bc_type=%d invalid", p->bc_type);
+    if (p->bc_type < 0 || p->bc_type > 2) return set_err(-1, "bc_type=%d invalid", p->bc_type);
+    if (p->bc_type == 1) {   // gradient pinned (apply_gradient_bc, scp_sim.c:1480-1555)
+        if (p->mode == 2)     // as scp_config.h:492-494
+            return set_err(-1, "bc_type=1 is incompatible with the gauged mode (pinning A is gauge-dependent)");
+        if (p->precision != SCP_PREC_F64) return set_err(-3, "bc_type=1 runs on the FP64 path only");
+        if (p->gradient_margin < 0) return set_err(-1, "gradient_margin=%d must be >= 0", p->gradient_margin);
+    } else if (p->gradient_margin != 0) {
+        return set_err(-1, "gradient_margin applies to bc_type=1 only (must be 0)");
+    }
     if (!(p->dx > 0) || !(p->dt > 0)) return set_err(-1, "dx/dt must be positive");
     if (p->mode == 2 && p->g < 0) return set_err(-1, "g must be >= 0");
     return 0;
@@ -113,6 +119,7 @@ extern "C" int64_t scp_grid_bytes(const scp_params *p) {
     if (p->precision == SCP_PREC_F64) b += 3LL * NP * fh * 8;           // P[2], W
     else b += 3LL * NP * fh * (4 + (p->lo_words == 0 ? 4 : p->lo_words == 1 ? 2 : 0));
     if (p->mode == 0 && (p->alpha_cs != 0 || p->beta_h != 0)) b += 6LL * fs * 8;
+    if (p->bc_type == 1) b += 2LL * NP * fs * 8 + 6LL * NP * p->N * p->N * 8;   // pins, side rows
     b += 1024LL * 32 * 8 + 4096;
     return b;
 }
@@ -144,6 +151,8 @@ static KParams make_kparams(const scp_params &p, int step_mode) {
     k.Rd = p.L - p.damp_width;               // scp_sim.c:1444
     k.DW = p.damp_width; k.DR = p.damp_rate;
     k.damp = (p.bc_type == 0 && p.damp_width > 0 && p.damp_rate > 0) ? 1 : 0;
+    k.bc1 = p.bc_type == 1 ? 1 : 0;
+    k.gm = p.bc_type == 1 ? p.gradient_margin : 0;
     return k;
 }
 
@@ -154,6 +163,7 @@ static void free_grid(scp_grid *g) {
     hipFree(g->Plo[0]); hipFree(g->Plo[1]); hipFree(g->Wlo);
     hipFree(g->aux); hipFree(g->acc); hipFree(g->partials); hipFree(g->dout); hipFree(g->argmin);
     hipFree(g->rho); hipFree(g->fpart);
+    hipFree(g->pinx); hipFree(g->pinv); hipFree(g->side);
     if (g->ev0) hipEventDestroy(g->ev0);
     if (g->ev1) hipEventDestroy(g->ev1);
     if (g->stream) hipStreamDestroy(g->stream);
@@ -200,6 +210,11 @@ extern "C" int scp_grid_create(const scp_params *p, scp_grid **out) {
     }
     e = e ? e : hipMalloc(&g->xo, grp);
     e = e ? e : hipMalloc(&g->vo, grp);
+    if (p->bc_type == 1) {
+        e = e ? e : hipMalloc(&g->pinx, grp);
+        e = e ? e : hipMalloc(&g->pinv, grp);
+        e = e ? e : hipMalloc(&g->side, (size_t)6 * g->NP * p->N * p->N * sizeof(double));
+    }
     if (p->mode == 0 && (p->alpha_cs != 0 || p->beta_h != 0)) {
         e = e ? e : hipMalloc(&g->aux, 6 * fsb);
         e = e ? e : hipMemsetAsync(g->aux, 0, 6 * fsb, g->stream);
@@ -244,6 +259,9 @@ Bufs<double> scp_internal::bufs_for(scp_grid *g, int kind) {
     b.w = g->W;
     b.aux = g->aux;
     b.acc = g->acc;
+    b.pinx = g->pinx;
+    b.pinv = g->pinv;
+    b.side = g->side;
     if (kind == K_INIT) { b.pin = g->xo; b.pout = g->P[0]; }
     else if (kind == K_STEP) { b.pin = g->P[g->cur]; b.pout = g->P[g->cur ^ 1]; }
     else { b.pin = g->xo; }
@@ -286,6 +304,11 @@ int scp_internal::run_init(scp_grid *g) {
     rc = launch_kind(g, K_INIT, false);
     if (rc) return rc;
     g->cur = 0;
+    if (g->pinx) {   // bc 1: the state at init is the pinned state (grid_save_pinned, C:381-400, 2845-2849)
+        const size_t nb = (size_t)g->NP * g->kp.fs * sizeof(double);
+        HIPCHK(hipMemcpyAsync(g->pinx, g->xo, nb, hipMemcpyDeviceToDevice, g->stream));
+        HIPCHK(hipMemcpyAsync(g->pinv, g->vo, nb, hipMemcpyDeviceToDevice, g->stream));
+    }
     return 0;
 }
 
@@ -357,6 +380,8 @@ static int enqueue_steps(scp_grid *g, int nsteps, const DiagArgs *da = nullptr) 
                                "march64 diag");
         else
             rc = launch_kind(g, K_STEP, emit);
+        if (!rc && g->side)   // bc 1: the y edges, from this launch's side rows
+            rc = kerr(launch_bc1_edges(emit, g->NP, g->kp, bufs_for(g, K_STEP), g->stream), "bc1 edges");
         if (rc) return rc;
         g->cur ^= 1;
     }

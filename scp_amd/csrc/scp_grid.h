@@ -31,6 +31,8 @@ struct scp_grid {
     double *rho;      // fused diagnostics: Gauss charge density (N^3, gauged), lazily allocated
     double *fpart;    // fused diagnostics: partial rows of the step and the Gauss tail, lazily allocated
     int fpart_rows;
+    double *pinx, *pinv;   // bc 1: pinned positions / velocities (emitted layout)
+    double *side;          // bc 1: side rows of the y-edge pass
     unsigned long long *argmin;
     int cur;
     int nblocks_diag;

@@ -29,7 +29,7 @@ struct KParams {
     double KH, ACS, BH, IA, IB, KG;   // real-mode optional terms
     double L, dx, Rd, DW, DR;  // sponge: R_damp = L - damp_width
     int32_t damp;              // 1 when the sponge is active (bc 0 with width, rate > 0)
-    int32_t pad;
+    int32_t bc1;               // 1: gradient-pinned boundary (bc 1), x-slabs of gm planes per face
     // Layout of the step state P, W and their compensation words (the arrays
     // the step streams): rows padded to 32 elements (128 B in FP32), so every
     // 64-site row segment of a field starts on a cache-line boundary whatever
@@ -38,6 +38,8 @@ struct KParams {
     uint32_t pitch;            // row pitch (elements) of P/W: N rounded up to 32 (N in the real mode)
     uint32_t plane;            // N * pitch
     uint32_t fsh;              // field stride of P/W: N * plane rounded up to 64
+    int32_t gm;                // bc 1: gradient_margin
+    int32_t pad;
 };
 
 // Device buffers. Each group is one allocation of nfields * fs elements.
@@ -53,6 +55,12 @@ struct Bufs {
     double *vo;     // emitted velocities (FP64), also the INIT source
     double *acc;    // forces-only output (FP64)
     const double *aux;  // real mode: Cosserat mismatch M[3] and hardening Q[3] (6 fields)
+    // gradient-pinned boundary (bc 1, FP64): pinned positions / velocities
+    // (emitted layout) and the side rows the y-edge pass reads, per field f:
+    // side[(f*6 + r)*N*N + i*N + k], r = 0/1 acceleration at j = 0 / N-1,
+    // r = 2..5 velocity at j = 1, 2, N-2, N-3
+    const double *pinx, *pinv;
+    double *side;
 };
 
 enum StepKind { K_INIT = 0, K_STEP = 1, K_FORCES = 2 };
@@ -121,6 +129,9 @@ inline int march_di(int N, int KW) {
     return di;
 }
 int launch_real_aux(const KParams &kp, const double *pin, double *aux, hipStream_t s);
+// gradient-pinned boundary: the y-edge rows after a K_STEP launch of the
+// direct FP64 kernel (nf fields; b: that launch's buffers)
+int launch_bc1_edges(bool emit, int nf, const KParams &kp, const Bufs<double> &b, hipStream_t s);
 int launch_real_aux_f32(const KParams &kp, const float *pin, double *aux, hipStream_t s);
 
 // Diagnostics: pass 1 writes per-block partials; the finish kernel reduces

@@ -112,6 +112,7 @@ extern "C" int scp_server_submit(scp_server *s, const scp_job *job, int64_t *id)
     if (job->nsteps < 1) return set_err(-1, "nsteps=%d must be >= 1", job->nsteps);
     if (job->nballs < 0 || (job->nballs > 0 && !job->balls)) return set_err(-1, "bad ball list");
     if (job->params.step_kernel != 0) return set_err(-1, "jobs run the LDS-staged march (step_kernel 0)");
+    if (job->params.bc_type == 1) return set_err(-1, "jobs run the periodic or sponge boundary (bc_type 0 or 2)");
     const int64_t bytes = scp_grid_bytes(&job->params);
     if (bytes < 0) return -1;   // message set by the parameter check
     if (bytes > s->budget)

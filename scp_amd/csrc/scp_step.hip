@@ -25,6 +25,22 @@ namespace scp {
 // the fused step kernel, FP64 storage
 // ---------------------------------------------------------------------------
 
+// Gradient-pinned boundary (bc 1, apply_gradient_bc, scp_sim.c:1480-1555), the
+// part a site can do alone: what the y-edge pass needs from this launch — the
+// edge rows' accelerations (the edges keep them) and the velocities of the two
+// rows inside each edge (the x-slab sites are pinned in the kernel itself).
+__device__ inline void bc1_side(const KParams &kp, const Bufs<double> &b, int f, int i, int j, int k, double a,
+                                double v) {
+    const int N = kp.N;
+    const size_t NN = (size_t)N * N, base = (size_t)f * 6 * NN + (size_t)i * N + k;
+    if (j == 0) b.side[base] = a;
+    if (j == N - 1) b.side[base + NN] = a;
+    if (j == 1) b.side[base + 2 * NN] = v;
+    if (j == 2) b.side[base + 3 * NN] = v;
+    if (j == N - 2) b.side[base + 4 * NN] = v;
+    if (j == N - 3) b.side[base + 5 * NN] = v;
+}
+
 template <int MODE, int KIND, bool EMIT>
 __global__ void __launch_bounds__(TK * TJ)
 k_step64(KParams kp, Bufs<double> b, int DI) {
@@ -86,19 +102,33 @@ k_step64(KParams kp, Bufs<double> b, int DI) {
         } else {
             double dmp = 1.0;
             const bool damp = (KIND == K_STEP) && kp.damp && sponge(kp, i, j, k, dmp);
+            // bc 1: the x-slabs (i < M or i >= N - M) return to the pinned state with
+            // zero acceleration (stage 5, C:1484-1508)
+            const bool pinned = (KIND == K_STEP && MODE != 2) && kp.bc1 && (i < kp.gm || i >= N - kp.gm);
 #pragma unroll
             for (int f = 0; f < NP; f++) {
                 const size_t o = (size_t)f * fh + chh, oc = (size_t)f * fs + cc;
-                double v;
+                double v, a = acc[f];
+                double x = c0[f];
                 if constexpr (KIND == K_INIT) {
                     v = b.vo[oc];
                 } else {
                     v = __builtin_nontemporal_load(&b.w[o]);
-                    v += kp.hdt * acc[f];                  // stage 4 (C:1680-1696)
+                    v += kp.hdt * a;                       // stage 4 (C:1680-1696)
                     if (damp) v *= dmp;                    // stage 5 (C:1454-1460)
                 }
-                double w = v + kp.hdt * acc[f];            // next stage 1 (C:1597-1613)
-                double x = c0[f];
+                if constexpr (KIND == K_STEP && MODE != 2) {
+                    if (kp.bc1) {                          // stage 5, bc 1
+                        if (pinned) {
+                            x = b.pinx[oc];
+                            v = b.pinv[oc];
+                            a = 0.0;
+                        } else {
+                            bc1_side(kp, b, f, i, j, k, a, v);
+                        }
+                    }
+                }
+                double w = v + kp.hdt * a;                 // next stage 1 (C:1597-1613)
                 double xn;
                 if (MODE == 2 && f >= F_TH) {              // next stage 2, links (C:1641-1651)
                     double t = x + kp.gad * w;
@@ -175,6 +205,53 @@ int launch_step_f64(int mode, int kind, bool emit, const KParams &kp, const Bufs
     if (mode == 2) return launch64<2>(kind, emit, kp, b, s);
     if (mode == 1) return launch64<1>(kind, emit, kp, b, s);
     return launch64<0>(kind, emit, kp, b, s);
+}
+
+// Gradient-pinned boundary, y edges (apply_gradient_bc, C:1510-1553): for i in
+// [M, N-M) the rows j = 0 and j = N-1 of every field are extrapolated linearly
+// from the two rows inside, positions and velocities alike (x_0 = 2 x_1 - x_2);
+// an edge keeps the acceleration of its own force evaluation. Runs after the
+// step launch, whose stencil source (b.pin) still holds the inside rows'
+// positions and whose side rows hold their velocities and the edges'
+// accelerations; recomputes the edges' W and P' (next stages 1-2) and the
+// emitted state. One thread per (field, edge, i, k).
+template <bool EMIT>
+__global__ void k_bc1_edges(KParams kp, Bufs<double> b, int nf) {
+    const int N = kp.N, M = kp.gm, NI = N - 2 * M;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (NI <= 0 || t >= (uint32_t)nf * 2u * (uint32_t)NI * (uint32_t)N) return;
+    const int k = t % N;
+    uint32_t r = t / N;
+    const int i = M + (int)(r % NI);
+    r /= NI;
+    const int e = r % 2, f = r / 2;
+    const int j = e ? N - 1 : 0, j1 = e ? N - 2 : 1, j2 = e ? N - 3 : 2;
+    const size_t NN = (size_t)N * N, base = (size_t)f * 6 * NN + (size_t)i * N + k;
+    const size_t fb = (size_t)f * kp.fsh + (size_t)i * kp.plane + k;
+    const double x1 = b.pin[fb + (size_t)j1 * kp.pitch], x2 = b.pin[fb + (size_t)j2 * kp.pitch];
+    const double v1 = b.side[base + (e ? 4 : 2) * NN], v2 = b.side[base + (e ? 5 : 3) * NN];
+    const double a = b.side[base + e * NN];
+    const double x = 2 * x1 - x2;
+    const double v = 2 * v1 - v2;
+    const double w = v + kp.hdt * a;                       // next stage 1
+    const double xn = x + kp.dt * w;                       // next stage 2
+    const size_t o = fb + (size_t)j * kp.pitch;
+    b.w[o] = w;
+    b.pout[o] = xn;
+    if constexpr (EMIT) {
+        const size_t oc = (size_t)f * kp.fs + (size_t)i * NN + (size_t)j * N + k;
+        b.xo[oc] = x;
+        b.vo[oc] = v;
+    }
+}
+
+int launch_bc1_edges(bool emit, int nf, const KParams &kp, const Bufs<double> &b, hipStream_t s) {
+    const int NI = kp.N - 2 * kp.gm;
+    if (NI <= 0) return 0;
+    const uint32_t n = (uint32_t)nf * 2u * (uint32_t)NI * (uint32_t)kp.N;
+    if (emit) hipLaunchKernelGGL((k_bc1_edges<true>), dim3((n + 255) / 256), dim3(256), 0, s, kp, b, nf);
+    else hipLaunchKernelGGL((k_bc1_edges<false>), dim3((n + 255) / 256), dim3(256), 0, s, kp, b, nf);
+    return (int)hipGetLastError();
 }
 
 int launch_real_aux(const KParams &kp, const double *pin, double *aux, hipStream_t s) {

@@ -51,7 +51,7 @@ class Params(C.Structure):
     """scp_params of include/scp_amd.h."""
     _fields_ = [("N", C.c_int32), ("mode", C.c_int32), ("precision", C.c_int32),
                 ("bc_type", C.c_int32), ("rmode", C.c_int32), ("lo_words", C.c_int32),
-                ("step_kernel", C.c_int32), ("reserved", C.c_int32),
+                ("step_kernel", C.c_int32), ("gradient_margin", C.c_int32),
                 ("L", C.c_double), ("dx", C.c_double), ("dt", C.c_double),
                 ("m2", C.c_double), ("mtheta2", C.c_double), ("eta", C.c_double),
                 ("mu", C.c_double), ("kappa", C.c_double), ("g", C.c_double),

@@ -22,6 +22,8 @@ def load(name):
         if k in prm:
             kw[k] = float(prm[k])
     kw["bc_type"] = int(prm.get("bc_type", 2))
+    if "gradient_margin" in prm:
+        kw["gradient_margin"] = int(prm["gradient_margin"])
     if "mode" in prm:
         kw["rmode"] = int(prm["mode"])
     if "g_gauge" in prm:

@@ -77,6 +77,10 @@ CASES = {
     "gauged_g03_n8": (2, 8, 3.5, {"complex_gauge": 1, "g_gauge": 0.3},
                       [(-2.0, 0, 0, 0.8, 1.2, 0.8, 1), (2.0, 0, 0, 0.8, 1.2, -0.8, -1)],
                       dict(link_sigma=0.3), 2.0, None),
+    "real_gradient_n10": (0, 10, 4.5, {"bc_type": 1, "gradient_margin": 2},
+                          [(0.7, -0.4, 0.3, 0.8, 1.5, 0, 1)], dict(vel_noise=0.05), 2.0, None),
+    "complex_gradient_n10": (1, 10, 4.5, {"bc_type": 1, "gradient_margin": 2},
+                             [(0.6, 0.5, -0.3, 0.8, 1.5, 0.8, 1)], dict(vel_noise=0.05), 2.0, None),
     "gauged_damp_n10": (2, 10, 4.5, {"complex_gauge": 1, "g_gauge": 0.1, "bc_type": 0,
                                      "damp_width": 2.0, "damp_rate": 0.05},
                         [(0, 0, 0, 0.8, 1.5, 0.8, 1)], dict(link_sigma=1e-2), 2.0, None),

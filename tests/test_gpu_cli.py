@@ -56,7 +56,8 @@ def compare_diag(text_ref, text_mine, skip=()):
 
 
 @pytest.mark.parametrize("name", ["complex_n8", "real_n8", "real_oscillon_n10", "gauged_n8", "gauged_damp_n10",
-                                  "complex_damp_n10", "real_terms_n8"])
+                                  "complex_damp_n10", "real_terms_n8", "real_gradient_n10",
+                                  "complex_gradient_n10"])
 def test_cli_reproduces_reference_run(tmp_path, name):
     fx = fixtures.load(name)
     seed = None
@@ -67,6 +68,9 @@ def test_cli_reproduces_reference_run(tmp_path, name):
     print(res.stdout[-2000:], res.stderr[-2000:], flush=True)
     assert res.returncode == 0
     assert "=== COMPLETE ===" in res.stdout and "Wall:" in res.stdout
+    for line in fx["stdout"].splitlines():   # the boundary banner lines of the reference
+        if line.startswith("Gradient BC:") or line.startswith("BC:"):
+            assert line in res.stdout.splitlines(), line
     with sfa.SFAFile(out) as f:
         assert f.total_frames == 2
         a0, a1 = f.read_frame(0), f.read_frame(1)

@@ -83,6 +83,31 @@ def seeded(N, mode, L, seed=7, link_sigma=0.0):
     return seeds.seed_balls(N, L, mode, balls, theta_sigma=0.01, link_sigma=link_sigma, seed=seed)
 
 
+@pytest.mark.parametrize("mode,N,margin,steps", [(0, 37, 3, 15), (1, 37, 3, 15), (1, 20, 0, 10), (0, 4, 1, 6),
+                                                 (1, 70, 5, 8)])
+def test_gradient_bc_matches_oracle(mode, N, margin, steps):
+    """bc 1 (pinned x-slabs, extrapolated y edges) against the oracle restatement
+    of apply_gradient_bc, bitwise; ragged tiles (N = 37, 70), no slabs (margin 0),
+    the smallest grid (N = 4: the edge source rows touch each other)."""
+    L = 0.25 * (N - 1)
+    st = seeded(N, max(mode, 1), L)
+    st = st[:12] if mode == 0 else st[:24]
+    print(f"case bc1 mode={mode} N={N} margin={margin} steps={steps}", flush=True)
+    p_o = oracle.make_params(N, mode, L, 0.1, bc_type=1, gradient_margin=margin)
+    ref, _ = oracle.step(p_o, st, steps)
+    with G.Grid(G.make_params(N, mode, L, 0.1, bc_type=1, gradient_margin=margin)) as g:
+        g.upload(st)
+        g.step(steps - 3)
+        g.step(3)
+        out = g.download()
+        d = g.step_diag(1)
+    ref1, _ = oracle.step(p_o, st, steps + 1)
+    assert np.array_equal(out, ref.reshape(out.shape)), f"max diff {np.abs(out - ref.reshape(out.shape)).max():.3e}"
+    dref = oracle.diag(p_o, ref1)
+    for k in ("E_total", "E_grad", "Q_total", "phi_max"):
+        assert abs(d[k] - dref[k]) <= 1e-12 * max(1.0, abs(dref[k])), k
+
+
 @pytest.mark.parametrize("mode,N,steps", [(1, 24, 40), (2, 24, 40), (0, 20, 40), (1, 37, 12), (2, 33, 12)])
 def test_step_matches_oracle_larger(mode, N, steps):
     """Larger and odd-sized (ragged tile) grids vs the oracle restatement."""
@@ -105,7 +130,8 @@ def test_step_matches_oracle_larger(mode, N, steps):
 
 
 @pytest.mark.parametrize("name", ["complex_n8", "complex_damp_n10", "gauged_n8", "gauged_damp_n10",
-                                  "real_n8", "real_terms_n8", "real_mode1_n8"])
+                                  "real_n8", "real_terms_n8", "real_mode1_n8", "real_gradient_n10",
+                                  "complex_gradient_n10"])
 def test_diag_matches_oracle(name):
     fx = fixtures.load(name)
     n, p = fixtures.nsteps(fx)

@@ -211,7 +211,14 @@ def test_abi_library_loads_and_refuses_bad_params():
     """Loadable without a GPU; argument errors come back as status codes, not exits."""
     p = G.make_params(8, 2, 3.5, 0.1)
     p.bc_type = 1
-    assert G.grid_bytes(p) == -1
+    assert G.grid_bytes(p) == -1                         # gauged: pinning is gauge-dependent
+    assert "incompatible" in G.last_error()
+    q = G.make_params(8, 1, 3.5, 0.1, bc_type=1, gradient_margin=2)
+    assert G.grid_bytes(q) > G.grid_bytes(G.make_params(8, 1, 3.5, 0.1))   # pins and edge rows
+    q.precision = G.PREC_F32C
+    assert G.grid_bytes(q) == -1 and "FP64" in G.last_error()
+    q = G.make_params(8, 1, 3.5, 0.1, gradient_margin=2)
+    assert G.grid_bytes(q) == -1 and "bc_type=1 only" in G.last_error()
     p.bc_type = 2
     assert G.grid_bytes(p) > 0
     p.N = 3
