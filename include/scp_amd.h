@@ -64,6 +64,12 @@ typedef struct {
     double damp_width, damp_rate;
     double G_offset;      /* frozen jellium Gauss offset (set by gauss_project or the host) */
     double qdiag_radius, qdiag_probe1, qdiag_probe2;
+    /* Real mode only: the terms the reference applies in its GPU step and not
+     * in its CPU step (scp_sim.cu:1144-1248), 0 = off: theta saturation and
+     * conversion (theta_sat, gamma_conv), gradient self-interaction with
+     * chirality bias (sigma_grad, chi_chiral), cubic self-interaction,
+     * frequency-mismatch conversion, cross potential, theta self potential. */
+    double theta_sat, gamma_conv, sigma_grad, chi_chiral, sigma_cubic, sigma_freq, sigma_cross, lambda_self;
 } scp_params;
 
 /* Diagnostics vector filled by scp_grid_diag: every value of one diag.tsv row

@@ -28,7 +28,10 @@ class OrParams(C.Structure):
                 ("damp_width", C.c_double), ("damp_rate", C.c_double), ("G_offset", C.c_double),
                 ("qdiag_radius", C.c_double), ("qdiag_probe1", C.c_double),
                 ("qdiag_probe2", C.c_double), ("gradient_margin", C.c_int),
-                ("pin", C.POINTER(C.c_double))]
+                ("pin", C.POINTER(C.c_double)),
+                ("theta_sat", C.c_double), ("gamma_conv", C.c_double), ("sigma_grad", C.c_double),
+                ("chi_chiral", C.c_double), ("sigma_cubic", C.c_double), ("sigma_freq", C.c_double),
+                ("sigma_cross", C.c_double), ("lambda_self", C.c_double)]
 
 
 def make_params(N, mode, L, dt_factor, **kw):

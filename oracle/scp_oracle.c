@@ -81,6 +81,8 @@ static inline double lap_at(const double *F, long c, const long np[3], const lon
 /* Real mode, scp_sim.c:743-922 (compute_forces), including the CPU-implemented
  * optional terms: mass modes 1/3, chiral kappa_h, Cosserat alpha_cs (two-pass),
  * hardening beta_h (two-pass). */
+static void gpu_terms(const OrParams *p, const double *state, double *acc);
+
 static void forces_real(const OrParams *p, const double *state, double *acc) {
     const int N = p->N; const long n3 = (long)N * N * N;
     const double h2 = 1.0 / (p->dx * p->dx), h1 = 1.0 / (2.0 * p->dx);
@@ -170,6 +172,92 @@ static void forces_real(const OrParams *p, const double *state, double *acc) {
         }
     }
     for (int a = 0; a < 3; a++) { free(M[a]); free(Qh[a]); }
+    gpu_terms(p, state, acc);
+}
+
+/* The real-mode terms the reference applies in its GPU force kernel only
+ * (compute_forces_kernel, scp_sim.cu:1144-1248), restated per site in that
+ * kernel's order, added onto the accelerations of compute_forces. The theta
+ * velocities are the state's (the half-kicked ones inside a step, as there).
+ * The frequency term uses omega = |theta_vel| / (|theta| + 0.01) at the site
+ * and its six neighbours (ip, im, jp, jm, kp, km). */
+static double omega_at(const View *w, long c) {
+    double tv = w->tud[0][c] * w->tud[0][c] + w->tud[1][c] * w->tud[1][c] + w->tud[2][c] * w->tud[2][c];
+    double th = w->tu[0][c] * w->tu[0][c] + w->tu[1][c] * w->tu[1][c] + w->tu[2][c] * w->tu[2][c];
+    return sqrt(tv) / (sqrt(th) + 0.01);
+}
+static void gpu_terms(const OrParams *p, const double *state, double *acc) {
+    const int on = (p->theta_sat > 0 && p->gamma_conv > 0) || p->sigma_grad > 0 || p->sigma_cubic > 0 ||
+                   p->sigma_freq > 0 || p->sigma_cross > 0 || p->lambda_self > 0;
+    if (!on) return;
+    const int N = p->N; const long n3 = (long)N * N * N;
+    const double h1 = 1.0 / (2.0 * p->dx);
+    View w = make_view(state, n3, 0);
+    #pragma omp parallel for schedule(static)
+    for (long c = 0; c < n3; c++) {
+        long np[3], nm[3]; neighbours(N, c, np, nm);
+        double *ap[3], *at[3];
+        for (int a = 0; a < 3; a++) { ap[a] = acc + (0 + a) * n3 + c; at[a] = acc + (3 + a) * n3 + c; }
+        const double t0 = w.tu[0][c], t1 = w.tu[1][c], t2 = w.tu[2][c];
+        const double th[3] = {t0, t1, t2};
+        double ct[3];
+        for (int a = 0; a < 3; a++) ct[a] = curl_at(w.tu, a, np, nm, h1);
+        if (p->theta_sat > 0 && p->gamma_conv > 0) {
+            double th2 = t0 * t0 + t1 * t1 + t2 * t2;
+            double excess = th2 - p->theta_sat * p->theta_sat;
+            if (excess > 0)
+                for (int a = 0; a < 3; a++) {
+                    *at[a] -= p->gamma_conv * excess * th[a];
+                    *ap[a] += p->gamma_conv * excess * ct[a];
+                }
+        }
+        if (p->sigma_grad > 0) {
+            double g2 = 0;
+            for (int a = 0; a < 3; a++) {
+                double dx = (w.tu[a][np[0]] - w.tu[a][nm[0]]) * h1;
+                double dy = (w.tu[a][np[1]] - w.tu[a][nm[1]]) * h1;
+                double dz = (w.tu[a][np[2]] - w.tu[a][nm[2]]) * h1;
+                g2 += dx * dx + dy * dy + dz * dz;
+            }
+            double th2 = t0 * t0 + t1 * t1 + t2 * t2;
+            double h = t0 * ct[0] + t1 * ct[1] + t2 * ct[2];
+            double se = p->sigma_grad * (1.0 + p->chi_chiral * h / (th2 + 0.01));
+            if (se < 0) se = 0;
+            for (int a = 0; a < 3; a++) {
+                *at[a] -= se * g2 * th[a];
+                *ap[a] += se * g2 * ct[a];
+            }
+        }
+        if (p->sigma_cubic > 0) {
+            double th2 = t0 * t0 + t1 * t1 + t2 * t2;
+            for (int a = 0; a < 3; a++) {
+                *at[a] -= p->sigma_cubic * th2 * th[a];
+                *ap[a] += p->sigma_cubic * th2 * ct[a];
+            }
+        }
+        if (p->sigma_freq > 0) {
+            double oc = omega_at(&w, c), dw = 0;
+            long nb[6] = {np[0], nm[0], np[1], nm[1], np[2], nm[2]};
+            for (int q = 0; q < 6; q++) dw += fabs(oc - omega_at(&w, nb[q]));
+            double dwa = dw / 6;
+            for (int a = 0; a < 3; a++) {
+                *at[a] -= p->sigma_freq * dwa * th[a];
+                *ap[a] += p->sigma_freq * dwa * ct[a];
+            }
+        }
+        if (p->sigma_cross > 0) {
+            double th2 = t0 * t0 + t1 * t1 + t2 * t2;
+            double ph2 = w.u[0][c] * w.u[0][c] + w.u[1][c] * w.u[1][c] + w.u[2][c] * w.u[2][c];
+            for (int a = 0; a < 3; a++) {
+                *ap[a] -= p->sigma_cross * th2 * w.u[a][c];
+                *at[a] -= p->sigma_cross * ph2 * th[a];
+            }
+        }
+        if (p->lambda_self > 0) {
+            double th2 = t0 * t0 + t1 * t1 + t2 * t2;
+            for (int a = 0; a < 3; a++) *at[a] -= p->lambda_self * th2 * th[a];
+        }
+    }
 }
 
 /* Complex mode, scp_sim.c:931-994 (compute_forces_complex). */

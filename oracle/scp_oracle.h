@@ -43,6 +43,8 @@ typedef struct {
     double qdiag_radius, qdiag_probe1, qdiag_probe2;
     int    gradient_margin;   /* bc 1: pinned x-slabs per face */
     const double *pin;        /* bc 1: the pinned state (same layout as the state) */
+    /* real mode: terms of the reference's GPU step only (scp_sim.cu:1144-1248) */
+    double theta_sat, gamma_conv, sigma_grad, chi_chiral, sigma_cubic, sigma_freq, sigma_cross, lambda_self;
 } OrParams;
 
 /* Diagnostics vector: every column of the reference diag.tsv row for one time

@@ -211,10 +211,6 @@ int scp_cfg_unsupported(const scp_config *c, char *err, int n) {
     else if (c->self_tune) bad = "self_tune";
     else if (c->sweep) bad = "sweep";
     else if (c->tune_dt != 0) bad = "tune_dt";
-    else if (c->theta_sat != 0 || c->gamma_conv != 0) bad = "theta_sat/gamma_conv";
-    else if (c->sigma_grad != 0 || c->sigma_cubic != 0 || c->sigma_freq != 0) bad = "sigma_*";
-    else if (c->sigma_cross != 0 || c->lambda_self != 0 || c->theta_vev != 0) bad = "sigma_cross/lambda_self/theta_vev";
-    else if (c->gamma_conf != 0) bad = "gamma_conf";
     else if (c->bc_type < 0 || c->bc_type > 2) bad = "bc_type";
     else if (c->output_split) bad = "output_split";
     if (bad) {

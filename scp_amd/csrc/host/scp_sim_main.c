@@ -223,6 +223,11 @@ int main(int argc, char **argv) {
     p.inv_alpha = c.inv_alpha; p.inv_beta = c.inv_beta; p.kappa_gamma = c.kappa_gamma;
     p.damp_width = c.damp_width; p.damp_rate = c.damp_rate;
     p.qdiag_radius = c.qdiag_radius; p.qdiag_probe1 = c.qdiag_probe1; p.qdiag_probe2 = c.qdiag_probe2;
+    /* real-mode terms of the reference's GPU step (scp_sim.cu:2930-2937); theta_vev and
+       gamma_conf are parsed and never used by the reference, so they change nothing here */
+    p.theta_sat = c.theta_sat; p.gamma_conv = c.gamma_conv; p.sigma_grad = c.sigma_grad;
+    p.chi_chiral = c.chi_chiral; p.sigma_cubic = c.sigma_cubic; p.sigma_freq = c.sigma_freq;
+    p.sigma_cross = c.sigma_cross; p.lambda_self = c.lambda_self;
     int64_t hbm = scp_grid_bytes(&p);
     if (hbm < 0) die_abi("parameters");
     printf("Allocating %.2f GB host + %.2f GB HBM (N=%d, %d fields%s)\n", ncols * n3 * 8.0 / 1e9, hbm / 1e9,

@@ -105,6 +105,10 @@ static int check_params(const scp_params *p) {
         return set_err(-1, "gradient_margin applies to bc_type=1 only (must be 0)");
     }
     if (!(p->dx > 0) || !(p->dt > 0)) return set_err(-1, "dx/dt must be positive");
+    if (p->mode != 0 && (p->theta_sat != 0 || p->gamma_conv != 0 || p->sigma_grad != 0 || p->chi_chiral != 0 ||
+                         p->sigma_cubic != 0 || p->sigma_freq != 0 || p->sigma_cross != 0 || p->lambda_self != 0))
+        return set_err(-1, "theta_sat/gamma_conv/sigma_*/chi_chiral/lambda_self are real-mode terms "
+                           "(scp_config.h:440-463 refuses them with complex_phi)");
     if (p->mode == 2 && p->g < 0) return set_err(-1, "g must be >= 0");
     return 0;
 }
@@ -118,7 +122,7 @@ extern "C" int64_t scp_grid_bytes(const scp_params *p) {
     int64_t b = 2LL * NP * fs * 8;                       // xo, vo
     if (p->precision == SCP_PREC_F64) b += 3LL * NP * fh * 8;           // P[2], W
     else b += 3LL * NP * fh * (4 + (p->lo_words == 0 ? 4 : p->lo_words == 1 ? 2 : 0));
-    if (p->mode == 0 && (p->alpha_cs != 0 || p->beta_h != 0)) b += 6LL * fs * 8;
+    if (p->mode == 0 && (p->alpha_cs != 0 || p->beta_h != 0 || p->sigma_freq > 0)) b += 7LL * fs * 8;
     if (p->bc_type == 1) b += 2LL * NP * fs * 8 + 6LL * NP * p->N * p->N * 8;   // pins, side rows
     b += 1024LL * 32 * 8 + 4096;
     return b;
@@ -147,6 +151,10 @@ static KParams make_kparams(const scp_params &p, int step_mode) {
     k.MU = p.mu; k.KAPPA = p.kappa; k.MASS2 = p.m2; k.MT2 = p.mtheta2; k.ETA = p.eta;
     k.KH = p.kappa_h; k.ACS = p.alpha_cs; k.BH = p.beta_h;
     k.IA = p.inv_alpha; k.IB = p.inv_beta; k.KG = p.kappa_gamma;
+    k.TSAT = p.theta_sat; k.GCONV = p.gamma_conv; k.SGRAD = p.sigma_grad; k.CHI = p.chi_chiral;
+    k.SCUB = p.sigma_cubic; k.SFREQ = p.sigma_freq; k.SCROSS = p.sigma_cross; k.LSELF = p.lambda_self;
+    k.XT = (p.mode == 0 && ((p.theta_sat > 0 && p.gamma_conv > 0) || p.sigma_grad > 0 || p.sigma_cubic > 0 ||
+                            p.sigma_freq > 0 || p.sigma_cross > 0 || p.lambda_self > 0)) ? 1 : 0;
     k.L = p.L; k.dx = dx;
     k.Rd = p.L - p.damp_width;               // scp_sim.c:1444
     k.DW = p.damp_width; k.DR = p.damp_rate;
@@ -215,9 +223,9 @@ extern "C" int scp_grid_create(const scp_params *p, scp_grid **out) {
         e = e ? e : hipMalloc(&g->pinv, grp);
         e = e ? e : hipMalloc(&g->side, (size_t)6 * g->NP * p->N * p->N * sizeof(double));
     }
-    if (p->mode == 0 && (p->alpha_cs != 0 || p->beta_h != 0)) {
-        e = e ? e : hipMalloc(&g->aux, 6 * fsb);
-        e = e ? e : hipMemsetAsync(g->aux, 0, 6 * fsb, g->stream);
+    if (p->mode == 0 && (p->alpha_cs != 0 || p->beta_h != 0 || p->sigma_freq > 0)) {
+        e = e ? e : hipMalloc(&g->aux, 7 * fsb);
+        e = e ? e : hipMemsetAsync(g->aux, 0, 7 * fsb, g->stream);
     }
     g->nblocks_diag = (int)std::min<int64_t>(1024, ((int64_t)g->kp.n3 + DIAG_BLOCK - 1) / DIAG_BLOCK);
     e = e ? e : hipMalloc(&g->partials, (size_t)g->nblocks_diag * 32 * sizeof(double));
@@ -268,9 +276,10 @@ Bufs<double> scp_internal::bufs_for(scp_grid *g, int kind) {
     return b;
 }
 
-static int run_aux(scp_grid *g, const double *src) {
+// real-mode pass 1 from FP64 positions and the velocities the force sees
+static int run_aux(scp_grid *g, const double *src, const double *vel) {
     if (!g->aux) return 0;
-    return kerr(launch_real_aux(g->kp, src, g->aux, g->stream), "real aux");
+    return kerr(launch_real_aux(g->kp, src, vel, g->aux, g->stream), "real aux");
 }
 
 Bufs32 scp_internal::bufs32_for(scp_grid *g, int kind) {
@@ -299,7 +308,7 @@ static int launch_kind(scp_grid *g, int kind, bool emit) {
 
 // initial force + first half kick + first drift, from xo/vo
 int scp_internal::run_init(scp_grid *g) {
-    int rc = run_aux(g, g->xo);
+    int rc = run_aux(g, g->xo, g->vo);
     if (rc) return rc;
     rc = launch_kind(g, K_INIT, false);
     if (rc) return rc;
@@ -368,8 +377,8 @@ static int enqueue_steps(scp_grid *g, int nsteps, const DiagArgs *da = nullptr) 
     for (int s = 0; s < nsteps; s++) {
         int rc = 0;
         if (g->aux) {
-            rc = g->f32 ? kerr(launch_real_aux_f32(g->kp, g->P32[g->cur], g->aux, g->stream), "real aux")
-                        : run_aux(g, g->P[g->cur]);
+            rc = g->f32 ? kerr(launch_real_aux_f32(g->kp, g->P32[g->cur], g->W32, g->aux, g->stream), "real aux")
+                        : run_aux(g, g->P[g->cur], g->W);
             if (rc) return rc;
         }
         const bool emit = (s == nsteps - 1);
@@ -587,7 +596,7 @@ extern "C" int scp_grid_forces(scp_grid *g, double *const *acc, int nacc) {
     if (nacc != want) return set_err(-1, "nacc=%d, mode %d needs %d", nacc, g->mode, want);
     if (!g->acc) HIPCHK(hipMalloc(&g->acc, (size_t)g->NP * g->kp.fs * sizeof(double)));
     HIPCHK(hipMemsetAsync(g->acc, 0, (size_t)g->NP * g->kp.fs * sizeof(double), g->stream));
-    int rc = run_aux(g, g->xo);
+    int rc = run_aux(g, g->xo, g->vo);
     if (rc) return rc;
     rc = launch_kind(g, K_FORCES, false);
     if (rc) return rc;

@@ -24,7 +24,7 @@ struct scp_grid {
     void *Plo[2];     // ... and compensation words
     void *Wlo;
     double *xo, *vo;
-    double *aux;     // real-mode Cosserat/hardening fields (6), when needed
+    double *aux;     // real-mode pass-1 fields (Cosserat, hardening, theta frequency: 7), when needed
     double *acc;     // forces output (lazily allocated)
     double *partials;
     double *dout;

@@ -27,6 +27,9 @@ struct KParams {
     double STP, GG;            // 1/(g dx^3), g
     double MU, KAPPA, MASS2, MT2, ETA;
     double KH, ACS, BH, IA, IB, KG;   // real-mode optional terms
+    // real-mode terms of the reference's GPU step (scp_sim.cu:1144-1248); XT: any of them on
+    double TSAT, GCONV, SGRAD, CHI, SCUB, SFREQ, SCROSS, LSELF;
+    int32_t XT, pad0;
     double L, dx, Rd, DW, DR;  // sponge: R_damp = L - damp_width
     int32_t damp;              // 1 when the sponge is active (bc 0 with width, rate > 0)
     int32_t bc1;               // 1: gradient-pinned boundary (bc 1), x-slabs of gm planes per face
@@ -54,7 +57,7 @@ struct Bufs {
     double *xo;     // emitted positions (FP64), also the INIT source
     double *vo;     // emitted velocities (FP64), also the INIT source
     double *acc;    // forces-only output (FP64)
-    const double *aux;  // real mode: Cosserat mismatch M[3] and hardening Q[3] (6 fields)
+    const double *aux;  // real mode: Cosserat mismatch M[3], hardening Q[3], theta frequency (7 fields)
     // gradient-pinned boundary (bc 1, FP64): pinned positions / velocities
     // (emitted layout) and the side rows the y-edge pass reads, per field f:
     // side[(f*6 + r)*N*N + i*N + k], r = 0/1 acceleration at j = 0 / N-1,
@@ -128,11 +131,12 @@ inline int march_di(int N, int KW) {
     while (di > 16 && (uint64_t)tiles * (uint64_t)((N + di - 1) / di) < 4096u / (uint32_t)KW) di /= 2;
     return di;
 }
-int launch_real_aux(const KParams &kp, const double *pin, double *aux, hipStream_t s);
+// real-mode pass 1 (positions pin, velocities vel the force sees; same layout)
+int launch_real_aux(const KParams &kp, const double *pin, const double *vel, double *aux, hipStream_t s);
 // gradient-pinned boundary: the y-edge rows after a K_STEP launch of the
 // direct FP64 kernel (nf fields; b: that launch's buffers)
 int launch_bc1_edges(bool emit, int nf, const KParams &kp, const Bufs<double> &b, hipStream_t s);
-int launch_real_aux_f32(const KParams &kp, const float *pin, double *aux, hipStream_t s);
+int launch_real_aux_f32(const KParams &kp, const float *pin, const float *vel, double *aux, hipStream_t s);
 
 // Diagnostics: pass 1 writes per-block partials; the finish kernel reduces
 // them in a fixed order into out (deterministic).

@@ -165,7 +165,10 @@ __device__ inline void forces_complex(const KParams &kp, const CV &c, const PV &
 }
 
 // Real-mode accelerations (compute_forces, C:743-922). AUX(f, idx) loads the
-// pass-1 fields M[0..2] (f=0..2) and Q[0..2] (f=3..5) when alpha_cs/beta_h != 0.
+// pass-1 fields M[0..2] (f=0..2) and Q[0..2] (f=3..5) when alpha_cs/beta_h != 0,
+// and the local theta frequency (f=6) when sigma_freq != 0. With kp.XT the
+// terms of the reference's GPU step follow (compute_forces_kernel,
+// scp_sim.cu:1144-1248), added in its order onto the base accelerations.
 template <typename R, class LD, class LA>
 __device__ inline void forces_real(const KParams &kp, const R *c, const R *ip,
                                    const R *im, LD X, LA AUX, uint32_t ci, uint32_t ipx,
@@ -271,6 +274,72 @@ __device__ inline void forces_real(const KParams &kp, const R *c, const R *ip,
         if (ACS != 0) cos_th = R(2.0) * ACS * AUX(a, ci);
         R hard_th = -BH * csq * c[TU + a];
         acc[TU + a] = lt - R(kp.MT2) * c[TU + a] + ETA * cp + cos_th + hard_th;
+    }
+    if (kp.XT) {
+        const R th[3] = {c[TU], c[TU + 1], c[TU + 2]};
+        const R ct[3] = {curlF(TU, 0), curlF(TU, 1), curlF(TU, 2)};
+        const R th2 = th[0] * th[0] + th[1] * th[1] + th[2] * th[2];
+        if (kp.TSAT > 0 && kp.GCONV > 0) {          // saturation -> conversion (CU:1144-1161)
+            const R excess = th2 - R(kp.TSAT) * R(kp.TSAT);
+            if (excess > 0) {
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    acc[TU + a] -= R(kp.GCONV) * excess * th[a];
+                    acc[a] += R(kp.GCONV) * excess * ct[a];
+                }
+            }
+        }
+        if (kp.SGRAD > 0) {                         // gradient self-interaction (CU:1163-1185)
+            R g2 = 0;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const R dx_ = (nbv(TU + a, 0, 1) - nbv(TU + a, 0, -1)) * h1;
+                const R dy_ = (nbv(TU + a, 1, 1) - nbv(TU + a, 1, -1)) * h1;
+                const R dz_ = (nbv(TU + a, 2, 1) - nbv(TU + a, 2, -1)) * h1;
+                g2 += dx_ * dx_ + dy_ * dy_ + dz_ * dz_;
+            }
+            const R hh = th[0] * ct[0] + th[1] * ct[1] + th[2] * ct[2];
+            R se = R(kp.SGRAD) * (R(1.0) + R(kp.CHI) * hh / (th2 + R(0.01)));
+            if (se < 0) se = 0;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                acc[TU + a] -= se * g2 * th[a];
+                acc[a] += se * g2 * ct[a];
+            }
+        }
+        if (kp.SCUB > 0) {                          // cubic self-interaction (CU:1188-1199)
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                acc[TU + a] -= R(kp.SCUB) * th2 * th[a];
+                acc[a] += R(kp.SCUB) * th2 * ct[a];
+            }
+        }
+        if (kp.SFREQ > 0) {                         // frequency mismatch (CU:1201-1229)
+            const R oc = AUX(6, ci);
+            R dw = 0;
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+#pragma unroll
+                for (int sg = 1; sg >= -1; sg -= 2) dw += fabs(oc - AUX(6, nbi(d, sg)));
+            const R dwa = dw / R(6);
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                acc[TU + a] -= R(kp.SFREQ) * dwa * th[a];
+                acc[a] += R(kp.SFREQ) * dwa * ct[a];
+            }
+        }
+        if (kp.SCROSS > 0) {                        // cross potential (CU:1232-1240)
+            const R ph2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                acc[a] -= R(kp.SCROSS) * th2 * c[a];
+                acc[TU + a] -= R(kp.SCROSS) * ph2 * th[a];
+            }
+        }
+        if (kp.LSELF > 0) {                         // theta self potential (CU:1243-1248)
+#pragma unroll
+            for (int a = 0; a < 3; a++) acc[TU + a] -= R(kp.LSELF) * th2 * th[a];
+        }
     }
 }
 

@@ -149,9 +149,13 @@ k_step64(KParams kp, Bufs<double> b, int DI) {
     }
 }
 
-// Real-mode pass 1 (C:755-786): Cosserat mismatch and hardening vector.
+// Real-mode pass 1 (C:755-786): Cosserat mismatch and hardening vector; and
+// the local theta frequency |theta_vel| / (|theta| + 0.01) of the
+// frequency-mismatch term (scp_sim.cu:1204-1217), from the velocities the
+// force sees (the half-kicked ones in a step). A pass of its own because a
+// fused step updates the velocities in place while neighbours read them.
 template <typename T>
-__global__ void k_real_aux(KParams kp, const T *pin, double *aux) {
+__global__ void k_real_aux(KParams kp, const T *pin, const T *vel, double *aux) {
     const int N = kp.N;
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= kp.n3) return;
@@ -172,6 +176,12 @@ __global__ void k_real_aux(KParams kp, const T *pin, double *aux) {
         double T2 = t0 * t0 + t1 * t1 + t2 * t2;
         double coeff = 0.5 * kp.BH * T2;
         for (int a = 0; a < 3; a++) aux[(size_t)(3 + a) * fs + c] = coeff * cp[a];
+    }
+    if (kp.SFREQ > 0) {
+        auto V = [&](int f) -> double { return (double)vel[(size_t)f * fs + c]; };
+        const double tv = V(3) * V(3) + V(4) * V(4) + V(5) * V(5);
+        const double th = F(3, c) * F(3, c) + F(4, c) * F(4, c) + F(5, c) * F(5, c);
+        aux[(size_t)6 * fs + c] = sqrt(tv) / (sqrt(th) + 0.01);
     }
 }
 
@@ -254,8 +264,8 @@ int launch_bc1_edges(bool emit, int nf, const KParams &kp, const Bufs<double> &b
     return (int)hipGetLastError();
 }
 
-int launch_real_aux(const KParams &kp, const double *pin, double *aux, hipStream_t s) {
-    hipLaunchKernelGGL((k_real_aux<double>), dim3((kp.n3 + 255) / 256), dim3(256), 0, s, kp, pin, aux);
+int launch_real_aux(const KParams &kp, const double *pin, const double *vel, double *aux, hipStream_t s) {
+    hipLaunchKernelGGL((k_real_aux<double>), dim3((kp.n3 + 255) / 256), dim3(256), 0, s, kp, pin, vel, aux);
     return (int)hipGetLastError();
 }
 

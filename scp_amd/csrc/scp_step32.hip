@@ -207,7 +207,8 @@ k_step32(KParams kp, Bufs32 b, int DI) {
     }
 }
 
-__global__ void k_real_aux32(KParams kp, const float *pin, double *aux) {
+// FP32C form of k_real_aux (scp_step.hip)
+__global__ void k_real_aux32(KParams kp, const float *pin, const float *vel, double *aux) {
     const int N = kp.N;
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= kp.n3) return;
@@ -227,6 +228,12 @@ __global__ void k_real_aux32(KParams kp, const float *pin, double *aux) {
         float t0 = F(3, c), t1 = F(4, c), t2 = F(5, c);
         float coeff = 0.5f * (float)kp.BH * (t0 * t0 + t1 * t1 + t2 * t2);
         for (int a = 0; a < 3; a++) aux[(size_t)(3 + a) * fs + c] = coeff * cp[a];
+    }
+    if (kp.SFREQ > 0) {
+        auto V = [&](int f) -> float { return vel[(size_t)f * fs + c]; };
+        const float tv = V(3) * V(3) + V(4) * V(4) + V(5) * V(5);
+        const float th = F(3, c) * F(3, c) + F(4, c) * F(4, c) + F(5, c) * F(5, c);
+        aux[(size_t)6 * fs + c] = (double)(sqrtf(tv) / (sqrtf(th) + 0.01f));
     }
 }
 
@@ -263,8 +270,8 @@ int launch_step_f32c(int mode, int kind, bool emit, int lo_kind, const KParams &
     return launch32m<0>(kind, emit, lo_kind, kp, b, s);
 }
 
-int launch_real_aux_f32(const KParams &kp, const float *pin, double *aux, hipStream_t s) {
-    hipLaunchKernelGGL(k_real_aux32, dim3((kp.n3 + 255) / 256), dim3(256), 0, s, kp, pin, aux);
+int launch_real_aux_f32(const KParams &kp, const float *pin, const float *vel, double *aux, hipStream_t s) {
+    hipLaunchKernelGGL(k_real_aux32, dim3((kp.n3 + 255) / 256), dim3(256), 0, s, kp, pin, vel, aux);
     return (int)hipGetLastError();
 }
 

@@ -59,7 +59,10 @@ class Params(C.Structure):
                 ("inv_alpha", C.c_double), ("inv_beta", C.c_double), ("kappa_gamma", C.c_double),
                 ("damp_width", C.c_double), ("damp_rate", C.c_double), ("G_offset", C.c_double),
                 ("qdiag_radius", C.c_double), ("qdiag_probe1", C.c_double),
-                ("qdiag_probe2", C.c_double)]
+                ("qdiag_probe2", C.c_double),
+                ("theta_sat", C.c_double), ("gamma_conv", C.c_double), ("sigma_grad", C.c_double),
+                ("chi_chiral", C.c_double), ("sigma_cubic", C.c_double), ("sigma_freq", C.c_double),
+                ("sigma_cross", C.c_double), ("lambda_self", C.c_double)]
 
 
 class QBall(C.Structure):

@@ -250,3 +250,62 @@ def test_march_kernel_equals_direct_kernel(mode, N):
             g.step(11)
             outs.append(g.download())
     assert np.array_equal(outs[0], outs[1])
+
+
+GPU_TERMS = {
+    "theta_sat": dict(theta_sat=0.05, gamma_conv=0.8),
+    "sigma_grad": dict(sigma_grad=0.4, chi_chiral=1.5),
+    "sigma_cubic": dict(sigma_cubic=0.7),
+    "sigma_freq": dict(sigma_freq=0.3),
+    "sigma_cross": dict(sigma_cross=0.5),
+    "lambda_self": dict(lambda_self=0.9),
+    "all": dict(theta_sat=0.05, gamma_conv=0.8, sigma_grad=0.4, chi_chiral=1.5, sigma_cubic=0.7,
+                sigma_freq=0.3, sigma_cross=0.5, lambda_self=0.9),
+}
+
+
+@pytest.mark.parametrize("terms", list(GPU_TERMS))
+@pytest.mark.parametrize("N", [12, 13])
+def test_real_gpu_terms_match_oracle(terms, N):
+    """Real mode with the terms of the reference's GPU step (scp_sim.cu:1144-1248)
+    against the oracle's restatement, bitwise in FP64 (theta noise large enough to
+    cross the saturation threshold; the frequency term reads the half-kicked
+    theta velocities through its own pass)."""
+    L = 0.25 * (N - 1)
+    st = seeded(N, 1, L)[:12].copy()
+    rng = np.random.default_rng(N)
+    st[3:6] = 0.2 * rng.standard_normal((3, N ** 3))
+    st[9:12] = 0.1 * rng.standard_normal((3, N ** 3))
+    kw = GPU_TERMS[terms]
+    print(f"case real terms={terms} N={N}", flush=True)
+    p_o = oracle.make_params(N, 0, L, 0.1, **kw)
+    ref, _ = oracle.step(p_o, st, 12)
+    with G.Grid(G.make_params(N, 0, L, 0.1, **kw)) as g:
+        g.upload(st)
+        g.step(5)
+        g.step(7)
+        out = g.download()
+    assert np.array_equal(out, ref.reshape(out.shape)), f"max diff {np.abs(out - ref.reshape(out.shape)).max():.3e}"
+    base, _ = oracle.step(oracle.make_params(N, 0, L, 0.1), st, 12)
+    assert np.abs(out - base.reshape(out.shape)).max() > 1e-6      # the terms act
+    with G.Grid(G.make_params(N, 0, L, 0.1, precision=G.PREC_F32C, lo_words=0, **kw)) as g:
+        g.upload(st)
+        g.step(12)
+        out32 = g.download()
+    assert rel_l2(out32, ref.reshape(out.shape)) <= 1e-5
+
+
+def test_real_gpu_terms_forces_match_oracle():
+    """scp_grid_forces with every term on: the initial velocities feed the frequency term."""
+    N = 10
+    L = 0.25 * (N - 1)
+    st = seeded(N, 1, L)[:12].copy()
+    rng = np.random.default_rng(3)
+    st[3:6] = 0.2 * rng.standard_normal((3, N ** 3))
+    st[9:12] = 0.1 * rng.standard_normal((3, N ** 3))
+    kw = GPU_TERMS["all"]
+    ref = oracle.forces(oracle.make_params(N, 0, L, 0.1, **kw), st)
+    with G.Grid(G.make_params(N, 0, L, 0.1, **kw)) as g:
+        g.upload(st)
+        acc = g.forces()
+    assert np.array_equal(acc, ref)

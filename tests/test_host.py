@@ -219,8 +219,26 @@ def test_abi_library_loads_and_refuses_bad_params():
     assert G.grid_bytes(q) == -1 and "FP64" in G.last_error()
     q = G.make_params(8, 1, 3.5, 0.1, gradient_margin=2)
     assert G.grid_bytes(q) == -1 and "bc_type=1 only" in G.last_error()
+    q = G.make_params(8, 1, 3.5, 0.1, sigma_cubic=0.5)          # real-mode term on a complex grid
+    assert G.grid_bytes(q) == -1 and "real-mode terms" in G.last_error()
+    q = G.make_params(8, 0, 3.5, 0.1, sigma_freq=0.5)           # frequency term: a 7-field pass-1 array
+    assert G.grid_bytes(q) == G.grid_bytes(G.make_params(8, 0, 3.5, 0.1)) + 7 * 512 * 8
     p.bc_type = 2
     assert G.grid_bytes(p) > 0
     p.N = 3
     assert G.grid_bytes(p) == -1
     assert "out of range" in G.last_error()
+
+
+def test_config_runs_the_gpu_step_real_terms():
+    """The real-mode keys of the reference's GPU step are run, not refused; theta_vev
+    and gamma_conf are parsed and never used by the reference, so they are accepted
+    and change nothing; the terms stay refused with complex_phi, as there."""
+    c = config.Config()
+    for k, v in (("theta_sat", "0.1"), ("gamma_conv", "0.5"), ("sigma_grad", "0.2"), ("chi_chiral", "1"),
+                 ("sigma_cubic", "0.3"), ("sigma_freq", "0.1"), ("sigma_cross", "0.2"), ("lambda_self", "0.4"),
+                 ("theta_vev", "0.5"), ("gamma_conf", "0.2")):
+        c.set(k, v)
+    assert c.validate() is None and c.unsupported() is None
+    c.set("complex_phi", "1")
+    assert "incompatible with sigma_grad" in c.validate()

@@ -290,3 +290,108 @@ def test_g_zero_dispatches_to_complex():
     a, _ = oracle.step(pg, st, 10)
     b, _ = oracle.step(pc, st[:24], 10)
     assert np.array_equal(a[:24], b) and np.array_equal(a[24:], st[24:])
+
+
+# ---------------------------------------------------------------------------
+# Real-mode terms of the reference's GPU step only (scp_sim.cu:1144-1248): no
+# executable reference here (the CUDA path), so the restatement is pinned by
+# closed forms on fields whose lattice derivatives are known exactly.
+# ---------------------------------------------------------------------------
+
+def _real_state(N, th_fn, phi=(0.3, -0.2, 0.5), thv_fn=None):
+    i, j, k = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    st = np.zeros((12, N ** 3))
+    for a in range(3):
+        st[a] = phi[a]
+    th = th_fn(i, j, k)
+    for a in range(3):
+        st[3 + a] = np.broadcast_to(th[a], i.shape).ravel()
+    if thv_fn is not None:
+        thv = thv_fn(i, j, k)
+        for a in range(3):
+            st[9 + a] = np.broadcast_to(thv[a], i.shape).ravel()
+    return st
+
+
+def _extra(N, st, **kw):
+    """acceleration change the GPU-step terms add, from the oracle"""
+    L = 2.5
+    base = oracle.forces(oracle.make_params(N, 0, L, 0.1), st)
+    return oracle.forces(oracle.make_params(N, 0, L, 0.1, **kw), st) - base
+
+
+def test_gpu_terms_uniform_fields_closed_form():
+    """Uniform fields: no gradients or curls, so each term reduces to its local
+    form: lambda |th|^2 th, sigma_x |th|^2 phi and |phi|^2 th, sigma_c |th|^2 th,
+    gamma (|th|^2 - th_sat^2) th above the threshold and nothing below."""
+    N = 6
+    th0 = np.array([0.4, 0.1, -0.3])
+    phi = np.array([0.3, -0.2, 0.5])
+    st = _real_state(N, lambda i, j, k: th0, phi=phi)
+    t2, p2 = float(th0 @ th0), float(phi @ phi)
+    cases = [
+        (dict(lambda_self=0.7), np.zeros(3), -0.7 * t2 * th0),
+        (dict(sigma_cross=0.3), -0.3 * t2 * phi, -0.3 * p2 * th0),
+        (dict(sigma_cubic=0.9), np.zeros(3), -0.9 * t2 * th0),
+        (dict(theta_sat=0.2, gamma_conv=1.3), np.zeros(3), -1.3 * (t2 - 0.04) * th0),
+        (dict(theta_sat=0.9, gamma_conv=1.3), np.zeros(3), np.zeros(3)),      # below saturation
+        (dict(theta_sat=0.2, gamma_conv=0.0), np.zeros(3), np.zeros(3)),      # both must be > 0
+        (dict(sigma_grad=0.5, chi_chiral=2.0), np.zeros(3), np.zeros(3)),     # no gradient
+        (dict(sigma_freq=0.5), np.zeros(3), np.zeros(3)),                     # no frequency mismatch
+        (dict(lambda_self=-0.7, sigma_cubic=-1.0), np.zeros(3), np.zeros(3)), # negative = off, as there
+    ]
+    for kw, dphi, dth in cases:
+        d = _extra(N, st, **kw)
+        for a in range(3):
+            assert np.abs(d[a] - dphi[a]).max() <= 1e-15, (kw, a)
+            assert np.abs(d[3 + a] - dth[a]).max() <= 1e-15, (kw, a)
+
+
+def test_gpu_terms_gradient_and_chirality_closed_form():
+    """th = (0, A sin(q i), B): the central difference of th_1 along i is
+    A sin(q) cos(q i) / dx exactly, so |grad th|^2 = g^2 with g that value, curl
+    th = (0, 0, g), helicity h = B g, sigma_eff = sigma (1 + chi h / (|th|^2 +
+    0.01)) clamped at 0; th_a gets -sigma_eff g^2 th_a and phi_2 +sigma_eff g^3."""
+    N, A, B, sg, chi = 8, 0.35, 0.25, 0.6, 1.7
+    q = 2 * np.pi / N
+    st = _real_state(N, lambda i, j, k: (0.0 * i, A * np.sin(q * i), B + 0.0 * i))
+    dx = 2 * 2.5 / (N - 1)
+    i = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")[0].ravel()
+    g = A * np.sin(q) * np.cos(q * i) / dx
+    th = np.stack([0.0 * i, A * np.sin(q * i), B + 0.0 * i])
+    t2 = (th ** 2).sum(axis=0)
+    se = np.maximum(sg * (1 + chi * B * g / (t2 + 0.01)), 0.0)
+    d = _extra(N, st, sigma_grad=sg, chi_chiral=chi)
+    for a in range(3):
+        assert np.abs(d[3 + a] + se * g * g * th[a]).max() <= 1e-13, a
+    assert np.abs(d[0]).max() == 0 and np.abs(d[1]).max() == 0
+    assert np.abs(d[2] - se * g * g * g).max() <= 1e-13
+    # the curl couples: a cubic term on the same field sources phi_2 with sigma_c |th|^2 g
+    d = _extra(N, st, sigma_cubic=0.8)
+    assert np.abs(d[2] - 0.8 * t2 * g).max() <= 1e-13
+
+
+def test_gpu_terms_frequency_mismatch_closed_form():
+    """Uniform th of modulus 0.3, theta velocity 0.5 along th at one site only:
+    omega there is 0.5 / 0.31 and 0 elsewhere, so that site sees the mean
+    mismatch omega0 and each of its six neighbours omega0 / 6; no curl, so phi
+    is untouched."""
+    N = 6
+    th0 = np.array([0.3, 0.0, 0.0])
+    s0 = (2, 3, 1)
+
+    def thv(i, j, k):
+        m = ((i == s0[0]) & (j == s0[1]) & (k == s0[2])).astype(float)
+        return (0.5 * m, 0.0 * m, 0.0 * m)
+    st = _real_state(N, lambda i, j, k: th0, thv_fn=thv)
+    d = _extra(N, st, sigma_freq=0.4)
+    om = 0.5 / (0.3 + 0.01)
+    expect = np.zeros((N, N, N))
+    expect[s0] = om
+    for dd in range(3):
+        for sgn in (1, -1):
+            n = list(s0)
+            n[dd] = (n[dd] + sgn) % N
+            expect[tuple(n)] = om / 6
+    assert np.abs(d[3] + 0.4 * expect.ravel() * th0[0]).max() <= 1e-15
+    assert np.abs(d[:3]).max() == 0 and np.abs(d[4:6]).max() == 0
