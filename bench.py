@@ -8,20 +8,23 @@ host from a fixed seed and resident in HBM before the timed region.
 
 A "step" is one full leapfrog step of the whole grid (one fused launch).
 value = site-updates/s over K timed steps (HIP events on the grid's stream)
-of the headline precision: FP32C with bfloat16 compensation words by default;
+of the headline precision: FP32C with int8 compensation words on the
+positions by default (lo_words 4);
 the FP64 path is timed on the same state after it ("fp64", "speedup_vs_fp64").
 roofline: algorithmic HBM bytes per site-update = every persistent state
-scalar read and written once at its storage width (gauged: 30 scalars; FP64
-8 B -> 480 B; FP32C 4 B hi + 4/2/0 B compensation word for lo_words 0/1/2 ->
-480/360/240 B), times N^3 sites per launch, over the average launch time;
-peak = 8 TB/s. traffic: measured HBM bytes per launch (rocprofv3 FETCH_SIZE
-and WRITE_SIZE scaled by a plain copy's counters), read from the counter
-summary profiles/hbm_traffic.txt when it holds a line for this kernel and
-size. cpu_baseline: the reference CPU binary built from source
+scalar read and written once at its storage width, compensation words not
+counted (gauged: 30 scalars; FP64 8 B -> 480 B; FP32C 4 B -> 240 B), times N^3
+sites per launch, over the average launch time; peak = 8 TB/s. The bytes the
+step actually streams per site (FP32C: 4 B hi words plus the compensation
+words, for lo_words 0/1/2/3/4 -> 480/360/240/304/272 B) are reported beside
+it as stored_bytes_per_site and stored_frac. traffic: measured HBM bytes per launch
+(rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a plain copy's counters), read
+from the counter summary profiles/hbm_traffic.txt when it holds a line for
+this kernel and size. cpu_baseline: the reference CPU binary built from source
 (oracle/_ref/scp_sim) on a bounded 64^3 gauged sample on this host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--prec f64|f32c]
-                       [--lo-words 0|1|2] [--no-compare] [--no-cpu-baseline]
+                       [--lo-words 0|1|2|3|4] [--no-compare] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -39,10 +42,22 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-def bytes_per_site(prec, lo_words):
-    """30 gauged state scalars, each read once and written once per step."""
-    width = 8 if prec == "f64" else 4 + {0: 4, 1: 2, 2: 0}[lo_words]
-    return 2 * 30 * width
+LO_NAMES = {0: "f32", 1: "bf16", 2: "no", 3: "int8", 4: "int8 position"}
+
+
+def bytes_per_site(prec):
+    """Algorithmic: 30 gauged state scalars, each read once and written once per
+    step at the state's width (compensation words are not state)."""
+    return 2 * 30 * (8 if prec == "f64" else 4)
+
+
+def stored_bytes_per_site(prec, lo_words):
+    """What the step streams: the state words plus the compensation words."""
+    if prec == "f64":
+        return 2 * 30 * 8
+    # 15 position and 15 velocity scalars; int8 words packed four fields to a word
+    lo_p, lo_w = {0: (4, 4), 1: (2, 2), 2: (0, 0), 3: (16 / 15, 16 / 15), 4: (16 / 15, 0)}[lo_words]
+    return 2 * 15 * (8 + lo_p + lo_w)
 
 
 def measured_traffic(kernel, N):
@@ -137,14 +152,16 @@ def run_one(G, N, L, st, prec, lo_words, steps, warmup):
         diag_ms = diag_step_timing(g)
     ms_step = ms / steps
     sites = float(N) ** 3
-    bps = bytes_per_site(prec, lo_words)
+    bps = bytes_per_site(prec)
     achieved = bps * sites / (ms_step * 1e-3) / 1e9
+    stored = stored_bytes_per_site(prec, lo_words) * sites / (ms_step * 1e-3) / 1e9
     log(f"{prec}: {steps} steps {ms:.2f} ms (wall {wall * 1e3:.1f} ms) -> {ms_step:.3f} ms/step; "
         f"E_total={d['E_total']:.9e} Q_total={d['Q_total']:.6e}")
     log(f"{prec}: diag step (wall, ms): step alone {diag_ms['step']:.2f}, step + stand-alone diagnostics "
         f"{diag_ms['separate']:.2f}, diagnostics fused into the step {diag_ms['fused']:.2f}")
     return {"ms_per_step": ms_step, "value": sites / (ms_step * 1e-3) / 1e9, "kernel": kernel,
             "bytes_per_site": bps, "achieved": achieved, "traffic": measured_traffic(kernel, N),
+            "stored_bytes_per_site": stored_bytes_per_site(prec, lo_words), "stored_GBps": stored,
             "diag_step_ms": diag_ms}
 
 
@@ -213,15 +230,17 @@ def sweep_bench(a):
     out = {"metric": "aggregate Gsite-updates/s, batched sweep of independent gauged grids",
            "value": sites / (ms_step * 1e-3) / 1e9, "unit": "Gsite-updates/s", "n_gpus": 1,
            "steps": a.steps, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": f"f32c ({['f32', 'bf16', 'no'][a.lo_words]} compensation words)",
+           "vs_baseline": None, "dtype": f"f32c ({LO_NAMES[a.lo_words]} compensation words)",
            "data": "synthetic (seeded single Q-ball per grid)",
            "config": {"workload": f"{len(grids)} gauged grids, edges " +
                                   ("from {32,48,64,80,96}" if a.sizes == "sweep" else "log-uniform in [48,256]") +
                                   ", one Q-ball each",
                       "grids": len(grids), "sites": sites, "seed": a.seed, "tuned": cfg},
-           "roofline": {"bound": "hbm", "achieved": bytes_per_site("f32c", a.lo_words) * sites / (ms_step * 1e-3) / 1e9,
+           "roofline": {"bound": "hbm", "achieved": bytes_per_site("f32c") * sites / (ms_step * 1e-3) / 1e9,
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": bytes_per_site("f32c", a.lo_words) * sites / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                        "frac": bytes_per_site("f32c") * sites / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "stored_frac": stored_bytes_per_site("f32c", a.lo_words) * sites / (ms_step * 1e-3) / 1e9
+                                       / HBM_PEAK_GBS},
            "unbatched": {"value": sites / (ms_alone_step * 1e-3) / 1e9, "ms_per_step": ms_alone_step},
            "untuned_ms_per_step": ms_default,
            "batch_speedup": ms_alone_step / ms_step}
@@ -298,7 +317,7 @@ def serve_bench(a):
     out = {"metric": "serving: p50/p99 job completion latency and aggregate Gsite-updates/s",
            "value": st["site_updates"] / elapsed / 1e9, "unit": "Gsite-updates/s", "n_gpus": 1,
            "higher_is_better": True, "vs_baseline": None,
-           "dtype": f"f32c ({['f32', 'bf16', 'no'][a.lo_words]} compensation words)",
+           "dtype": f"f32c ({LO_NAMES[a.lo_words]} compensation words)",
            "data": "synthetic (Poisson arrivals, seeded Q-ball jobs)",
            "config": {"workload": "gauged jobs, edge log-uniform [48,256], 1-4 Q-balls", "rate_per_s": a.rate,
                       "duration_s": a.duration, "horizon_s": a.horizon, "steps": [a.steps_lo, a.steps_hi],
@@ -320,8 +339,9 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--prec", default="f32c", choices=["f64", "f32c"],
                     help="precision of the headline value (default: FP32C, the production form)")
-    ap.add_argument("--lo-words", type=int, default=1,
-                    help="FP32C compensation words: 0 FP32, 1 bfloat16 (default), 2 none (plain FP32)")
+    ap.add_argument("--lo-words", type=int, default=4,
+                    help="FP32C compensation words: 0 FP32, 1 bfloat16, 2 none (plain FP32), "
+                         "3 signed byte in units of ulp/256, 4 such bytes on the positions only (default)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="step", choices=["step", "sweep", "serve"],
@@ -370,7 +390,7 @@ def main():
         cmp64 = run_one(G, N, L, st, "f64", 0, a.steps, a.warmup)
     del st
     cpu = None if a.no_cpu_baseline else cpu_baseline()
-    dtype = "f64" if a.prec == "f64" else f"f32c ({['f32', 'bf16', 'no'][a.lo_words]} compensation words)"
+    dtype = "f64" if a.prec == "f64" else f"f32c ({LO_NAMES[a.lo_words]} compensation words)"
     out = {
         "metric": "Gsite-updates/s & %HBM peak, 512^3 gauged, FP32c vs FP64; p50/p99 job latency",
         "value": head["value"], "unit": "Gsite-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -380,7 +400,9 @@ def main():
                    "N": N, "mode": "gauged", "precision": a.prec, "lo_words": a.lo_words, "g": 0.05},
         "roofline": {"bound": "hbm", "achieved": head["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": head["achieved"] / HBM_PEAK_GBS, "traffic": head["traffic"],
-                     "kernel": head["kernel"], "bytes_per_site": head["bytes_per_site"]},
+                     "kernel": head["kernel"], "bytes_per_site": head["bytes_per_site"],
+                     "stored_bytes_per_site": head["stored_bytes_per_site"],
+                     "stored_frac": head["stored_GBps"] / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
         "diag_step_ms": head["diag_step_ms"],
     }

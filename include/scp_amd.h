@@ -53,7 +53,9 @@ typedef struct {
                              FP64), 2 periodic */
     int32_t rmode;        /* real-mode mass coupling: 0 const, 1 inverse, 3 density-kappa */
     int32_t lo_words;     /* SCP_PREC_F32C compensation-word storage: 0 FP32 (8 B/scalar),
-                             1 bfloat16 (6 B/scalar), 2 none (plain FP32, 4 B/scalar) */
+                             1 bfloat16 (6 B/scalar), 2 none (plain FP32, 4 B/scalar),
+                             3 signed byte in units of ulp(hi)/256 (5 B/scalar),
+                             4 such bytes on the positions only (4.5 B/scalar) */
     int32_t step_kernel;  /* 0: LDS-staged march (default; complex and gauged steps),
                              1: direct-neighbour march (every mode; the real mode always uses it) */
     int32_t gradient_margin;  /* bc_type 1: x-slabs pinned on each face (gradient_margin,

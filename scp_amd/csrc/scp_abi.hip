@@ -92,7 +92,7 @@ static int check_params(const scp_params *p) {
     if (p->mode < 0 || p->mode > 2) return set_err(-1, "mode=%d invalid", p->mode);
     if (p->precision != SCP_PREC_F64 && p->precision != SCP_PREC_F32C)
         return set_err(-1, "precision=%d invalid", p->precision);
-    if (p->precision == SCP_PREC_F32C && (p->lo_words < 0 || p->lo_words > 2))
+    if (p->precision == SCP_PREC_F32C && (p->lo_words < 0 || p->lo_words > 4))
         return set_err(-1, "lo_words=%d invalid", p->lo_words);
     if (p->step_kernel < 0 || p->step_kernel > 1) return set_err(-1, "step_kernel=%d invalid", p->step_kernel);
     if (p->bc_type < 0 || p->bc_type > 2) return set_err(-1, "bc_type=%d invalid", p->bc_type);
@@ -121,7 +121,7 @@ extern "C" int64_t scp_grid_bytes(const scp_params *p) {
     const int64_t fh = step_field_stride(p->N, p->mode);
     int64_t b = 2LL * NP * fs * 8;                       // xo, vo
     if (p->precision == SCP_PREC_F64) b += 3LL * NP * fh * 8;           // P[2], W
-    else b += 3LL * NP * fh * (4 + (p->lo_words == 0 ? 4 : p->lo_words == 1 ? 2 : 0));
+    else b += fh * (12LL * NP + lo_site_bytes(lo_kind_w(p->lo_words), NP) + 2LL * lo_site_bytes(lo_kind_p(p->lo_words), NP));
     if (p->mode == 0 && (p->alpha_cs != 0 || p->beta_h != 0 || p->sigma_freq > 0)) b += 7LL * fs * 8;
     if (p->bc_type == 1) b += 2LL * NP * fs * 8 + 6LL * NP * p->N * p->N * 8;   // pins, side rows
     b += 1024LL * 32 * 8 + 4096;
@@ -206,15 +206,16 @@ extern "C" int scp_grid_create(const scp_params *p, scp_grid **out) {
         e = e ? e : hipMalloc(&g->W, grph);
     } else {
         const size_t g32 = (size_t)g->kp.fsh * 4 * g->NP;
-        const size_t glo = (size_t)g->kp.fsh * g->NP * (g->lo_kind == LO_F32 ? 4 : 2);
+        const size_t glw = (size_t)g->kp.fsh * lo_site_bytes(lo_kind_w(g->lo_kind), g->NP);
+        const size_t glp = (size_t)g->kp.fsh * lo_site_bytes(lo_kind_p(g->lo_kind), g->NP);
         e = e ? e : hipMalloc(&g->P32[0], g32);
         e = e ? e : hipMalloc(&g->P32[1], g32);
         e = e ? e : hipMalloc(&g->W32, g32);
-        if (g->lo_kind != LO_NONE) {
-            e = e ? e : hipMalloc(&g->Plo[0], glo);
-            e = e ? e : hipMalloc(&g->Plo[1], glo);
-            e = e ? e : hipMalloc(&g->Wlo, glo);
+        if (glp) {
+            e = e ? e : hipMalloc(&g->Plo[0], glp);
+            e = e ? e : hipMalloc(&g->Plo[1], glp);
         }
+        if (glw) e = e ? e : hipMalloc(&g->Wlo, glw);
     }
     e = e ? e : hipMalloc(&g->xo, grp);
     e = e ? e : hipMalloc(&g->vo, grp);

@@ -70,7 +70,68 @@ enum StepKind { K_INIT = 0, K_STEP = 1, K_FORCES = 2 };
 
 // FP32 compensated state: hi words (FP32) plus a compensation ("lo") word per
 // scalar whose storage is selected by LoKind.
-enum LoKind { LO_F32 = 0, LO_BF16 = 1, LO_NONE = 2 };
+enum LoKind { LO_F32 = 0, LO_BF16 = 1, LO_NONE = 2, LO_Q8 = 3, LO_Q8P = 4 };
+// LO_Q8P: LO_Q8 words on the positions P only, velocities W without
+constexpr int lo_kind_w(int k) { return k == LO_Q8P ? LO_NONE : k; }   // storage of W's words
+constexpr int lo_kind_p(int k) { return k == LO_Q8P ? LO_Q8 : k; }     // storage of P's words
+
+// Storage type and codec of a compensation word. The lo word of a pair holds
+// at most ulp(hi)/2 in magnitude (the exact error of the TwoSum that formed hi).
+//   LO_F32  the FP32 value;
+//   LO_BF16 the FP32 value rounded to bfloat16 (nearest even);
+//   LO_Q8   a signed byte q in units of ulp(hi)/256: the byte keeps what a
+//           bfloat16 keeps of the largest lo words in half the storage;
+//   LO_NONE nothing (plain FP32).
+template <int LOK> struct LoWord;
+template <> struct LoWord<LO_F32> {
+    using W = float;
+    static constexpr bool STORED = true;
+    __device__ static float dec(W w, float) { return w; }
+    __device__ static W enc(float lo, float) { return lo; }
+};
+template <> struct LoWord<LO_BF16> {
+    using W = uint16_t;
+    static constexpr bool STORED = true;
+    __device__ static float dec(W w, float) { return __uint_as_float((uint32_t)w << 16); }
+    __device__ static W enc(float lo, float) {     // round to nearest even
+        uint32_t u = __float_as_uint(lo);
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        return (W)(u >> 16);
+    }
+};
+template <> struct LoWord<LO_Q8> {
+    using W = int8_t;
+    static constexpr bool STORED = true;
+    // lo = q * 2^(E - 32) with hi = m * 2^E, 1/2 <= |m| < 1 (the exponent frexp
+    // gives): 256 steps per ulp(hi) = 2^(E - 24), exact scaling by v_ldexp
+    __device__ static float dec(W w, float hi) {
+        return __builtin_amdgcn_ldexpf((float)w, __builtin_amdgcn_frexp_expf(hi) - 32);
+    }
+    __device__ static W enc(float lo, float hi) {   // round to nearest, |q| <= 127
+        const int q = (int)rintf(__builtin_amdgcn_ldexpf(lo, 32 - __builtin_amdgcn_frexp_expf(hi)));
+        return (W)min(max(q, -127), 127);
+    }
+};
+template <> struct LoWord<LO_NONE> {
+    using W = uint8_t;
+    static constexpr bool STORED = false;
+    __device__ static float dec(W, float) { return 0.0f; }
+    __device__ static W enc(float, float) { return 0; }
+};
+// LO_Q8 words of one site are stored four to a 32-bit word, field f in byte
+// f % 4 of word f / 4 (a word array per four fields, laid out like a field):
+// a wave then stores whole 256-byte runs, never a part of a cache line that a
+// block on another XCD completes.
+struct LoQuads {
+    uint32_t q[4];
+    __device__ int8_t get(int f) const { return (int8_t)(q[f >> 2] >> (8 * (f & 3))); }
+    __device__ void set(int f, int8_t v) { q[f >> 2] |= (uint32_t)(uint8_t)v << (8 * (f & 3)); }
+};
+inline int lo_quads(int nf) { return (nf + 3) / 4; }
+// bytes of compensation words per site of one NP-field buffer
+inline int lo_site_bytes(int lo_kind, int np) {
+    return lo_kind == LO_F32 ? 4 * np : lo_kind == LO_BF16 ? 2 * np : lo_kind == LO_Q8 ? 4 * lo_quads(np) : 0;
+}
 struct Bufs32 {
     const float *pin;    // hi words of P at the stencil level
     float *pout;

@@ -29,6 +29,7 @@
 // at the same relative LDS offset.
 #pragma once
 
+#include "scp_comp32.h"
 #include "scp_diag_site.h"
 #include "scp_physics.h"
 
@@ -173,37 +174,7 @@ struct LdsNB {
 // leapfrog updates of one scalar (identical to the direct-neighbour kernels)
 // ---------------------------------------------------------------------------
 
-constexpr float TWO_PI_HI_M = 6.28318548202514648f;      // fl32(2 pi)
-constexpr float TWO_PI_LO_M = -1.7484555314695172e-07f;  // 2 pi - TWO_PI_HI_M
-constexpr float INV_TWO_PI_M = 0.159154943091895336f;
-
-// A constant as an unevaluated pair c = h + l of FP32 words (the FP64 step
-// constants dt, dt/2, -g dx dt split once), so increments are formed with the
-// FP64 constant and not with its FP32 rounding.
-struct DF {
-    float h, l;
-};
-__device__ inline DF df_split(double c) {
-    const float h = (float)c;
-    return {h, (float)(c - (double)h)};
-}
-
-// (hi, lo) += c * (xh + xl): the increment is formed with FMAs and added by
-// Knuth's TwoSum, which is exact for any magnitudes (velocities and fields
-// pass through zero, so |hi| >= |increment| cannot be assumed).
-__device__ inline void comp_add(float &hi, float &lo, DF c, float xh, float xl) {
-    const float y = fmaf(c.h, xh, fmaf(c.l, xh, fmaf(c.h, xl, lo)));
-    const float t = hi + y;
-    const float bp = t - hi;
-    lo = (hi - (t - bp)) + (y - bp);
-    hi = t;
-}
-__device__ inline void df_scale_m(float &hi, float &lo, float d) {
-    float p = __fmul_rn(hi, d);
-    float e = fmaf(hi, d, -p) + __fmul_rn(lo, d);
-    hi = p + e;
-    lo = e - (hi - p);
-}
+// FP32C: DF, comp_add, leap32 (scp_comp32.h)
 
 struct S64 {
     static constexpr int MINB = 1;             // the staged planes fill the LDS: one block per CU
@@ -212,16 +183,21 @@ struct S64 {
     using T = double;
     using B = Bufs<double>;
     struct Own { double w; };
+    struct Lo {};                              // no compensation words
     __device__ static const double *pin(const B &b) { return b.pin; }
-    __device__ static Own own_load(const B &b, int f, uint32_t fh, uint32_t e) {
+    template <int NF>
+    __device__ static void lo_load(const B &, uint32_t, uint32_t, Lo &) {}
+    template <int NF>
+    __device__ static void lo_store(const B &, uint32_t, uint32_t, const Lo &) {}
+    __device__ static Own own_load(const B &b, int f, uint32_t fh, uint32_t e, const Lo &) {
         return {ldo_nt(fbase(b.w, f, fh), e)};
     }
     // fh, e: field stride and index of the step state; fs, ec: of the emitted state;
     // xe, ve: the emitted position and velocity (x_n, v_n in FP64)
     template <int MODE, bool EMIT>
     __device__ static void update(const KParams &kp, const B &b, int f, uint32_t fh, uint32_t e, uint32_t fs,
-                                  uint32_t ec, double a, double x, Own s, bool damp, double dmp, double &xe,
-                                  double &ve) {
+                                  uint32_t ec, double a, double x, Own s, Lo &, bool damp, double dmp,
+                                  double &xe, double &ve) {
         double v = s.w;
         v += kp.hdt * a;                               // stage 4 (C:1680-1696)
         if (damp) v *= dmp;                            // stage 5 (C:1454-1460)
@@ -252,28 +228,19 @@ struct S64D : S64 {
     static constexpr bool SPLIT_DAMP = true;
 };
 
-template <int LOK> struct LoIO;
-template <> struct LoIO<LO_F32> {
-    __device__ static float get(const void *p, int f, uint32_t fs, uint32_t e) {
-        return ldo_nt(fbase((const float *)p, f, fs), e);
+// Compensation words in the march's addressing, one word per field (LO_F32,
+// LO_BF16): raw word loads (decoded against their hi word) and encoded stores.
+// LO_Q8 words go through S32's packed per-site words instead.
+template <int LOK> struct LoIO {
+    using LW = LoWord<LOK>;
+    using W = typename LW::W;
+    __device__ static W raw(const void *p, int f, uint32_t fs, uint32_t e) {
+        if constexpr (LW::STORED) return ldo_nt(fbase((const W *)p, f, fs), e);
+        else return W{};
     }
-    __device__ static void put(void *p, int f, uint32_t fs, uint32_t e, float v) {
-        sto_nt(fbase((float *)p, f, fs), e, v);
+    __device__ static void put(void *p, int f, uint32_t fs, uint32_t e, float hi, float lo) {
+        if constexpr (LW::STORED) sto_nt(fbase((W *)p, f, fs), e, LW::enc(lo, hi));
     }
-};
-template <> struct LoIO<LO_BF16> {
-    __device__ static float get(const void *p, int f, uint32_t fs, uint32_t e) {
-        return __uint_as_float((uint32_t)ldo_nt(fbase((const uint16_t *)p, f, fs), e) << 16);
-    }
-    __device__ static void put(void *p, int f, uint32_t fs, uint32_t e, float v) {   // round to nearest even
-        uint32_t u = __float_as_uint(v);
-        u += 0x7FFFu + ((u >> 16) & 1u);
-        sto_nt(fbase((uint16_t *)p, f, fs), e, (uint16_t)(u >> 16));
-    }
-};
-template <> struct LoIO<LO_NONE> {
-    __device__ static float get(const void *, int, uint32_t, uint32_t) { return 0.0f; }
-    __device__ static void put(void *, int, uint32_t, uint32_t, float) {}
 };
 
 template <int LOK>
@@ -283,41 +250,56 @@ struct S32 {
     static constexpr bool SPLIT_DAMP = true;
     using T = float;
     using B = Bufs32;
-    struct Own { float wh, wl, xl; };
+    static constexpr int LKW = lo_kind_w(LOK), LKP = lo_kind_p(LOK);   // word storage of W and of P
+    using LWW = LoWord<LKW>;
+    using LWP = LoWord<LKP>;
+    static constexpr bool PACKW = LKW == LO_Q8, PACKP = LKP == LO_Q8;
+    struct Own { float wh; typename LWW::W wr; typename LWP::W xr; };   // W hi word; raw lo words of W and P
+    // packed compensation words of the site: W's and P's (loaded), W's and P's out (stored)
+    struct Lo { LoQuads w, x; };
     __device__ static const float *pin(const B &b) { return b.pin; }
-    __device__ static Own own_load(const B &b, int f, uint32_t fh, uint32_t e) {
-        return {ldo_nt(fbase(b.w, f, fh), e), LoIO<LOK>::get(b.wlo, f, fh, e), LoIO<LOK>::get(b.pinlo, f, fh, e)};
+    template <int NF>
+    __device__ static void lo_load(const B &b, uint32_t fh, uint32_t e, Lo &lo) {
+#pragma unroll
+        for (int q = 0; q < (NF + 3) / 4; q++) {
+            if constexpr (PACKW) lo.w.q[q] = ldo_nt(fbase((const uint32_t *)b.wlo, q, fh), e);
+            if constexpr (PACKP) lo.x.q[q] = ldo_nt(fbase((const uint32_t *)b.pinlo, q, fh), e);
+        }
+    }
+    template <int NF>
+    __device__ static void lo_store(const B &b, uint32_t fh, uint32_t e, const Lo &lo) {
+#pragma unroll
+        for (int q = 0; q < (NF + 3) / 4; q++) {
+            if constexpr (PACKW) sto_nt(fbase((uint32_t *)b.wlo, q, fh), e, lo.w.q[q]);
+            if constexpr (PACKP) sto_nt(fbase((uint32_t *)b.poutlo, q, fh), e, lo.x.q[q]);
+        }
+    }
+    __device__ static Own own_load(const B &b, int f, uint32_t fh, uint32_t e, const Lo &lo) {
+        Own o;
+        o.wh = ldo_nt(fbase(b.w, f, fh), e);
+        if constexpr (PACKW) o.wr = lo.w.get(f);
+        else o.wr = LoIO<LKW>::raw(b.wlo, f, fh, e);
+        if constexpr (PACKP) o.xr = lo.x.get(f);
+        else o.xr = LoIO<LKP>::raw(b.pinlo, f, fh, e);
+        return o;
     }
     // fh, e: field stride and index of the step state; fs, ec: of the emitted state;
     // xe, ve: the emitted position and velocity (hi + lo in FP64)
     template <int MODE, bool EMIT>
     __device__ static void update(const KParams &kp, const B &b, int f, uint32_t fh, uint32_t e, uint32_t fs,
-                                  uint32_t ec, float a, float xh, Own s, bool damp, double dmpd, double &xe,
-                                  double &ve) {
+                                  uint32_t ec, float a, float xh, Own s, Lo &lo, bool damp, double dmpd,
+                                  double &xe, double &ve) {
         const DF hdt = df_split(kp.hdt), dt = df_split(kp.dt), gad = df_split(kp.gad);
-        float wh = s.wh, wl = s.wl, vh = 0.0f, vl = 0.0f;
-        // both half kicks, every step (also between emitted steps): the result then
-        // does not depend on how a run is split into calls or batch chains
-        vh = wh; vl = wl;
-        comp_add(vh, vl, hdt, a, 0.0f);                       // stage 4: v = W + dt/2 a
-        if (damp) df_scale_m(vh, vl, (float)dmpd);                 // stage 5
-        wh = vh; wl = vl;
-        comp_add(wh, wl, hdt, a, 0.0f);                       // next stage 1
-        float xl = s.xl;
+        float wh = s.wh, wl = LWW::dec(s.wr, s.wh), xl = LWP::dec(s.xr, xh), vh, vl;
         const float xh0 = xh, xl0 = xl;
-        if (MODE == 2 && f >= F_TH) {                       // link drift + wrap
-            comp_add(xh, xl, gad, wh, wl);
-            const float n = rintf(xh * INV_TWO_PI_M);
-            if (n != 0.0f) {
-                comp_add(xh, xl, DF{-n, 0.0f}, TWO_PI_HI_M, TWO_PI_LO_M);
-            }
-        } else {
-            comp_add(xh, xl, dt, wh, wl);                   // next stage 2
-        }
+        leap32(MODE == 2 && f >= F_TH, a, damp, (float)dmpd, hdt, dt, gad, wh, wl, xh, xl, vh, vl);
         sto_nt(fbase(b.w, f, fh), e, wh);
-        LoIO<LOK>::put(b.wlo, f, fh, e, wl);
         sto_nt(fbase(b.pout, f, fh), e, xh);
-        LoIO<LOK>::put(b.poutlo, f, fh, e, xl);
+        // packed words are stored by lo_store once every field is done
+        if constexpr (PACKW) lo.w.set(f, LWW::enc(wl, wh));
+        else LoIO<LKW>::put(b.wlo, f, fh, e, wh, wl);
+        if constexpr (PACKP) lo.x.set(f, LWP::enc(xl, xh));
+        else LoIO<LKP>::put(b.poutlo, f, fh, e, xh, xl);
         xe = (double)xh0 + (double)xl0;
         ve = (double)vh + (double)vl;
         if constexpr (EMIT) {
@@ -446,8 +428,10 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
         if (t.valid) {
 #pragma unroll
             for (int f = 0; f < NF; f++) cpp[f] = ld(f, p2e + col);
+            typename S::Lo li;
+            S::template lo_load<NF>(b, fs, e, li);
 #pragma unroll
-            for (int f = 0; f < NF; f++) sw[f] = S::own_load(b, f, fs, e);
+            for (int f = 0; f < NF; f++) sw[f] = S::own_load(b, f, fs, e, li);
         } else {
 #pragma unroll
             for (int f = 0; f < NF; f++) cpp[f] = T(0);
@@ -468,26 +452,28 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
                 forces_complex(kp, c0, c1, cm, nb, acc);
             double dmp = 1.0;
             double xe[NF], ve[NF];   // the emitted state of this site (used by DIAG)
+            typename S::Lo lo{};     // packed compensation words out
             if constexpr (S::SPLIT_DAMP) {
                 // separate damped / undamped paths: a select between them would keep both live
                 if (kp.damp && sponge(kp, i, t.j, t.k, dmp)) {
 #pragma unroll
                     for (int f = 0; f < NF; f++)
-                        S::template update<MODE, EMIT>(kp, b, f, fs, e, kp.fs, ec, acc[f], c0[f], sw[f], true, dmp,
+                        S::template update<MODE, EMIT>(kp, b, f, fs, e, kp.fs, ec, acc[f], c0[f], sw[f], lo, true, dmp,
                                                        xe[f], ve[f]);
                 } else {
 #pragma unroll
                     for (int f = 0; f < NF; f++)
-                        S::template update<MODE, EMIT>(kp, b, f, fs, e, kp.fs, ec, acc[f], c0[f], sw[f], false, 1.0,
+                        S::template update<MODE, EMIT>(kp, b, f, fs, e, kp.fs, ec, acc[f], c0[f], sw[f], lo, false, 1.0,
                                                        xe[f], ve[f]);
                 }
             } else {
                 const bool damp = kp.damp && sponge(kp, i, t.j, t.k, dmp);
 #pragma unroll
                 for (int f = 0; f < NF; f++)
-                    S::template update<MODE, EMIT>(kp, b, f, fs, e, kp.fs, ec, acc[f], c0[f], sw[f], damp, dmp,
+                    S::template update<MODE, EMIT>(kp, b, f, fs, e, kp.fs, ec, acc[f], c0[f], sw[f], lo, damp, dmp,
                                                    xe[f], ve[f]);
             }
+            S::template lo_store<NF>(b, fs, e, lo);
             if constexpr (DIAG) {
                 // neighbours at the stencil level of this launch: plane i+1 (own column),
                 // plane i-1 (matter column and th_0), in-plane from the staged tile

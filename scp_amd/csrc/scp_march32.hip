@@ -18,6 +18,8 @@ template <int KW>
 static int launch_kw(int mode, bool emit, int lo_kind, const KParams &kp, const Bufs32 &b, int DI, hipStream_t s) {
     if (lo_kind == LO_F32) return launch_lo<LO_F32, KW>(mode, emit, kp, b, DI, s);
     if (lo_kind == LO_BF16) return launch_lo<LO_BF16, KW>(mode, emit, kp, b, DI, s);
+    if (lo_kind == LO_Q8) return launch_lo<LO_Q8, KW>(mode, emit, kp, b, DI, s);
+    if (lo_kind == LO_Q8P) return launch_lo<LO_Q8P, KW>(mode, emit, kp, b, DI, s);
     return launch_lo<LO_NONE, KW>(mode, emit, kp, b, DI, s);
 }
 
@@ -43,6 +45,8 @@ int launch_march_f32c_diag(int mode, int lo_kind, const KParams &kp, const Bufs3
     if (mode != 1 && mode != 2) return (int)hipErrorInvalidValue;
     if (lo_kind == LO_F32) return diag_lo<LO_F32>(mode, kp, b, DI, da, s);
     if (lo_kind == LO_BF16) return diag_lo<LO_BF16>(mode, kp, b, DI, da, s);
+    if (lo_kind == LO_Q8) return diag_lo<LO_Q8>(mode, kp, b, DI, da, s);
+    if (lo_kind == LO_Q8P) return diag_lo<LO_Q8P>(mode, kp, b, DI, da, s);
     return diag_lo<LO_NONE>(mode, kp, b, DI, da, s);
 }
 
@@ -58,6 +62,8 @@ int launch_march_batch_f32c(int mode, int lo_kind, const BatchGrid<Bufs32> *grid
     if (mode != 1 && mode != 2) return (int)hipErrorInvalidValue;
     if (lo_kind == LO_F32) return batch_lo<LO_F32>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
     if (lo_kind == LO_BF16) return batch_lo<LO_BF16>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
+    if (lo_kind == LO_Q8) return batch_lo<LO_Q8>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
+    if (lo_kind == LO_Q8P) return batch_lo<LO_Q8P>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
     return batch_lo<LO_NONE>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
 }
 

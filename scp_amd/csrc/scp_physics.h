@@ -21,25 +21,57 @@ namespace scp {
 template <typename R>
 constexpr bool diff_form() { return std::is_same<R, float>::value; }
 
-// cos(th) - 1 from (c, s) = (cos th, sin th) without cancellation near th = 0
-template <typename R>
-__device__ inline R cos_m1(R c, R s) { return c >= R(0) ? -(s * s) / (R(1) + c) : c - R(1); }
-
 #define SCP_PI 3.14159265358979323846
 
-__device__ inline void scp_sincos(double x, double *s, double *c) { sincos(x, s, c); }
-#ifdef SCP_F32_HW_TRIG
-// hardware v_sin/v_cos (argument scaled to revolutions): ~1e-6 absolute
-__device__ inline void scp_sincos(float x, float *s, float *c) { *s = __sinf(x); *c = __cosf(x); }
-#else
-__device__ inline void scp_sincos(float x, float *s, float *c) { sincosf(x, s, c); }
-#endif
+// FP32 sine, cosine and cos - 1 of a link or plaquette angle. x = k pi/2 + r
+// with pi/2 split into three FP32 parts (Cody-Waite, each step one FMA), so
+// |r| <= pi/4 carries about one rounding of r whatever k is; sin r and cos r
+// are their Taylor series to r^9 and r^10 (truncation below 2e-9 and 2e-10 of
+// the value on |r| <= pi/4, under half an FP32 ulp) by Horner's rule with FMAs;
+// k mod 4 selects and signs them. cos x - 1 keeps its relative accuracy near
+// x = 0: for k = 0 it is the cosine series without its constant term, and in
+// the other quadrants cos x - 1 is below -0.29, where the subtraction is exact
+// to one rounding. Measured against double precision: sin and cos within 1.5
+// ulp, cos - 1 within 3.5 ulp for |x| <= 100 (link angles lie in [-pi, pi],
+// plaquette angles in [-4 pi, 4 pi]).
+struct SinCos32 { float s, c, cm1; };
+__device__ inline SinCos32 sincos32(float x) {
+    constexpr float TWO_OVER_PI = 0.636619746685028076f;
+    constexpr float PIO2_1 = 1.57079637050628662f;        // fl32(pi/2)
+    constexpr float PIO2_2 = -4.37113882867379300e-08f;   // fl32(pi/2 - PIO2_1)
+    constexpr float PIO2_3 = -1.71512451000588190e-15f;   // fl32(pi/2 - PIO2_1 - PIO2_2)
+    const float k = rintf(x * TWO_OVER_PI);
+    float r = fmaf(k, -PIO2_1, x);
+    r = fmaf(k, -PIO2_2, r);
+    r = fmaf(k, -PIO2_3, r);
+    const float r2 = r * r;
+    // 1/3!, 1/5!, ... and 1/2!, 1/4!, ... rounded to FP32
+    float ps = fmaf(r2, 2.75573188446287530e-06f, -1.98412701138295233e-04f);
+    ps = fmaf(r2, ps, 8.33333376795053482e-03f);
+    ps = fmaf(r2, ps, -1.66666671633720398e-01f);
+    ps = fmaf(r * r2, ps, r);                                             // sin r
+    float pm = fmaf(r2, -2.75573199814971304e-07f, 2.48015876422869042e-05f);
+    pm = fmaf(r2, pm, -1.38888892251998186e-03f);
+    pm = fmaf(r2, pm, 4.16666679084300995e-02f);
+    pm = fmaf(r2, pm, -0.5f);
+    pm = pm * r2;                                                         // cos r - 1
+    const float pc = 1.0f + pm;
+    const int q = (int)k & 3;
+    float sv = (q & 1) ? pc : ps, cv = (q & 1) ? ps : pc;
+    if (q & 2) sv = -sv;
+    if ((q + 1) & 2) cv = -cv;
+    return {sv, cv, q == 0 ? pm : cv - 1.0f};
+}
+
+// sin, cos (and cos - 1 for the FP32 difference forms; 0 in FP64, which does
+// not use it) of a link angle
+__device__ inline void link_sincos(double x, double &s, double &c, double &cm1) { sincos(x, &s, &c); cm1 = 0.0; }
+__device__ inline void link_sincos(float x, float &s, float &c, float &cm1) {
+    const SinCos32 t = sincos32(x);
+    s = t.s; c = t.c; cm1 = t.cm1;
+}
 __device__ inline double scp_sin(double x) { return sin(x); }
-#ifdef SCP_F32_HW_TRIG
-__device__ inline float scp_sin(float x) { return __sinf(x); }
-#else
-__device__ inline float scp_sin(float x) { return sinf(x); }
-#endif
+__device__ inline float scp_sin(float x) { return sincos32(x).s; }
 
 constexpr int TK = 64, TJ = 4;
 
@@ -346,7 +378,7 @@ __device__ inline void forces_real(const KParams &kp, const R *c, const R *ip,
 // Carried per-column gauge quantities from plane i-1.
 template <typename R>
 struct GaugeCarry {
-    R c0, s0;      // cos/sin th_0(x - e0)
+    R c0, s0, m0;  // cos, sin, cos - 1 of th_0(x - e0)
     R p0, p2;           // plaquette sines (0,1) and (2,0) at x - e0
 };
 
@@ -362,12 +394,13 @@ __device__ inline void forces_gauge(const KParams &kp, const CV &c, const PV &ip
         return nb.face(f, d, sg);
     };
     // links and their cos/sin at x and x - e_d (C:1071-1079, 1144-1145)
-    R cP[3], sP[3], cM[3], sM[3];
+    // (mP, mM: cos - 1, read by the FP32 difference forms only)
+    R cP[3], sP[3], cM[3], sM[3], mP[3], mM[3];
 #pragma unroll
-    for (int d = 0; d < 3; d++) scp_sincos(c[F_TH + d], &sP[d], &cP[d]);
-    cM[0] = cy.c0; sM[0] = cy.s0;
-    scp_sincos(nbv(F_TH + 1, 1, -1), &sM[1], &cM[1]);
-    scp_sincos(nbv(F_TH + 2, 2, -1), &sM[2], &cM[2]);
+    for (int d = 0; d < 3; d++) link_sincos(c[F_TH + d], sP[d], cP[d], mP[d]);
+    cM[0] = cy.c0; sM[0] = cy.s0; mM[0] = cy.m0;
+    link_sincos(nbv(F_TH + 1, 1, -1), sM[1], cM[1], mM[1]);
+    link_sincos(nbv(F_TH + 2, 2, -1), sM[2], cM[2], mM[2]);
 
     // plaquettes (C:1088-1109) at x, x-e1, x-e2; x-e0 carried
     R th0 = c[F_TH + 0], th1 = c[F_TH + 1], th2 = c[F_TH + 2];
@@ -411,8 +444,8 @@ __device__ inline void forces_gauge(const KParams &kp, const CV &c, const PV &ip
         R TRp[6], TIp[6], tup[3], tvp[3];
         R Jg = R(0.0);
         // FP32: transported differences U x_n - x (see diff_form)
-        const R cmP = diff_form<R>() ? cos_m1(cP[d], sP[d]) : R(0);
-        const R cmM = diff_form<R>() ? cos_m1(cM[d], sM[d]) : R(0);
+        const R cmP = diff_form<R>() ? mP[d] : R(0);
+        const R cmM = diff_form<R>() ? mM[d] : R(0);
 #pragma unroll
         for (int a = 0; a < 3; a++) {
 #pragma unroll
@@ -483,7 +516,7 @@ __device__ inline void forces_gauge(const KParams &kp, const CV &c, const PV &ip
         acc[F_TU + a] = lUT - R(kp.MT2) * fu[3 + a] + R(kp.ETA) * reDxP;
         acc[F_TV + a] = lVT - R(kp.MT2) * fv[3 + a] + R(kp.ETA) * imDxP;
     }
-    cy.c0 = cP[0]; cy.s0 = sP[0]; cy.p0 = px0; cy.p2 = px2;
+    cy.c0 = cP[0]; cy.s0 = sP[0]; cy.m0 = mP[0]; cy.p0 = px0; cy.p2 = px2;
 }
 
 // carry for the segment's first plane, from the links at (i-1, j, k) (mt0..2),
@@ -491,7 +524,7 @@ __device__ inline void forces_gauge(const KParams &kp, const CV &c, const PV &ip
 template <typename R>
 __device__ inline void gauge_prologue(R mt0, R mt1, R mt2, R th1, R th2, R t0_mjp, R t0_mkp,
                                       GaugeCarry<R> &cy) {
-    scp_sincos(mt0, &cy.s0, &cy.c0);
+    link_sincos(mt0, cy.s0, cy.c0, cy.m0);
     cy.p0 = plaq(mt0, th1, t0_mjp, mt1);
     cy.p2 = plaq(mt2, t0_mkp, th2, mt0);
 }

@@ -2,89 +2,64 @@
 //
 // Storage: every scalar is a pair (hi, lo): hi is an FP32 word, lo the
 // compensation word that carries what hi cannot represent (|lo| <= ulp(hi)/2),
-// stored as FP32 (LO_F32, 8 bytes/scalar), bfloat16 (LO_BF16, 6 bytes) or not
-// at all (LO_NONE, plain FP32 for comparison). The stencil — forces, curls,
-// covariant transport, plaquettes — is evaluated in FP32 on the hi words; the
-// leapfrog accumulations v += h a, x += dt v, E and link updates are
-// compensated FP32 additions (TwoSum with the increment formed by FMAs), so
-// increments far below ulp(x) accumulate in lo instead of being lost. Both half
-// kicks are taken every step, so results do not depend on how a run is split
-// into calls. FP64 appears
-// only in INIT (from the FP64 host state) and in the emitted snapshot state
-// (hi + lo), which the FP64 diagnostics (energy, Q, Gauss residual) read.
+// stored as FP32 (LO_F32, 8 bytes/scalar), bfloat16 (LO_BF16, 6 bytes), a
+// signed byte in units of ulp(hi)/256 (LO_Q8, 5 bytes; LO_Q8P: on the
+// positions only, 4.5 bytes on average) or not at all (LO_NONE, plain FP32). The stencil — forces, curls, covariant
+// transport, plaquettes — is evaluated in FP32 on the hi words; the leapfrog
+// accumulations (kicks, drift, E and link updates) are compensated FP32
+// additions (leap32, scp_comp32.h), so increments far below ulp(x) accumulate
+// in lo instead of being lost, and the state a run reaches does not depend on
+// how it is split into calls. FP64 appears only in INIT (from the FP64 host
+// state) and in the emitted snapshot state (hi + lo), which the FP64
+// diagnostics (energy, Q, Gauss residual) read.
 //
 // Same schedule and z-march decomposition as the FP64 kernel (scp_step.hip).
-// Compiled with FMA contraction allowed: the compensated sums contain no
-// products (the increment is formed by an explicit fmaf), so contraction
-// cannot change them.
+#include "scp_comp32.h"
 #include "scp_physics.h"
 
 #include <hip/hip_runtime.h>
 
 namespace scp {
 
-constexpr float TWO_PI_HI = 6.28318548202514648f;      // fl32(2 pi)
-constexpr float TWO_PI_LO = -1.7484555314695172e-07f;  // 2 pi - TWO_PI_HI
-constexpr float INV_TWO_PI = 0.159154943091895336f;
-
-__device__ inline void fast_two_sum(float a, float b, float &s, float &e) {
-    s = a + b;
-    e = b - (s - a);
-}
-// A constant as an unevaluated pair c = h + l of FP32 words (the FP64 step
-// constants dt, dt/2, -g dx dt split once), so increments are formed with the
-// FP64 constant and not with its FP32 rounding.
-struct DF {
-    float h, l;
-};
-__device__ inline DF df_split(double c) {
-    const float h = (float)c;
-    return {h, (float)(c - (double)h)};
-}
-
-// (hi, lo) += c * (xh + xl): the increment is formed with FMAs and added by
-// Knuth's TwoSum, which is exact for any magnitudes (velocities and fields
-// pass through zero, so |hi| >= |increment| cannot be assumed).
-__device__ inline void comp_add(float &hi, float &lo, DF c, float xh, float xl) {
-    const float y = fmaf(c.h, xh, fmaf(c.l, xh, fmaf(c.h, xl, lo)));
-    const float t = hi + y;
-    const float bp = t - hi;
-    lo = (hi - (t - bp)) + (y - bp);
-    hi = t;
-}
-// (hi, lo) *= d
-__device__ inline void df_scale(float &hi, float &lo, float d) {
-    float p = __fmul_rn(hi, d);
-    float e = fmaf(hi, d, -p) + __fmul_rn(lo, d);
-    fast_two_sum(p, e, hi, lo);
-}
-
-template <int LO> struct LoStore;
-template <> struct LoStore<LO_F32> {
-    __device__ static float get(const void *p, size_t o) { return __builtin_nontemporal_load((const float *)p + o); }
-    __device__ static void put(void *p, size_t o, float v) { __builtin_nontemporal_store(v, (float *)p + o); }
-};
-template <> struct LoStore<LO_BF16> {
-    __device__ static float get(const void *p, size_t o) {
-        uint32_t b = (uint32_t)__builtin_nontemporal_load((const uint16_t *)p + o) << 16;
-        return __uint_as_float(b);
+// Compensation words of one site in one buffer (W, or P in / out) at hot index
+// c, field f at f * fh + c: raw loads are decoded against their hi word (the
+// LO_Q8 scale depends on it), stores encode against it. LO_Q8 words are packed
+// four fields to a 32-bit word (LoQuads): load() reads the site's words before
+// the field loop, store() writes them after it.
+template <int LO, int NP> struct LoSite {
+    using LW = LoWord<LO>;
+    using W = typename LW::W;
+    static constexpr bool PACKED = LO == LO_Q8;
+    LoQuads in{}, out{};
+    __device__ void load(const void *p, size_t fh, uint32_t c) {
+        if constexpr (PACKED) {
+#pragma unroll
+            for (int q = 0; q < (NP + 3) / 4; q++)
+                in.q[q] = __builtin_nontemporal_load((const uint32_t *)p + q * fh + c);
+        }
     }
-    __device__ static void put(void *p, size_t o, float v) {      // round to nearest even
-        uint32_t u = __float_as_uint(v);
-        u += 0x7FFFu + ((u >> 16) & 1u);
-        __builtin_nontemporal_store((uint16_t)(u >> 16), (uint16_t *)p + o);
+    __device__ float get(const void *p, int f, size_t fh, uint32_t c, float hi) const {
+        if constexpr (PACKED) return LW::dec(in.get(f), hi);
+        else if constexpr (LW::STORED) return LW::dec(__builtin_nontemporal_load((const W *)p + f * fh + c), hi);
+        else return 0.0f;
     }
-};
-template <> struct LoStore<LO_NONE> {
-    __device__ static float get(const void *, size_t) { return 0.0f; }
-    __device__ static void put(void *, size_t, float) {}
+    __device__ void put(void *p, int f, size_t fh, uint32_t c, float hi, float lo) {
+        if constexpr (PACKED) out.set(f, LW::enc(lo, hi));
+        else if constexpr (LW::STORED) __builtin_nontemporal_store(LW::enc(lo, hi), (W *)p + f * fh + c);
+    }
+    __device__ void store(void *p, size_t fh, uint32_t c) const {
+        if constexpr (PACKED) {
+#pragma unroll
+            for (int q = 0; q < (NP + 3) / 4; q++)
+                __builtin_nontemporal_store(out.q[q], (uint32_t *)p + q * fh + c);
+        }
+    }
 };
 
 template <int MODE, int KIND, bool EMIT, int LO>
 __global__ void __launch_bounds__(TK * TJ)
 k_step32(KParams kp, Bufs32 b, int DI) {
     constexpr int NP = MODE == 2 ? 15 : (MODE == 1 ? 12 : 6);
-    using LS = LoStore<LO>;
     const int N = kp.N;
     const Geo g = geometry(N, DI);
     if (!g.valid || g.i0 >= N) return;
@@ -144,6 +119,8 @@ k_step32(KParams kp, Bufs32 b, int DI) {
             for (int f = 0; f < NP; f++) b.acc[(size_t)f * fs + cc] = (double)acc[f];
         } else if constexpr (KIND == K_INIT) {
             // first half kick and drift in FP64 from the FP64 state, then split
+            LoSite<lo_kind_w(LO), NP> lw;
+            LoSite<lo_kind_p(LO), NP> lp;
 #pragma unroll
             for (int f = 0; f < NP; f++) {
                 const size_t o = (size_t)f * fh + chh, oc = (size_t)f * fs + cc;
@@ -157,10 +134,12 @@ k_step32(KParams kp, Bufs32 b, int DI) {
                 }
                 float wh = (float)w, xh = (float)xn;
                 b.w[o] = wh;
-                LS::put(b.wlo, o, (float)(w - (double)wh));
+                lw.put(b.wlo, f, fh, chh, wh, (float)(w - (double)wh));
                 b.pout[o] = xh;
-                LS::put(b.poutlo, o, (float)(xn - (double)xh));
+                lp.put(b.poutlo, f, fh, chh, xh, (float)(xn - (double)xh));
             }
+            lw.store(b.wlo, fh, chh);
+            lp.store(b.poutlo, fh, chh);
         } else {
             float dmp = 1.0f;
             bool damp = false;
@@ -169,38 +148,28 @@ k_step32(KParams kp, Bufs32 b, int DI) {
                 damp = sponge(kp, i, j, k, dd);
                 dmp = (float)dd;
             }
+            LoSite<lo_kind_w(LO), NP> lw;
+            LoSite<lo_kind_p(LO), NP> lp;
+            lw.load(b.wlo, fh, chh);
+            lp.load(b.pinlo, fh, chh);
 #pragma unroll
             for (int f = 0; f < NP; f++) {
                 const size_t o = (size_t)f * fh + chh, oc = (size_t)f * fs + cc;
-                float wh = __builtin_nontemporal_load(&b.w[o]), wl = LS::get(b.wlo, o);
-                float vh = 0.0f, vl = 0.0f;
-                // both half kicks, every step (also between emitted steps): the result then
-                // does not depend on how a run is split into calls or batch chains
-                vh = wh; vl = wl;
-                comp_add(vh, vl, hdt, acc[f], 0.0f);                       // stage 4: v = W + dt/2 a
-                if (damp) df_scale(vh, vl, dmp);                 // stage 5
-                wh = vh; wl = vl;
-                comp_add(wh, wl, hdt, acc[f], 0.0f);                       // next stage 1
-                float xh = c0[f], xl = LS::get(b.pinlo, o);
+                float wh = __builtin_nontemporal_load(&b.w[o]), wl = lw.get(b.wlo, f, fh, chh, wh);
+                float xh = c0[f], xl = lp.get(b.pinlo, f, fh, chh, xh), vh, vl;
                 const float xh0 = xh, xl0 = xl;
-                if (MODE == 2 && f >= F_TH) {                       // link drift + wrap
-                    comp_add(xh, xl, gad, wh, wl);
-                    float n = rintf(xh * INV_TWO_PI);
-                    if (n != 0.0f) {
-                        comp_add(xh, xl, DF{-n, 0.0f}, TWO_PI_HI, TWO_PI_LO);
-                    }
-                } else {
-                    comp_add(xh, xl, dt, wh, wl);                     // next stage 2
-                }
+                leap32(MODE == 2 && f >= F_TH, acc[f], damp, dmp, hdt, dt, gad, wh, wl, xh, xl, vh, vl);
                 __builtin_nontemporal_store(wh, &b.w[o]);
-                LS::put(b.wlo, o, wl);
+                lw.put(b.wlo, f, fh, chh, wh, wl);
                 __builtin_nontemporal_store(xh, &b.pout[o]);
-                LS::put(b.poutlo, o, xl);
+                lp.put(b.poutlo, f, fh, chh, xh, xl);
                 if constexpr (EMIT) {
                     __builtin_nontemporal_store((double)xh0 + (double)xl0, &b.xo[oc]);
                     __builtin_nontemporal_store((double)vh + (double)vl, &b.vo[oc]);
                 }
             }
+            lw.store(b.wlo, fh, chh);
+            lp.store(b.poutlo, fh, chh);
         }
 #pragma unroll
         for (int f = 0; f < NP; f++) { cm[f] = c0[f]; c0[f] = cp[f]; }
@@ -260,6 +229,8 @@ template <int MODE>
 static int launch32m(int kind, bool emit, int lo, const KParams &kp, const Bufs32 &b, hipStream_t s) {
     if (lo == LO_F32) return launch32<MODE, LO_F32>(kind, emit, kp, b, s);
     if (lo == LO_BF16) return launch32<MODE, LO_BF16>(kind, emit, kp, b, s);
+    if (lo == LO_Q8) return launch32<MODE, LO_Q8>(kind, emit, kp, b, s);
+    if (lo == LO_Q8P) return launch32<MODE, LO_Q8P>(kind, emit, kp, b, s);
     return launch32<MODE, LO_NONE>(kind, emit, kp, b, s);
 }
 

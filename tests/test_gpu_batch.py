@@ -30,7 +30,7 @@ def make(N, seed, prec, lo):
     return g
 
 
-@pytest.mark.parametrize("prec,lo", [(G.PREC_F64, 0), (G.PREC_F32C, 1), (G.PREC_F32C, 0)])
+@pytest.mark.parametrize("prec,lo", [(G.PREC_F64, 0), (G.PREC_F32C, 1), (G.PREC_F32C, 0), (G.PREC_F32C, 3), (G.PREC_F32C, 4)])
 def test_batch_equals_each_grid_alone(prec, lo):
     print(f"case prec={prec} lo={lo} sizes={SIZES}", flush=True)
     alone = [make(N, 100 + i, prec, lo) for i, N in enumerate(SIZES)]

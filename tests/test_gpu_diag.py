@@ -72,11 +72,11 @@ def test_fused_diag_equals_standalone_f64(mode, N, bc):
             assert abs(d_f[k] - d_s[k]) <= tol, f"{k}: fused {d_f[k]!r} stand-alone {d_s[k]!r}"
 
 
-@pytest.mark.parametrize("mode,N,bc", [(2, 33, 2), (1, 37, 2), (2, 48, 0)])
-def test_fused_diag_f32c(mode, N, bc):
+@pytest.mark.parametrize("mode,N,bc,lo", [(2, 33, 2, 1), (1, 37, 2, 1), (2, 48, 0, 1), (2, 40, 0, 4)])
+def test_fused_diag_f32c(mode, N, bc, lo):
     L, st, kw = _case(mode, N, bc)
-    print(f"case f32c mode={mode} N={N} bc={bc}", flush=True)
-    d_f, s_f, d_s, s_s = _pair(G.make_params(N, mode, L, 0.1, precision=G.PREC_F32C, lo_words=1, **kw), st, 9)
+    print(f"case f32c mode={mode} N={N} bc={bc} lo={lo}", flush=True)
+    d_f, s_f, d_s, s_s = _pair(G.make_params(N, mode, L, 0.1, precision=G.PREC_F32C, lo_words=lo, **kw), st, 9)
     assert np.array_equal(s_f, s_s)
     worst = {}
     for k in G.DIAG_NAMES:

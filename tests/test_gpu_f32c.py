@@ -42,7 +42,7 @@ def ball_state(N, L, link=0.05, seed=22):
     return seeds.seed_balls(N, L, 2, balls, theta_sigma=0.01, link_sigma=link, seed=seed)
 
 
-@pytest.mark.parametrize("lo", [0, 1])
+@pytest.mark.parametrize("lo", [0, 1, 3, 4])
 def test_f32c_gauged_error_vs_f64(lo):
     N, L, steps = 32, 7.75, 200
     st = ball_state(N, L)
@@ -68,12 +68,13 @@ def test_compensation_reduces_long_run_drift():
     p64 = G.make_params(N, G.MODE_GAUGED, L, 0.2, g=0.1)
     a, _, d64 = run(p64, st, steps)
     err = {}
-    for lo in (0, 2):
+    for lo in (0, 1, 3, 4, 2):
         p = G.make_params(N, G.MODE_GAUGED, L, 0.2, precision=G.PREC_F32C, lo_words=lo, g=0.1)
         b, _, d = run(p, st, steps)
         err[lo] = (np.linalg.norm(b - a) / np.linalg.norm(a), abs(d["E_total"] - d64["E_total"]))
-    print(f"compensated {err[0]}, plain {err[2]}")
-    assert err[0][0] <= err[2][0] and err[0][1] <= 1e-6 * abs(d64["E_total"])
+    print(f"compensated FP32 {err[0]}, bf16 {err[1]}, byte {err[3]}, byte on positions {err[4]}; plain {err[2]}")
+    for lo in (0, 1, 3, 4):
+        assert err[lo][0] <= err[2][0] and err[lo][1] <= 1e-6 * abs(d64["E_total"]), lo
 
 
 def test_f32c_complex_and_real_modes_run():
@@ -88,13 +89,32 @@ def test_f32c_complex_and_real_modes_run():
         assert abs(d32["E_total"] - d64["E_total"]) <= 1e-6 * abs(d64["E_total"])
 
 
+@pytest.mark.parametrize("lo", [1, 3, 4])
+def test_f32c_march_and_direct_kernels_agree(lo):
+    """The LDS-staged march and the direct-neighbour kernel decode and encode
+    the compensation words alike: after 20 steps of a gauged 24^3 ball both
+    stay within the FP32C bar of FP64 and of each other."""
+    N = 24
+    L = 0.25 * (N - 1)
+    st = ball_state(N, L)
+    a, _, d64 = run(G.make_params(N, G.MODE_GAUGED, L, 0.2, g=0.1), st, 20, project=False)
+    outs = []
+    for sk in (0, 1):
+        p = G.make_params(N, G.MODE_GAUGED, L, 0.2, precision=G.PREC_F32C, lo_words=lo, g=0.1, step_kernel=sk)
+        b, _, d = run(p, st, 20, project=False)
+        outs.append(b)
+        assert np.linalg.norm(b - a) / np.linalg.norm(a) <= 2e-5, sk
+        assert abs(d["E_total"] - d64["E_total"]) <= 1e-6 * abs(d64["E_total"]), sk
+    assert np.linalg.norm(outs[0] - outs[1]) / np.linalg.norm(a) <= 2e-5
+
+
 def q_halves(st, N):
     """Noether charge of Phi in the x < 0 and x >= 0 halves (FP64, host)."""
     q = sum(st[a] * st[18 + a] - st[12 + a] * st[6 + a] for a in range(3)).reshape(N, N, N)
     return q[: N // 2].sum(), q[N // 2:].sum()
 
 
-@pytest.mark.parametrize("lo", [1, 0])
+@pytest.mark.parametrize("lo", [1, 0, 3, 4, 2])
 def test_f32c_ten_thousand_steps_two_balls(lo):
     """128^3 gauged, two opposite Q-balls 24 cells apart (omega = +-0.8), links
     N(0, 1e-3), E = 0, Gauss-projected; 10^4 steps FP32C vs FP64."""
@@ -118,19 +138,20 @@ def test_f32c_ten_thousand_steps_two_balls(lo):
     assert field <= 1e-4
 
 
-def test_f32c_ten_thousand_steps_charged_ball():
+@pytest.mark.parametrize("lo", [1, 3, 4, 2])
+def test_f32c_ten_thousand_steps_charged_ball(lo):
     """One charged ball (net Q != 0, jellium offset from the projection), 48^3,
     10^4 steps: |dQ|/Q and dE/E against FP64."""
     N, steps = 48, 10000
     L = 0.25 * (N - 1)
     st = ball_state(N, L, link=1e-3)
     p64 = G.make_params(N, G.MODE_GAUGED, L, 0.2, g=0.05)
-    p32 = G.make_params(N, G.MODE_GAUGED, L, 0.2, precision=G.PREC_F32C, lo_words=1, g=0.05)
+    p32 = G.make_params(N, G.MODE_GAUGED, L, 0.2, precision=G.PREC_F32C, lo_words=lo, g=0.05)
     a, e0, d64 = run(p64, st, steps)
     b, _, d32 = run(p32, st, steps)
     dE = abs(d32["E_total"] - d64["E_total"]) / abs(d64["E_total"])
     dQ = abs(d32["Q_total"] - d64["Q_total"]) / abs(d64["Q_total"])
-    print(f"charged ball: Q {d64['Q_total']:.6e} dE/E {dE:.3e} |dQ|/Q {dQ:.3e} gauss {d32['gauss_max']:.3e}")
+    print(f"charged ball lo={lo}: Q {d64['Q_total']:.6e} dE/E {dE:.3e} |dQ|/Q {dQ:.3e} gauss {d32['gauss_max']:.3e}")
     assert abs(d64["Q_total"]) > 1.0
     assert dE <= 1e-6
     assert dQ <= 1e-6
