@@ -23,6 +23,8 @@
  *                      scp_sim.c:2461-2556)
  *   scp_grid_gauss_project  init_gauss_project (scp_sim.c:2284-2400) on the GPU
  *   scp_grid_rescale_phi    rescale_phi_kernel (scp_sim.cu:2619-2624)
+ *   scp_grid_vec_fit   fit_patch over the six fields of a vector frame
+ *                      (scp_sim.c:2567-2640, 3106-3117)
  *
  * Column order of every host state buffer = the reference snapshot order
  * (scp_sim.c:2862-2897): phi_x..z, theta_x..z, phi_vx..vz, theta_vx..vz,
@@ -117,6 +119,15 @@ int  scp_grid_step_diag(scp_grid *g, int nsteps, double *out, int nout);
 /* Accelerations of the current positions: 6/12/15 columns in the order
  * phi_acc, theta_acc, [phi_im_acc, theta_im_acc], [E_acc]. */
 int  scp_grid_forces(scp_grid *g, double *const *acc, int nacc);
+/* Vector-frame fits of the real mode: the tricubic Chebyshev least-squares
+ * coefficients of every block_size^3 patch (patch p at origin
+ * (bi, bj, bk) * block_size, p = (bi * bn + bj) * bn + bk, bn = N / block_size)
+ * of phi_x..z and theta_x..z (the current state rounded to FP32), 384 floats
+ * per patch: field f at 64 f + 16 a + 4 b + c for T_a(x) T_b(y) T_c(z). Replaces
+ * fit_patch (scp_sim.c:2567-2640) over the six fields (scp_sim.c:3106-3117);
+ * 4 <= block_size <= 16. n_coeffs must be bn^3 * 384. The temporal model and
+ * the FRVD frames stay on the host (libscp_io: vecframes.h). */
+int  scp_grid_vec_fit(scp_grid *g, int block_size, float *coeffs, int64_t n_coeffs);
 /* Gauss projection of the current E (gauged mode): CG solve, E correction,
  * G_offset freeze. Writes iterations and the post-projection max|G - G_offset|. */
 int  scp_grid_gauss_project(scp_grid *g, int *iters, double *gauss_max);

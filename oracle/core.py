@@ -135,3 +135,52 @@ def gauge_transform(p, state):
     st = np.array(state, dtype=np.float64).reshape(ncols(p.mode), -1).copy()
     lib().or_gauge_transform(C.byref(p), _ptr(st))
     return st
+
+
+# ---------------------------------------------------------------------------
+# Vector frames: restatement of fit_patch (scp_sim.c:2567-2640). Parity is
+# pinned by tests/test_oracle.py against the reference binary's I-frame
+# coefficients (tests/golden/real_vec_*.npz) and by a closed form (a field that
+# is a tricubic Chebyshev polynomial is fitted exactly).
+# ---------------------------------------------------------------------------
+
+def vec_basis(bs):
+    """(bs^3, 64) basis values T_a(sx) T_b(sy) T_c(sz) at the patch samples, column
+    a*16 + b*4 + c, s = 2 d/(bs-1) - 1 (scp_sim.c:2577-2597)."""
+    d = np.arange(bs, dtype=np.float64)
+    s = 2.0 * (d / (bs - 1)) - 1.0 if bs > 1 else np.zeros(1)
+    T = np.stack([np.ones_like(s), s, 2 * s * s - 1, 4 * s * s * s - 3 * s], axis=1)   # (bs, 4)
+    return np.einsum("ia,jb,kc->ijkabc", T, T, T).reshape(bs ** 3, 64)
+
+
+def vec_fit_patch(field32, N, origin, bs):
+    """Least-squares tricubic coefficients (float32, 64) of one bs^3 patch of a
+    float32 field (N^3, z fastest): normal equations accumulated in double,
+    Cholesky factor, forward and back substitution, as the reference does."""
+    A = vec_basis(bs)
+    i0, j0, k0 = origin
+    ii = np.minimum(i0 + np.arange(bs), N - 1)
+    jj = np.minimum(j0 + np.arange(bs), N - 1)
+    kk = np.minimum(k0 + np.arange(bs), N - 1)
+    f = field32.reshape(N, N, N)[np.ix_(ii, jj, kk)].astype(np.float64).ravel()
+    AtA = A.T @ A
+    Atb = A.T @ f
+    L = np.linalg.cholesky(AtA)
+    y = np.linalg.solve(L, Atb)
+    x = np.linalg.solve(L.T, y)
+    return x.astype(np.float32)
+
+
+def vec_fit(fields32, N, bs):
+    """All patches (origins in the reference's order) of the six fields:
+    ((N // bs)^3, 6, 64) float32."""
+    bn = N // bs
+    out = np.empty((bn ** 3, 6, 64), dtype=np.float32)
+    p = 0
+    for bi in range(bn):
+        for bj in range(bn):
+            for bk in range(bn):
+                for f in range(6):
+                    out[p, f] = vec_fit_patch(fields32[f], N, (bi * bs, bj * bs, bk * bs), bs)
+                p += 1
+    return out

@@ -207,7 +207,6 @@ int scp_cfg_unsupported(const scp_config *c, char *err, int n) {
     else if (c->complex_gauge && c->higgs_v > 0) bad = "higgs_v";
     else if (c->complex_gauge && c->g_gauge2 != 0) bad = "g_gauge2";
     else if (c->n_locks > 0) bad = "n_locks";
-    else if (c->vec_snap_dt > 0) bad = "vec_snap_dt";
     else if (c->self_tune) bad = "self_tune";
     else if (c->sweep) bad = "sweep";
     else if (c->tune_dt != 0) bad = "tune_dt";
@@ -216,6 +215,14 @@ int scp_cfg_unsupported(const scp_config *c, char *err, int n) {
     if (bad) {
         snprintf(msg, sizeof msg, "ERROR: %s is not implemented by this MI355X build", bad);
         return fail(err, n, msg);
+    }
+    if (c->vec_snap_dt > 0) {       /* vector frames: the fit needs 4 samples per axis; frame % interval */
+        const int bs = c->vec_block_size > 0 ? c->vec_block_size : 8;
+        if (bs < 4 || bs > 16 || bs > c->N) {
+            snprintf(msg, sizeof msg, "ERROR: vec_block_size=%d must lie in [4, min(16, N)] in this MI355X build", bs);
+            return fail(err, n, msg);
+        }
+        if (c->vec_iframe_interval < 1) return fail(err, n, "ERROR: vec_iframe_interval must be >= 1");
     }
     return 0;
 }

@@ -89,6 +89,15 @@ static void convert(const uint8_t *src, int dtype, double *dst, long n) {
 }
 
 /* scp_init.h:58-161 */
+/* Index of the k-th voxel frame from the end (k = 1: the last), -1 if there is
+ * none. Vector frames (FRVD) hold fitted patches of six fields and no
+ * velocities, so a restart or template reads the voxel frames only. */
+static int voxel_frame_from_end(sfaio *s, int k) {
+    for (int f = (int)s->total_frames - 1; f >= 0; f--)
+        if (sfaio_frame_type(s, (uint32_t)f) == SFAIO_FRAME_VOXEL && --k == 0) return f;
+    return -1;
+}
+
 static int init_from_sfa(const scp_config *c, int mode, const char *path, int frame, double *st, char *err, int n) {
     printf("Init: SFA file '%s' frame=%d\n", path, frame);
     sfaio *s = sfaio_open(path);
@@ -99,7 +108,7 @@ static int init_from_sfa(const scp_config *c, int mode, const char *path, int fr
         sfaio_close(s);
         return -1;
     }
-    if (frame < 0) frame = (int)s->total_frames + frame;
+    if (frame < 0) frame = voxel_frame_from_end(s, -frame);
     if (frame < 0 || frame >= (int)s->total_frames) {
         snprintf(err, (size_t)n, "FATAL: frame %d out of range [0,%u)", frame, s->total_frames);
         sfaio_close(s);
@@ -182,7 +191,8 @@ static int init_template(const scp_config *c, double *st, char *err, int n) {
     printf("  Template: %d^3, L=%.2f, dx=%.4f\n", TN, TL, Tdx);
     uint8_t *buf = malloc(s->frame_bytes);
     double *tv = calloc((size_t)12 * T3, sizeof(double));
-    if (!buf || !tv || sfaio_read_frame(s, s->total_frames - 1, buf)) {
+    const int tf = voxel_frame_from_end(s, 1);
+    if (!buf || !tv || tf < 0 || sfaio_read_frame(s, (uint32_t)tf, buf)) {
         free(buf); free(tv); sfaio_close(s);
         return ferr(err, n, "FATAL: cannot read template '%s'", c->init_sfa);
     }

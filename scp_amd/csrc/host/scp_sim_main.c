@@ -23,6 +23,7 @@
 #include "scp_config.h"
 #include "scp_init.h"
 #include "sfa_io.h"
+#include "vecframes.h"
 
 static void die(const char *msg) {
     fprintf(stderr, "%s\n", msg);
@@ -294,6 +295,20 @@ int main(int argc, char **argv) {
     if (c.precision != 2) x.cast = malloc((size_t)ncols * n3 * (c.precision == 0 ? 2 : 4));
     snap(&x, 0.0);
 
+    /* vector frames (FRVD) into the same file, real mode (scp_sim.c:2942-2988) */
+    scp_vecframes vf;
+    int vec_every = 0;
+    const int vec_on = c.vec_snap_dt > 0;
+    if (vec_on) {
+        if (!zst) die("ERROR: vector frames need libzstd (FRVD payloads are zstd-compressed)");
+        vec_every = (int)lround(c.vec_snap_dt / dt);
+        if (vec_every < 1) vec_every = 1;
+        if (scp_vf_init(&vf, c.N, c.vec_block_size > 0 ? c.vec_block_size : 8, c.vec_iframe_interval, c.vec_delta_tol))
+            die("FATAL: malloc failed (vector frames)");
+        printf("Vec frames: BS=%d, %d patches, %d coeffs/patch, snap every %d steps (temporal)\n", vf.bs,
+               vf.n_patches, vf.nc, vec_every);
+    }
+
     FILE *fp = fopen(c.diag_file, "w");
     if (!fp) die("ERROR: cannot open diag file");
     fprintf(fp, "t\tE_phi_kin\tE_theta_kin\tE_grad\tE_mass\tE_pot\tE_tgrad\tE_tmass\t"
@@ -352,8 +367,9 @@ int main(int argc, char **argv) {
         }
         const int do_snap = in_burst || step % snap_every == 0;
         const int do_diag = step % diag_every == 0;
+        const int do_vec = vec_on && step % vec_every == 0;
         pending++;
-        if (!do_snap && !do_diag && step != n_steps) continue;
+        if (!do_snap && !do_diag && !do_vec && step != n_steps) continue;
         /* a diag step takes the diagnostics fused into its last launch */
         if (do_diag) {
             if (scp_grid_step_diag(x.g, pending, d, SCP_D_COUNT)) die_abi("step");
@@ -362,6 +378,10 @@ int main(int argc, char **argv) {
         }
         pending = 0;
         if (do_snap) snap(&x, t);
+        if (do_vec) {                                         /* scp_sim.c:3098-3193 */
+            if (scp_grid_vec_fit(x.g, vf.bs, vf.coeffs, (int64_t)vf.n_patches * vf.nc)) die_abi("vector fit");
+            if (scp_vf_write(&vf, x.out, t)) die("FATAL: cannot write a vector frame");
+        }
         if (do_diag) {
             ROW(t);
             fflush(fp);
@@ -382,6 +402,10 @@ int main(int argc, char **argv) {
     }
     uint32_t nf = x.out->total_frames;
     sfaio_close(x.out);
+    if (vec_on) {
+        printf("Vec frames: %d written to %s (temporal)\n", vf.frame, c.output);
+        scp_vf_free(&vf);
+    }
     if (scp_grid_diag(x.g, d, SCP_D_COUNT)) die_abi("diag");
     double wall = omp_get_wtime() - wall0;
     double et = d[SCP_D_E_TOTAL], Qp = d[SCP_D_Q_PHI], Qt = d[SCP_D_Q_THETA];

@@ -258,6 +258,35 @@ static uint8_t *encode_frame(sfaio *s, const uint8_t *raw, uint64_t *out_len) {
     return p;
 }
 
+/* Before appending a frame: open a new L2 table (JMPF) when the current one is full. */
+static void open_slot(sfaio *s) {
+    FILE *fp = s->fp;
+    uint32_t zero32 = 0;
+    if (s->cur_jmpf_entries < s->jmpf_max) return;
+    fseeko(fp, 0, SEEK_END);
+    uint64_t nj = (uint64_t)ftello(fp);
+    write_jmpf(fp, s->jmpf_max);
+    s->cur_l1++;
+    uint64_t ent = s->jtop_offset + 12 + 16 + (uint64_t)s->cur_l1 * 16;
+    wr_at(fp, ent, &nj, 8); wr(fp, &s->total_frames, 4); wr(fp, &zero32, 4);
+    uint32_t ne = s->cur_l1 + 1;
+    wr_at(fp, s->jtop_offset + 12 + 4, &ne, 4);
+    s->cur_jmpf_offset = nj; s->cur_jmpf_entries = 0;
+}
+
+/* JMPF entry of the frame just appended: time, chunk offset, payload size, CRC, type. */
+static int index_frame(sfaio *s, double time, uint64_t off, uint64_t size, uint32_t crc, uint32_t type) {
+    FILE *fp = s->fp;
+    uint64_t slot = s->cur_jmpf_offset + 12 + 8 + (uint64_t)s->cur_jmpf_entries * 32;
+    wr_at(fp, slot, &time, 8); wr(fp, &off, 8); wr(fp, &size, 8); wr(fp, &crc, 4); wr(fp, &type, 4);
+    s->cur_jmpf_entries++;
+    wr_at(fp, s->cur_jmpf_offset + 12 + 4, &s->cur_jmpf_entries, 4);
+    wr_at(fp, s->jtop_offset + 12 + 16 + (uint64_t)s->cur_l1 * 16 + 12, &s->cur_jmpf_entries, 4);
+    s->total_frames++;
+    fflush(fp);
+    return fseeko(fp, 0, SEEK_END);
+}
+
 int sfaio_write_frame(sfaio *s, double time, const void *const *cols) {
     if (!s->writing) return -1;
     FILE *fp = s->fp;
@@ -273,31 +302,35 @@ int sfaio_write_frame(sfaio *s, double time, const void *const *cols) {
     uint8_t *payload = encode_frame(s, raw, &plen);
     free(raw);
     if (!payload) return -1;
-    uint32_t zero32 = 0;
-    if (s->cur_jmpf_entries >= s->jmpf_max) {               /* open a new L2 table */
-        fseeko(fp, 0, SEEK_END);
-        uint64_t nj = (uint64_t)ftello(fp);
-        write_jmpf(fp, s->jmpf_max);
-        s->cur_l1++;
-        uint64_t ent = s->jtop_offset + 12 + 16 + (uint64_t)s->cur_l1 * 16;
-        wr_at(fp, ent, &nj, 8); wr(fp, &s->total_frames, 4); wr(fp, &zero32, 4);
-        uint32_t ne = s->cur_l1 + 1;
-        wr_at(fp, s->jtop_offset + 12 + 4, &ne, 4);
-        s->cur_jmpf_offset = nj; s->cur_jmpf_entries = 0;
-    }
+    open_slot(s);
     fseeko(fp, 0, SEEK_END);
     uint64_t frmd = (uint64_t)ftello(fp);
     int e = chunk_hdr(fp, "FRMD", 12 + plen) | wr(fp, payload, plen);
     free(payload);
     if (e) return -1;
-    uint64_t slot = s->cur_jmpf_offset + 12 + 8 + (uint64_t)s->cur_jmpf_entries * 32;
-    wr_at(fp, slot, &time, 8); wr(fp, &frmd, 8); wr(fp, &plen, 8); wr(fp, &crc, 4); wr(fp, &zero32, 4);
-    s->cur_jmpf_entries++;
-    wr_at(fp, s->cur_jmpf_offset + 12 + 4, &s->cur_jmpf_entries, 4);
-    wr_at(fp, s->jtop_offset + 12 + 16 + (uint64_t)s->cur_l1 * 16 + 12, &s->cur_jmpf_entries, 4);
-    s->total_frames++;
-    fflush(fp);
-    return fseeko(fp, 0, SEEK_END);
+    return index_frame(s, time, frmd, plen, crc, SFAIO_FRAME_VOXEL);
+}
+
+/* Vector frame (FRVD chunk, sfa.h:1563-1640): type(4), size(8) = 8 + the
+ * compressed length, the frame time(8), the zstd-compressed payload; indexed
+ * like a voxel frame with its frame type and the CRC32C of the payload. */
+int sfaio_write_vec(sfaio *s, double time, uint32_t frame_type, const void *payload, uint64_t len) {
+    if (!s->writing || !zstd_load()) return -1;
+    FILE *fp = s->fp;
+    size_t bound = Z.bound(len);
+    uint8_t *comp = malloc(bound);
+    if (!comp) return -1;
+    size_t clen = Z.comp(comp, bound, payload, len, 3);
+    if (Z.iserr(clen)) { free(comp); return -1; }
+    uint32_t crc = sfaio_crc32c(payload, len);
+    open_slot(s);
+    fseeko(fp, 0, SEEK_END);
+    uint64_t off = (uint64_t)ftello(fp);
+    uint64_t c64 = clen;
+    int e = chunk_hdr(fp, "FRVD", 8 + c64) | wr(fp, &time, 8) | wr(fp, comp, clen);
+    free(comp);
+    if (e) return -1;
+    return index_frame(s, time, off, c64, crc, frame_type);
 }
 
 int sfaio_close(sfaio *s) {
@@ -446,6 +479,7 @@ static int decode_payload(const sfaio *s, int codec, const uint8_t *comp, uint64
 int sfaio_read_frame(sfaio *s, uint32_t frame, void *buf) {
     l2ent e;
     if (find_frame(s, frame, &e)) return -1;
+    if (e.type != SFAIO_FRAME_VOXEL) return -4;              /* a vector frame: sfaio_read_vec */
     if (e.size == 0 || e.size > 50000000000ULL) return -1;
     uint8_t *comp = malloc(e.size);
     if (!comp) return -1;
@@ -456,6 +490,27 @@ int sfaio_read_frame(sfaio *s, uint32_t frame, void *buf) {
     /* Unlike the reference reader, COLZSTD frames are checksummed too. */
     if (sfaio_crc32c(buf, s->frame_bytes) != e.crc) return -2;
     return 0;
+}
+
+int sfaio_frame_type(sfaio *s, uint32_t frame) {
+    l2ent e;
+    return find_frame(s, frame, &e) ? -1 : (int)e.type;
+}
+
+int64_t sfaio_read_vec(sfaio *s, uint32_t frame, void *buf, uint64_t cap) {
+    l2ent e;
+    if (find_frame(s, frame, &e)) return -1;
+    if (e.type != SFAIO_FRAME_VEC_I && e.type != SFAIO_FRAME_VEC_P) return -4;
+    if (!zstd_load()) return -3;
+    if (e.size == 0 || e.size > 50000000000ULL) return -1;
+    uint8_t *comp = malloc(e.size);
+    if (!comp) return -1;
+    if (seek(s->fp, e.offset + 12 + 8) || rd(s->fp, comp, e.size)) { free(comp); return -1; }
+    size_t r = Z.decomp(buf, cap, comp, e.size);
+    free(comp);
+    if (Z.iserr(r)) return -1;
+    if (e.crc && sfaio_crc32c(buf, r) != e.crc) return -2;   /* an index rebuilt by fixup holds 0 */
+    return (int64_t)r;
 }
 
 double sfaio_frame_time(sfaio *s, uint32_t frame) {
@@ -508,14 +563,23 @@ int sfaio_fixup_index(const char *path) {
         char t[4]; uint64_t size;
         if (seek(fp, pos) || rd(fp, t, 4) || rd(fp, &size, 8) || size < 12 || pos + size > fsize) break;
         if (memcmp(t, "JMPF", 4) == 0) { pos += size; continue; }
-        if (memcmp(t, "FRMD", 4) != 0) break;
-        uint64_t clen = size - 12;
-        uint8_t *comp = malloc(clen);
-        if (rd(fp, comp, clen)) { free(comp); break; }
-        int ok = decode_payload(s, codec, comp, clen, buf) == 0;
-        free(comp);
-        if (!ok) break;                       /* truncated or corrupt tail: stop here */
-        uint32_t crc = sfaio_crc32c(buf, s->frame_bytes);
+        const int vec = memcmp(t, "FRVD", 4) == 0;
+        if (!vec && memcmp(t, "FRMD", 4) != 0) break;
+        /* FRMD: size counts the chunk header; FRVD: size = time(8) + payload, header not counted */
+        uint64_t clen = vec ? size - 8 : size - 12, span = vec ? 12 + size : size;
+        if (pos + span > fsize) break;
+        double vt = 0;
+        uint32_t crc = 0;                     /* vector frames are indexed without a CRC (as sfa.h does) */
+        if (vec) {
+            if (rd(fp, &vt, 8)) break;
+        } else {
+            uint8_t *comp = malloc(clen);
+            if (rd(fp, comp, clen)) { free(comp); break; }
+            int ok = decode_payload(s, codec, comp, clen, buf) == 0;
+            free(comp);
+            if (!ok) break;                   /* truncated or corrupt tail: stop here */
+            crc = sfaio_crc32c(buf, s->frame_bytes);
+        }
         uint32_t slot = count % s->jmpf_max;
         if (count > 0 && slot == 0) {         /* needs the next L2 table from JTOP */
             l1++;
@@ -524,16 +588,22 @@ int sfaio_fixup_index(const char *path) {
             jm = nj;
         }
         double tm = 0;
+        uint32_t type = SFAIO_FRAME_VOXEL;
         uint64_t so = jm + 12 + 8 + (uint64_t)slot * 32;
         if (seek(fp, so) || rd(fp, &tm, 8)) break;
-        if (tm == 0.0 && count > 0) tm = s->dt * count;
-        uint32_t zero = 0;
-        wr_at(fp, so, &tm, 8); wr(fp, &pos, 8); wr(fp, &clen, 8); wr(fp, &crc, 4); wr(fp, &zero, 4);
+        if (vec) {                            /* the time is in the chunk; keep an I/P type already indexed */
+            tm = vt;
+            if (seek(fp, so + 28) || rd(fp, &type, 4)) break;
+            if (type != SFAIO_FRAME_VEC_I && type != SFAIO_FRAME_VEC_P) type = SFAIO_FRAME_VEC_I;
+        } else if (tm == 0.0 && count > 0) {
+            tm = s->dt * count;
+        }
+        wr_at(fp, so, &tm, 8); wr(fp, &pos, 8); wr(fp, &clen, 8); wr(fp, &crc, 4); wr(fp, &type, 4);
         uint32_t inl2 = slot + 1;
         wr_at(fp, jm + 12 + 4, &inl2, 4);
         wr_at(fp, s->jtop_offset + 12 + 16 + (uint64_t)l1 * 16 + 12, &inl2, 4);
         count++;
-        pos += size;
+        pos += span;
     }
     free(buf);
     uint32_t nl1 = l1 + 1;

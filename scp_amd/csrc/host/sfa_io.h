@@ -25,6 +25,8 @@ extern "C" {
 enum { SFAIO_F16 = 0, SFAIO_F32 = 1, SFAIO_F64 = 2 };
 enum { SFAIO_POSITION = 0, SFAIO_ANGLE = 1, SFAIO_VELOCITY = 2 };
 enum { SFAIO_CODEC_RAW = 0, SFAIO_CODEC_ZSTD = 1, SFAIO_CODEC_BSS = 2, SFAIO_CODEC_COLZSTD = 7 };
+/* JMPF frame types (sfa.h:73-76) */
+enum { SFAIO_FRAME_VOXEL = 0, SFAIO_FRAME_VEC_I = 1, SFAIO_FRAME_VEC_P = 2 };
 #define SFAIO_FLAG_STREAMING (1u << 5)
 #define SFAIO_MAX_COLS 64
 #define SFAIO_MAX_KVMD 16
@@ -79,10 +81,17 @@ int    sfaio_add_kvmd(sfaio *s, uint16_t set_id, uint32_t first_frame, uint32_t 
                       const char *const *keys, const char *const *vals, int n_pairs);
 int    sfaio_finalize_header(sfaio *s);
 int    sfaio_write_frame(sfaio *s, double time, const void *const *cols);
+/* Vector frame payload (the FRVD I/P layouts of sfa.h:1643-1720), zstd level 3. */
+int    sfaio_write_vec(sfaio *s, double time, uint32_t frame_type, const void *payload, uint64_t len);
 int    sfaio_close(sfaio *s);
 
 sfaio *sfaio_open(const char *path);
+/* Voxel frames only: a vector frame returns -4. */
 int    sfaio_read_frame(sfaio *s, uint32_t frame, void *buf);
+/* Frame type from the index (SFAIO_FRAME_*), -1 if out of range. */
+int    sfaio_frame_type(sfaio *s, uint32_t frame);
+/* Decompressed payload of a vector frame into buf (capacity cap); its length, or < 0. */
+int64_t sfaio_read_vec(sfaio *s, uint32_t frame, void *buf, uint64_t cap);
 double sfaio_frame_time(sfaio *s, uint32_t frame);
 /* Convenience getters for FFI callers that do not map the struct. */
 void   sfaio_info(const sfaio *s, uint32_t u[6], double d[4]);
@@ -90,7 +99,7 @@ int    sfaio_column_info(const sfaio *s, int c, char name[12], uint8_t meta[4]);
 int    sfaio_kvmd_count(const sfaio *s);
 int    sfaio_kvmd_pair(const sfaio *s, int set, int pair, const char **key, const char **val,
                        uint32_t *first_frame);
-/* Streaming repair: index FRMD chunks whose JMPF slots are zero (a writer
+/* Streaming repair: index FRMD and FRVD chunks whose JMPF slots are zero (a writer
  * that died mid-run), patch total_frames, clear the streaming flag.
  * Returns the number of indexed frames or -1. */
 int    sfaio_fixup_index(const char *path);

@@ -609,6 +609,27 @@ extern "C" int scp_grid_forces(scp_grid *g, double *const *acc, int nacc) {
     return 0;
 }
 
+extern "C" int scp_grid_vec_fit(scp_grid *g, int block_size, float *coeffs, int64_t n_coeffs) {
+    if (!g || !coeffs) return set_err(-1, "null argument");
+    if (g->mode != 0)
+        return set_err(-1, "vector frames are real-mode output (complex_phi=1 refuses vec_snap_dt, "
+                           "scp_config.h:458)");
+    if (block_size < 4 || block_size > 16 || block_size > g->kp.N)
+        return set_err(-1, "vec_block_size=%d outside [4, min(16, N)] (a tricubic fit needs 4 samples per axis)",
+                       block_size);
+    const int64_t bn = g->kp.N / block_size, want = bn * bn * bn * 384;
+    if (n_coeffs != want) return set_err(-1, "n_coeffs=%lld, (N/bs)^3 x 384 = %lld", (long long)n_coeffs, (long long)want);
+    float *d = nullptr;
+    HIPCHK(hipMalloc(&d, (size_t)want * sizeof(float)));
+    int rc = kerr(launch_vec_fit(g->xo, g->kp.fs, g->kp.N, block_size, d, g->stream), "vec fit");
+    hipError_t e = rc ? hipSuccess : hipMemcpyAsync(coeffs, d, (size_t)want * sizeof(float), hipMemcpyDeviceToHost, g->stream);
+    if (!rc && e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    hipFree(d);
+    if (rc) return rc;
+    HIPCHK(e);
+    return 0;
+}
+
 extern "C" int scp_grid_rescale_phi(scp_grid *g, double f) {
     if (!g) return set_err(-1, "null grid");
     const int nphi = g->mode == 0 ? 3 : 6;    // u (and v)

@@ -160,6 +160,11 @@ struct BatchTile {
     uint32_t grid, tile;
 };
 
+// Tricubic patch fits of the real-mode vector frames (scp_vec.hip): the six
+// fields of the emitted state xo, patches of bs^3 (4 <= bs <= 16), 384 floats
+// per patch in the reference's order (field, then a*16 + b*4 + c).
+int launch_vec_fit(const double *xo, uint32_t fs, int N, int bs, float *out, hipStream_t s);
+
 // Launchers (scp_step.hip, scp_step32.hip). Return hipError_t as int.
 int launch_step_f64(int mode, int kind, bool emit, const KParams &kp, const Bufs<double> &b,
                     hipStream_t s);

@@ -40,7 +40,8 @@ ABI_SYMBOLS = ["scp_abi_version", "scp_last_error", "scp_grid_bytes", "scp_grid_
                "scp_batch_size", "scp_batch_step", "scp_batch_step_all", "scp_batch_time_steps",
                "scp_batch_tune", "scp_batch_set_config", "scp_batch_config", "scp_grid_seed_balls",
                "scp_server_create", "scp_server_destroy", "scp_server_submit", "scp_server_round",
-               "scp_server_job", "scp_server_take", "scp_server_stats_get", "scp_server_now"]
+               "scp_server_job", "scp_server_take", "scp_server_stats_get", "scp_server_now",
+               "scp_grid_vec_fit"]
 
 
 class ScpError(RuntimeError):
@@ -145,6 +146,7 @@ def lib():
         L.scp_grid_diag.argtypes = [VP, C.POINTER(C.c_double), C.c_int]
         L.scp_grid_step_diag.argtypes = [VP, C.c_int, C.POINTER(C.c_double), C.c_int]
         L.scp_grid_forces.argtypes = [VP, DPP, C.c_int]
+        L.scp_grid_vec_fit.argtypes = [VP, C.c_int, C.POINTER(C.c_float), C.c_int64]
         L.scp_grid_gauss_project.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.scp_grid_set_g_offset.argtypes = [VP, C.c_double]
         L.scp_grid_g_offset.restype = C.c_double
@@ -259,6 +261,14 @@ class Grid:
         out = (C.c_double * len(DIAG_NAMES))()
         _check(lib().scp_grid_step_diag(self._h, int(n), out, len(DIAG_NAMES)), "scp_grid_step_diag")
         return dict(zip(DIAG_NAMES, list(out)))
+
+    def vec_fit(self, block_size=8):
+        """Vector-frame patch fits (real mode): ((N // bs)^3, 6, 64) float32."""
+        bn = self.params.N // block_size
+        out = np.empty((bn ** 3, 6, 64), dtype=np.float32)
+        _check(lib().scp_grid_vec_fit(self._h, int(block_size), out.ctypes.data_as(C.POINTER(C.c_float)), out.size),
+               "scp_grid_vec_fit")
+        return out
 
     def forces(self):
         acc = np.empty((nacc(self.mode), self.n3))

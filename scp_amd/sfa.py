@@ -5,6 +5,8 @@ Column layout of the sim snapshots (reference sfa/sim/scp_sim.c:2862-2897):
 """
 import ctypes as C
 
+import struct
+
 import numpy as np
 
 from . import _native
@@ -56,6 +58,12 @@ def _lib():
         lib.sfaio_kvmd_pair.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_char_p),
                                         C.POINTER(C.c_char_p), C.POINTER(C.c_uint32)]
         lib.sfaio_fixup_index.argtypes = [C.c_char_p]
+        lib.sfaio_frame_type.argtypes = [C.c_void_p, C.c_uint32]
+        lib.scp_vf_init.argtypes = [C.POINTER(VecFrames), C.c_int, C.c_int, C.c_int, C.c_double]
+        lib.scp_vf_write.argtypes = [C.POINTER(VecFrames), C.c_void_p, C.c_double]
+        lib.scp_vf_free.argtypes = [C.POINTER(VecFrames)]
+        lib.sfaio_read_vec.restype = C.c_int64
+        lib.sfaio_read_vec.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]
         lib.sfaio_crc32c.restype = C.c_uint32
         lib.sfaio_crc32c.argtypes = [C.c_void_p, C.c_size_t]
         lib._sfa_typed = True
@@ -67,10 +75,29 @@ def crc32c(buf):
     return _lib().sfaio_crc32c(b.ctypes.data, b.size)
 
 
-def write_sim_sfa(path, N, L, dt, frames, dtype=F64, codec=CODEC_RAW, kvmd=None, ncols=None):
-    """Write frames [(time, state[ncols, N^3] float64), ...] as a sim snapshot archive."""
+class VecFrames(C.Structure):
+    """scp_vecframes (vecframes.h): the host side of the real-mode vector frames."""
+    _fields_ = [("bs", C.c_int), ("n_patches", C.c_int), ("nc", C.c_int), ("iframe_interval", C.c_int),
+                ("refit_interval", C.c_int), ("delta_tol", C.c_double), ("omega", C.c_float),
+                ("frame", C.c_int), ("count", C.c_int), ("origins", C.POINTER(C.c_int16)),
+                ("coeffs", C.POINTER(C.c_float)), ("mean", C.POINTER(C.c_float)), ("amp", C.POINTER(C.c_float)),
+                ("phase", C.POINTER(C.c_float)), ("sum_mean", C.POINTER(C.c_float)),
+                ("sum_cos", C.POINTER(C.c_float)), ("sum_sin", C.POINTER(C.c_float)),
+                ("delta", C.POINTER(C.c_float)), ("idx", C.POINTER(C.c_uint32))]
+
+
+def write_sim_sfa(path, N, L, dt, frames, dtype=F64, codec=CODEC_RAW, kvmd=None, ncols=None, vec=None):
+    """Write frames [(time, state[ncols, N^3] float64), ...] as a sim snapshot archive.
+    With vec = (block_size, iframe_interval, delta_tol), entries ("vec", time,
+    coeffs[(N // bs)^3, 384] float32) are vector frames: the temporal model and
+    the I/P-frame choice of scp_vf_write (libscp_io) applied to the given fits."""
     lib = _lib()
-    ncols = ncols or frames[0][1].shape[0]
+    ncols = ncols or next(f[1].shape[0] for f in frames if f[0] != "vec")
+    vf = None
+    if vec:
+        vf = VecFrames()
+        if lib.scp_vf_init(C.byref(vf), N, vec[0], vec[1], vec[2]):
+            raise MemoryError("scp_vf_init")
     s = lib.sfaio_create(str(path).encode(), N, N, N, L, L, L, dt)
     if not s:
         raise OSError(f"cannot create {path}")
@@ -85,7 +112,17 @@ def write_sim_sfa(path, N, L, dt, frames, dtype=F64, codec=CODEC_RAW, kvmd=None,
     for name, sem, comp in SIM_COLUMNS[:ncols]:
         lib.sfaio_add_column(s, name.encode(), dtype, sem, comp)
     lib.sfaio_finalize_header(s)
-    for t, st in frames:
+    for fr in frames:
+        if fr[0] == "vec":
+            c = np.ascontiguousarray(fr[2], dtype=np.float32).ravel()
+            if c.size != vf.n_patches * vf.nc:
+                raise ValueError(f"vector frame needs {vf.n_patches} x {vf.nc} coefficients")
+            C.memmove(vf.coeffs, c.ctypes.data, c.nbytes)
+            if lib.scp_vf_write(C.byref(vf), s, float(fr[1])):
+                lib.sfaio_close(s)
+                raise OSError("vector frame write failed")
+            continue
+        t, st = fr
         st = np.asarray(st, dtype=np.float64).reshape(ncols, -1)
         if dtype == F64:
             cols = [np.ascontiguousarray(st[c]) for c in range(ncols)]
@@ -97,6 +134,8 @@ def write_sim_sfa(path, N, L, dt, frames, dtype=F64, codec=CODEC_RAW, kvmd=None,
         if lib.sfaio_write_frame(s, float(t), ptrs):
             lib.sfaio_close(s)
             raise OSError("frame write failed")
+    if vf is not None:
+        lib.scp_vf_free(C.byref(vf))
     if lib.sfaio_close(s):
         raise OSError("close failed")
 
@@ -152,6 +191,37 @@ class SFAFile:
             cols.append(a)
             off += nb
         return np.stack(cols) if as_f64 else cols
+
+    def frame_type(self, f):
+        """0 voxel frame (FRMD), 1 vector I-frame, 2 vector P-frame (FRVD)."""
+        return self._lib.sfaio_frame_type(self._s, f)
+
+    def read_vec(self, f):
+        """A vector frame's payload, parsed (layouts of sfa.h:1643-1720)."""
+        cap = 100 * self.n_total + 4096       # an I-frame with its model, patches of 4^3 or larger
+        buf = np.empty(cap, dtype=np.uint8)
+        n = self._lib.sfaio_read_vec(self._s, f, buf.ctypes.data, cap)
+        if n < 0:
+            raise OSError(f"read_vec({f}) failed rc={n}")
+        b = buf[:n].tobytes()
+        t = self.frame_type(f)
+        out = {"type": t, "time": self.frame_time(f)}
+        if t == 1:
+            npch, bs, nc, flags = struct.unpack_from("<IBHB", b, 0)
+            o = 8
+            out.update(n_patches=npch, block_size=bs, n_coeffs=nc, flags=flags)
+            out["origins"] = np.frombuffer(b, np.int16, npch * 3, o).reshape(npch, 3); o += npch * 6
+            out["coeffs"] = np.frombuffer(b, np.float32, npch * nc, o).reshape(npch, nc); o += npch * nc * 4
+            if flags & 1:
+                out["omega"] = struct.unpack_from("<f", b, o)[0]; o += 4
+                for k in ("mean", "amp", "phase"):
+                    out[k] = np.frombuffer(b, np.float32, npch * nc, o).reshape(npch, nc); o += npch * nc * 4
+        else:
+            nd, nc = struct.unpack_from("<IH", b, 0)
+            out.update(n_deltas=nd, n_coeffs=nc)
+            out["indices"] = np.frombuffer(b, np.uint32, nd, 8)
+            out["deltas"] = np.frombuffer(b, np.float32, nd * nc, 8 + 4 * nd).reshape(nd, nc)
+        return out
 
     def close(self):
         if self._s:

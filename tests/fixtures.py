@@ -10,7 +10,9 @@ GOLDEN = os.path.join(HERE, "golden")
 
 
 def names():
-    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+    """The stepping fixtures (the vector-frame fixtures, real_vec_*, hold FRVD payloads)."""
+    return sorted(n for n in (os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+                  if not n.startswith("real_vec_"))
 
 
 def load(name):

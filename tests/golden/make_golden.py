@@ -9,7 +9,9 @@ runs it on small seeded states loaded through init=sfa and stores, per case:
   times    snapshot times
   diag     the reference diag.tsv text
   stdout   the reference console log (Gauss projection line, Wall line)
-Run from the repo root:  python tests/golden/make_golden.py
+and, for the vector-frame cases (VEC_CASES), every FRVD payload parsed plus
+the f32 fields at the I-frame times.
+Run from the repo root:  python tests/golden/make_golden.py [case ...]
 """
 import json
 import os
@@ -137,13 +139,84 @@ def run_case(name, spec, workdir):
     print(f"{name}: frames={nf} t={times.tolist()} bytes={os.path.getsize(os.path.join(OUT, name + '.npz'))}")
 
 
+# Vector frames (FRVD, real mode): name -> (N, L, block size, I-frame interval, delta tol, T).
+# Snapshots every vector frame in f32 (the fit reads the fields rounded to f32).
+VEC_CASES = {
+    "real_vec_n16": (16, 3.5, 8, 4, 0.075, 1.4),
+    "real_vec_n12_bs5": (12, 2.75, 5, 3, 0.05, 1.0),
+}
+
+
+def run_vec_case(name, spec, workdir):
+    """The reference's vector frames: every FRVD payload, parsed, and the f32
+    voxel frame (phi and theta columns) at the first I-frame. The seed is
+    rounded to f32 (stored that way) before the reference runs from it."""
+    N, L, bs, iint, tol, T = spec
+    rng = np.random.default_rng(sum(map(ord, name)))
+    dtf = 0.1
+    dx = 2 * L / (N - 1)
+    dt = dtf * dx
+    vdt = 4 * dt
+    st = seed_state(N, L, 0, rng, [(0.3, -0.2, 0.1, 0.8, 1.2 * dx * 3, 0, 1)], vel_noise=0.05)
+    st = st.astype(np.float32).astype(np.float64)
+    seed_path = os.path.join(workdir, name + "_seed.sfa")
+    sfa.write_sim_sfa(seed_path, N, L, dt, [(0.0, st)], codec=sfa.CODEC_RAW)
+    cfg = {"N": N, "L": L, "T": T, "dt_factor": dtf, "precision": "f32", "bc_type": 2,
+           "snap_dt": vdt, "diag_dt": T / 2, "vec_snap_dt": vdt, "vec_block_size": bs,
+           "vec_iframe_interval": iint, "vec_delta_tol": tol, "init": "sfa", "init_sfa": seed_path,
+           "output": os.path.join(workdir, name + ".sfa"), "diag_file": os.path.join(workdir, name + "_diag.tsv")}
+    cfg_path = os.path.join(workdir, name + ".cfg")
+    with open(cfg_path, "w") as f:
+        for k, v in cfg.items():
+            f.write(f"{k} = {v}\n")
+    env = dict(os.environ, OMP_NUM_THREADS="4")
+    res = subprocess.run([REF, cfg_path], capture_output=True, text=True, env=env, cwd=workdir)
+    if res.returncode != 0:
+        raise RuntimeError(f"{name}: reference failed\n{res.stdout}\n{res.stderr}")
+    out = {}
+    vk, last_voxel = 0, None
+    with sfa.SFAFile(cfg["output"]) as f:
+        types = [f.frame_type(i) for i in range(f.total_frames)]
+        for i, t in enumerate(types):
+            if t == 0:
+                last_voxel = i
+                continue
+            v = f.read_vec(i)
+            pre = f"v{vk}_"
+            out[pre + "time"] = v["time"]
+            out[pre + "type"] = t
+            if t == 1:
+                for k in ("origins", "coeffs", "mean", "amp", "phase"):
+                    out[pre + k] = np.array(v[k])
+                out[pre + "omega"] = v["omega"]
+                if vk == 0:   # the voxel frame of the same step (the snapshot precedes the vector frame)
+                    fr = f.read_frame(last_voxel, as_f64=False)
+                    out["fields0"] = np.stack([fr[c] for c in (0, 1, 2, 3, 4, 5)])
+                    out["fields0_time"] = f.frame_time(last_voxel)
+            else:
+                out[pre + "indices"] = np.array(v["indices"])
+                out[pre + "deltas"] = np.array(v["deltas"])
+            vk += 1
+        times = np.array([f.frame_time(i) for i in range(f.total_frames)])
+    params = {k: v for k, v in cfg.items() if k not in ("output", "diag_file", "init_sfa")}
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), params=json.dumps(params), frame0=st.astype(np.float32),
+                        types=np.array(types), times=times, n_vec=vk, stdout=res.stdout, **out)
+    kinds = "".join("IP"[t - 1] for t in types if t)
+    nsel = [len(out[f"v{k}_indices"]) for k in range(vk) if f"v{k}_indices" in out]
+    print(f"{name}: {len(types)} frames, vector frames {kinds}, P-frame patches {nsel}, "
+          f"bytes={os.path.getsize(os.path.join(OUT, name + '.npz'))}")
+
+
 def main():
     if not os.path.exists(REF):
         sys.exit(f"{REF} missing: run `make -C oracle ref` first")
-    names = sys.argv[1:] or list(CASES)
+    names = sys.argv[1:] or list(CASES) + list(VEC_CASES)
     with tempfile.TemporaryDirectory() as wd:
         for n in names:
-            run_case(n, CASES[n], wd)
+            if n in VEC_CASES:
+                run_vec_case(n, VEC_CASES[n], wd)
+            else:
+                run_case(n, CASES[n], wd)
 
 
 if __name__ == "__main__":

@@ -242,3 +242,59 @@ def test_config_runs_the_gpu_step_real_terms():
     assert c.validate() is None and c.unsupported() is None
     c.set("complex_phi", "1")
     assert "incompatible with sigma_grad" in c.validate()
+
+
+def test_vector_frames_in_the_container(tmp_path):
+    """FRVD chunks (sfa.h:1563-1720): indexed with their I/P type, read back as
+    payloads, refused as voxel data, and re-indexed by the streaming fixup (type
+    I where the slot was lost, time from the chunk, as sfa_fixup_index does)."""
+    N, bs = 8, 4
+    rng = np.random.default_rng(3)
+    st = rng.standard_normal((12, N ** 3))
+    fits = [rng.standard_normal((8, 384)).astype(np.float32) for _ in range(3)]
+    frames = [(0.0, st), ("vec", 0.5, fits[0]), ("vec", 1.0, fits[1]), ("vec", 1.5, fits[2])]
+    p = tmp_path / "v.sfa"
+    sfa.write_sim_sfa(p, N, 2.0, 0.1, frames, codec=sfa.CODEC_COLZSTD, vec=(bs, 2, 0.0))
+    with sfa.SFAFile(p) as f:
+        assert [f.frame_type(i) for i in range(4)] == [0, 1, 2, 1]
+        assert np.array_equal(f.read_frame(0), st)
+        with pytest.raises(OSError, match="rc=-4"):
+            f.read_frame(1)
+        v = f.read_vec(1)
+        assert v["n_patches"] == 8 and v["block_size"] == bs and v["n_coeffs"] == 384 and v["flags"] == 1
+        assert np.array_equal(v["coeffs"], fits[0]) and np.array_equal(v["mean"], fits[0])
+        assert np.array_equal(v["origins"][:, 2], [0, 4] * 4)
+        w = f.read_vec(2)                      # tolerance 0: every patch differs from the bootstrap model
+        assert w["n_deltas"] == 8 and np.allclose(w["deltas"], fits[1] - fits[0], atol=1e-6)
+        assert [f.frame_time(i) for i in range(4)] == [0.0, 0.5, 1.0, 1.5]
+    b = bytearray(open(p, "rb").read())
+    jtop = struct.unpack_from("<Q", b, 72)[0]
+    jm = struct.unpack_from("<Q", b, jtop + 28)[0]
+    struct.pack_into("<I", b, 16, struct.unpack_from("<I", b, 16)[0] | 0x20)
+    struct.pack_into("<I", b, 68, 0)
+    for i in range(1, 4):
+        b[jm + 20 + 32 * i: jm + 20 + 32 * i + 32] = bytes(32)
+    open(p, "wb").write(bytes(b))
+    assert sfa.fixup_index(p) == 4
+    with sfa.SFAFile(p) as f:
+        assert [f.frame_type(i) for i in range(4)] == [0, 1, 1, 1]
+        assert [f.frame_time(i) for i in range(4)] == [0.0, 0.5, 1.0, 1.5]
+        assert np.array_equal(f.read_vec(3)["coeffs"], fits[2])
+
+
+def test_config_vector_frames():
+    """vec_snap_dt runs in the real mode; complex_phi refuses it (scp_config.h:458);
+    this build needs 4 <= vec_block_size <= 16 and vec_iframe_interval >= 1."""
+    c = config.Config()
+    c.set("N", "32")
+    c.set("vec_snap_dt", "1.0")
+    assert c.validate() is None and c.unsupported() is None
+    c.set("vec_block_size", "3")
+    assert "vec_block_size=3" in c.unsupported()
+    c.set("vec_block_size", "0")               # 0: the default 8
+    assert c.unsupported() is None
+    c.set("vec_iframe_interval", "0")
+    assert "vec_iframe_interval" in c.unsupported()
+    c.set("vec_iframe_interval", "5")
+    c.set("complex_phi", "1")
+    assert "incompatible with vec_snap_dt" in c.validate()

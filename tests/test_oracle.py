@@ -395,3 +395,120 @@ def test_gpu_terms_frequency_mismatch_closed_form():
             expect[tuple(n)] = om / 6
     assert np.abs(d[3] + 0.4 * expect.ravel() * th0[0]).max() <= 1e-15
     assert np.abs(d[:3]).max() == 0 and np.abs(d[4:6]).max() == 0
+
+
+# ---------------------------------------------------------------------------
+# vector frames (real mode, FRVD)
+# ---------------------------------------------------------------------------
+
+VEC_FIXTURES = ["real_vec_n16", "real_vec_n12_bs5"]
+
+
+def _vec_fixture(name):
+    import json
+    z = np.load(os.path.join(fixtures.GOLDEN, name + ".npz"))
+    return z, json.loads(str(z["params"]))
+
+
+@pytest.mark.parametrize("name", VEC_FIXTURES)
+def test_vec_fit_oracle_vs_reference_iframe(name):
+    """fit_patch restated (normal equations, Cholesky) on the reference's own f32
+    fields reproduces the reference's first I-frame: bitwise but for rare last-bit
+    differences of near-zero coefficients."""
+    z, p = _vec_fixture(name)
+    N, bs = int(p["N"]), int(p["vec_block_size"])
+    mine = oracle.vec_fit(z["fields0"], N, bs).reshape(-1, 384)
+    ref = z["v0_coeffs"]
+    assert np.array_equal(z["v0_origins"][:, 0] % bs, np.zeros(len(ref)))
+    exact = np.mean(mine == ref)
+    err = np.abs(mine.astype(np.float64) - ref) / np.maximum(np.abs(ref), 1e-30 + np.abs(ref).max() * 1e-6)
+    print(f"{name}: {exact:.5f} of the coefficients bitwise, largest scaled difference {err.max():.2e}")
+    assert exact >= 0.999
+    assert err.max() <= 2e-7
+
+
+def test_vec_fit_oracle_closed_form():
+    """A field that is a tricubic Chebyshev polynomial on each patch is fitted exactly
+    (to FP32 rounding of the samples); the coefficient order is a*16 + b*4 + c with a
+    along x (the slowest index)."""
+    rng = np.random.default_rng(4)
+    for bs in (4, 5, 8):
+        N = 2 * bs
+        A = oracle.vec_basis(bs)
+        truth = rng.standard_normal((8, 64))
+        field = np.zeros((N, N, N))
+        p = 0
+        for bi in range(2):
+            for bj in range(2):
+                for bk in range(2):
+                    field[bi * bs:(bi + 1) * bs, bj * bs:(bj + 1) * bs, bk * bs:(bk + 1) * bs] = \
+                        (A @ truth[p]).reshape(bs, bs, bs)
+                    p += 1
+        f32 = field.astype(np.float32).ravel()
+        got = oracle.vec_fit(np.stack([f32] * 6), N, bs)
+        assert np.abs(got[:, 0, :] - truth).max() <= 1e-5 * np.abs(truth).max()
+        # T_1(x) alone (a linear ramp along i inside a patch) lands in coefficient 16
+        ramp = np.zeros((N, N, N), dtype=np.float32)
+        s = 2.0 * (np.arange(bs) / (bs - 1)) - 1.0
+        ramp[:bs, :bs, :bs] = s[:, None, None]
+        c = oracle.vec_fit(np.stack([ramp.ravel()] * 6), N, bs)[0, 0]
+        assert abs(c[16] - 1.0) <= 1e-6 and np.abs(np.delete(c, 16)).max() <= 1e-6
+
+
+@pytest.mark.parametrize("name", VEC_FIXTURES)
+def test_vector_frames_host_side_vs_reference(tmp_path, name):
+    """The product's host side of the vector frames (scp_vf_write: temporal model,
+    I/P choice, FRVD payloads, JMPF types), fed with the oracle's states (the real
+    mode is bitwise the reference) and the oracle's fits, reproduces every vector
+    frame the reference binary wrote."""
+    z, p = _vec_fixture(name)
+    N, L, bs = int(p["N"]), float(p["L"]), int(p["vec_block_size"])
+    iint, tol, T = int(p["vec_iframe_interval"]), float(p["vec_delta_tol"]), float(p["T"])
+    dx = 2 * L / (N - 1)
+    dt = 0.1 * dx
+    every = max(1, round(float(p["vec_snap_dt"]) / dt))
+    n_steps = round(T / dt)
+    op = oracle.make_params(N, 0, L, 0.1)
+    st = z["frame0"].astype(np.float64)
+    frames, acc = [(0.0, st)], None
+    for step in range(every, n_steps + 1, every):
+        st, acc = oracle.step(op, st, every, acc=acc)
+        f32 = st[[0, 1, 2, 3, 4, 5]].astype(np.float32)
+        frames.append(("vec", step * dt, oracle.vec_fit(f32, N, bs).reshape(-1, 384)))
+    out = tmp_path / "v.sfa"
+    sfa.write_sim_sfa(out, N, L, dt, frames, vec=(bs, iint, tol))
+    _compare_vec_frames(out, z, first_frame=1)
+
+
+def _compare_vec_frames(path, z, first_frame):
+    """Every vector frame of `path` against the reference's (fixture z)."""
+    nv = int(z["n_vec"])
+    with sfa.SFAFile(path) as f:
+        kinds = [f.frame_type(i) for i in range(f.total_frames)]
+        vec_idx = [i for i, t in enumerate(kinds) if t in (1, 2)]
+        assert len(vec_idx) == nv
+        ref_kinds = [int(t) for t in z["types"]]
+        assert [k for k in kinds if k] == [k for k in ref_kinds if k]
+        for k, i in enumerate(vec_idx):
+            v = f.read_vec(i)
+            pre = f"v{k}_"
+            assert v["type"] == int(z[pre + "type"])
+            assert abs(v["time"] - float(z[pre + "time"])) <= 1e-12
+            if v["type"] == 1:
+                assert np.array_equal(v["origins"], z[pre + "origins"])
+                assert v["omega"] == float(z[pre + "omega"])
+                ref = z[pre + "coeffs"]
+                scale = np.abs(ref).max()
+                assert np.abs(v["coeffs"] - ref).max() <= 2e-7 * scale
+                for key in ("mean", "amp"):
+                    assert np.abs(v[key] - z[pre + key]).max() <= 1e-6 * scale, key
+                # the model's prediction (phase is ill-defined where the amplitude vanishes)
+                for t in (v["time"], v["time"] + 1.0):
+                    pm = v["mean"] + v["amp"] * np.cos(v["omega"] * t + v["phase"])
+                    pr = z[pre + "mean"] + z[pre + "amp"] * np.cos(v["omega"] * t + z[pre + "phase"])
+                    assert np.abs(pm - pr).max() <= 1e-5 * scale
+            else:
+                assert np.array_equal(v["indices"], z[pre + "indices"]), (k, v["indices"], z[pre + "indices"])
+                if len(v["indices"]):
+                    d = z[pre + "deltas"]
+                    assert np.abs(v["deltas"] - d).max() <= 1e-5 * max(np.abs(d).max(), 1e-3)
