@@ -178,6 +178,9 @@ struct LdsNB {
 
 struct S64 {
     static constexpr int MINB = 1;             // the staged planes fill the LDS: one block per CU
+    // own-site words loaded with their plane: one plane ahead (30 more VGPRs) measured
+    // 1-6 % slower at 512^3 with one 4-wave block per CU
+    static constexpr bool WAHEAD = false;
     static constexpr bool REGQ = true;         // registers to spare: own column of planes i, i+1 in registers
     static constexpr bool SPLIT_DAMP = false;
     using T = double;
@@ -246,6 +249,7 @@ template <int LOK> struct LoIO {
 template <int LOK>
 struct S32 {
     static constexpr int MINB = 2;             // two blocks per CU: registers capped at 256
+    static constexpr bool WAHEAD = true;       // own-site words loaded one plane ahead (+3 % at 512^3)
     static constexpr bool REGQ = false;        // own column of planes i, i+1 read back from LDS
     static constexpr bool SPLIT_DAMP = true;
     using T = float;
@@ -416,22 +420,40 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
         }
     }
 
+    // own-site words (W and compensation words) of the plane at element offset e
+    auto load_own = [&](uint32_t e, typename S::Own *o) {
+        typename S::Lo li;
+        S::template lo_load<NF>(b, fs, e, li);
+#pragma unroll
+        for (int f = 0; f < NF; f++) o[f] = S::own_load(b, f, fs, e, li);
+    };
+    typename S::Own swn[S::WAHEAD ? NF : 1];
+    if constexpr (S::WAHEAD) {
+        if (t.valid && t.i0 < t.i1) load_own((uint32_t)t.i0 * NN + col, swn);
+    }
+
     int bb = 0;
     for (int i = t.i0; i < t.i1; i++) {
         const int nbuf = (bb + 1 == MNBUF) ? 0 : bb + 1;
         const uint32_t p2e = (uint32_t)wp(wp(i, N), N) * NN;
         const uint32_t e = (uint32_t)i * NN + col;
         const uint32_t ec = (uint32_t)i * (uint32_t)N * (uint32_t)N + colc;
-        // prefetch plane i+2 (own column + halo) and this plane's own-site velocity words
+        // prefetch plane i+2 (own column + halo) and the own-site velocity words of
+        // plane i+1 (WAHEAD: a whole plane of compute to arrive) or of plane i
         T cpp[NF], hrr[RQ];
         typename S::Own sw[NF];
+        if constexpr (S::WAHEAD) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) sw[f] = swn[f];
+        }
         if (t.valid) {
 #pragma unroll
             for (int f = 0; f < NF; f++) cpp[f] = ld(f, p2e + col);
-            typename S::Lo li;
-            S::template lo_load<NF>(b, fs, e, li);
-#pragma unroll
-            for (int f = 0; f < NF; f++) sw[f] = S::own_load(b, f, fs, e, li);
+            if constexpr (S::WAHEAD) {
+                if (i + 1 < t.i1) load_own(e + NN, swn);
+            } else {
+                load_own(e, sw);
+            }
         } else {
 #pragma unroll
             for (int f = 0; f < NF; f++) cpp[f] = T(0);

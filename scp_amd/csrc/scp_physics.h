@@ -63,14 +63,60 @@ __device__ inline SinCos32 sincos32(float x) {
     return {sv, cv, q == 0 ? pm : cv - 1.0f};
 }
 
+// FP64 sine and cosine of a link or plaquette angle, by the same scheme:
+// pi/2 in three FP64 parts (FMA reduction), the Taylor series of sin r to
+// r^17 and of cos r to r^16 on |r| <= pi/4 (truncation below 1e-19 and 3e-18 of
+// the value, under 2^-53), Horner with FMAs, the quadrant k mod 4. Measured
+// against 60-digit references: sin within 1.04 ulp, cos within 1.18 ulp (the
+// device library's are of the same order; glibc's, which the reference uses,
+// within 0.5), for |x| <= 13. Like any device sin/cos it is not glibc's bit for
+// bit, which the gauged FP64 parity bar (1e-12 relative) allows for.
+struct SinCos64 { double s, c; };
+__device__ inline SinCos64 sincos64(double x) {
+    constexpr double TWO_OVER_PI = 0.6366197723675814;
+    constexpr double PIO2_1 = 1.5707963267948966;        // fl64(pi/2)
+    constexpr double PIO2_2 = 6.123233995736766e-17;     // fl64(pi/2 - PIO2_1)
+    constexpr double PIO2_3 = -1.4973849048591698e-33;   // fl64(pi/2 - PIO2_1 - PIO2_2)
+    const double k = rint(x * TWO_OVER_PI);
+    double r = fma(k, -PIO2_1, x);
+    r = fma(k, -PIO2_2, r);
+    r = fma(k, -PIO2_3, r);
+    const double r2 = r * r;
+    // (-1)^n / (2n+1)! and (-1)^n / (2n)!, n >= 1, rounded to FP64
+    double ps = fma(r2, 2.8114572543455206e-15, -7.647163731819816e-13);
+    ps = fma(r2, ps, 1.6059043836821613e-10);
+    ps = fma(r2, ps, -2.505210838544172e-08);
+    ps = fma(r2, ps, 2.7557319223985893e-06);
+    ps = fma(r2, ps, -1.984126984126984e-04);
+    ps = fma(r2, ps, 8.333333333333333e-03);
+    ps = fma(r2, ps, -1.6666666666666666e-01);
+    ps = fma(r * r2, ps, r);                                              // sin r
+    double pm = fma(r2, 4.779477332387385e-14, -1.1470745597729725e-11);
+    pm = fma(r2, pm, 2.08767569878681e-09);
+    pm = fma(r2, pm, -2.755731922398589e-07);
+    pm = fma(r2, pm, 2.48015873015873e-05);
+    pm = fma(r2, pm, -1.388888888888889e-03);
+    pm = fma(r2, pm, 4.1666666666666664e-02);
+    pm = fma(r2, pm, -0.5);
+    const double pc = fma(pm, r2, 1.0);                                   // cos r
+    const int q = (int)k & 3;
+    double sv = (q & 1) ? pc : ps, cv = (q & 1) ? ps : pc;
+    if (q & 2) sv = -sv;
+    if ((q + 1) & 2) cv = -cv;
+    return {sv, cv};
+}
+
 // sin, cos (and cos - 1 for the FP32 difference forms; 0 in FP64, which does
 // not use it) of a link angle
-__device__ inline void link_sincos(double x, double &s, double &c, double &cm1) { sincos(x, &s, &c); cm1 = 0.0; }
+__device__ inline void link_sincos(double x, double &s, double &c, double &cm1) {
+    const SinCos64 t = sincos64(x);
+    s = t.s; c = t.c; cm1 = 0.0;
+}
 __device__ inline void link_sincos(float x, float &s, float &c, float &cm1) {
     const SinCos32 t = sincos32(x);
     s = t.s; c = t.c; cm1 = t.cm1;
 }
-__device__ inline double scp_sin(double x) { return sin(x); }
+__device__ inline double scp_sin(double x) { return sincos64(x).s; }
 __device__ inline float scp_sin(float x) { return sincos32(x).s; }
 
 constexpr int TK = 64, TJ = 4;
