@@ -216,6 +216,7 @@ int main(int argc, char **argv) {
     scp_params p;
     memset(&p, 0, sizeof p);
     p.N = c.N; p.mode = mode; p.precision = c.step_precision ? SCP_PREC_F32C : SCP_PREC_F64;
+    p.lo_words = 4;   /* FP32C: int8 compensation words on the positions (the production form) */
     p.bc_type = c.bc_type; p.rmode = c.mode;
     p.gradient_margin = c.bc_type == 1 ? c.gradient_margin : 0;
     p.L = c.L; p.dx = dx; p.dt = dt;

@@ -11,7 +11,7 @@
 // function returns the site's charge density rho (C:2176-2181) for it.
 #pragma once
 
-#include "scp_kernels.h"
+#include "scp_physics.h"
 
 #include <hip/hip_runtime.h>
 
@@ -41,7 +41,10 @@ __device__ inline double diag_site(const KParams &kp, const DiagParams &dp, cons
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         cP[d] = 1; sP[d] = 0; cM[d] = 1; sM[d] = 0;
-        if (gauged) { sincos(xc[F_TH + d], &sP[d], &cP[d]); sincos(xn(F_TH + d, d, -1), &sM[d], &cM[d]); }
+        if (gauged) {   // this build's FP64 sincos, as in the step (scp_physics.h)
+            const SinCos64 p = sincos64(xc[F_TH + d]), m = sincos64(xn(F_TH + d, d, -1));
+            sP[d] = p.s; cP[d] = p.c; sM[d] = m.s; cM[d] = m.c;
+        }
     }
     // D of field pair (fu_, fv_) along d: (re, im)
     auto D = [&](int fu_, int fv_, int d, double &re, double &im) {
@@ -123,7 +126,7 @@ __device__ inline double diag_site(const KParams &kp, const DiagParams &dp, cons
             for (int p = 0; p < 3; p++) {
                 const int a = PA[p], b = PB[p];
                 const double ang = xc[F_TH + a] + xn(F_TH + b, a, 1) - xn(F_TH + a, b, 1) - xc[F_TH + b];
-                bb += 1.0 - cos(ang);
+                bb += 1.0 - sincos64(ang).c;
             }
             S[DS_EEM_E] += (ee + dp.BPRE * bb) * dV;
         }

@@ -105,6 +105,21 @@ def test_cli_restart_from_snapshot_is_exact(tmp_path):
         assert np.array_equal(f.read_frame(f.total_frames - 1), g.read_frame(g.total_frames - 1))
 
 
+def test_cli_fp32c_step(tmp_path):
+    """step_precision = 1 (a key of this build) runs the FP32C step in its production
+    form; the final frame stays within the FP32C bar of the reference's FP64 frame."""
+    fx = fixtures.load("gauged_n8")
+    z = np.load(os.path.join(fixtures.GOLDEN, "gauged_n8.npz"))
+    res, out, _ = run_cfg(tmp_path, fx, extra={"step_precision": 1}, seed_state=z["seed"])
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "Step:    FP32 compensated" in res.stdout
+    with sfa.SFAFile(out) as f:
+        fin = f.read_frame(f.total_frames - 1)
+    err = np.linalg.norm(fin - fx["final"]) / np.linalg.norm(fx["final"])
+    print(f"FP32C CLI run: field L2 {err:.2e} against the reference FP64 frame")
+    assert err <= 2e-5
+
+
 def test_cli_gauge_xform_selftest(tmp_path):
     fx = fixtures.load("gauged_g03_n8")
     z = np.load(os.path.join(fixtures.GOLDEN, "gauged_g03_n8.npz"))
