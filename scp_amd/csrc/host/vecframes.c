@@ -12,6 +12,7 @@
 
 int scp_vf_init(scp_vecframes *v, int N, int bs, int iframe_interval, double delta_tol) {
     memset(v, 0, sizeof *v);
+    if (bs < 1 || bs > N || iframe_interval < 1) return -1;
     const int bn = N / bs;
     v->bs = bs;
     v->n_patches = bn * bn * bn;

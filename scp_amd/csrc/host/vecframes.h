@@ -32,7 +32,8 @@ typedef struct {
     uint32_t *idx;                   /* P-frame patch indices */
 } scp_vecframes;
 
-/* N: grid edge; bs: block edge (vec_block_size); 0 or -1 on allocation failure. */
+/* N: grid edge; bs: block edge (vec_block_size, 1..N); iframe_interval >= 1;
+ * 0, or -1 on bad arguments or allocation failure. */
 int  scp_vf_init(scp_vecframes *v, int N, int bs, int iframe_interval, double delta_tol);
 /* Update the temporal model with v->coeffs at time t and write an I- or P-frame. */
 int  scp_vf_write(scp_vecframes *v, sfaio *out, double t);

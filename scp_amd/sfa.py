@@ -97,7 +97,7 @@ def write_sim_sfa(path, N, L, dt, frames, dtype=F64, codec=CODEC_RAW, kvmd=None,
     if vec:
         vf = VecFrames()
         if lib.scp_vf_init(C.byref(vf), N, vec[0], vec[1], vec[2]):
-            raise MemoryError("scp_vf_init")
+            raise ValueError("scp_vf_init: bad block size or I-frame interval, or out of memory")
     s = lib.sfaio_create(str(path).encode(), N, N, N, L, L, L, dt)
     if not s:
         raise OSError(f"cannot create {path}")
