@@ -45,7 +45,8 @@ def job_list(seed=9, n=6):
         N = int(rng.choice([16, 24, 33, 40]))
         L = 0.25 * (N - 1)
         prec = G.PREC_F32C if i % 3 else G.PREC_F64
-        p = G.make_params(N, G.MODE_GAUGED, L, 0.2, g=0.05, precision=prec, lo_words=1)
+        # FP32C jobs in the production storage form and in bfloat16 words (two batch classes)
+        p = G.make_params(N, G.MODE_GAUGED, L, 0.2, g=0.05, precision=prec, lo_words=4 if i % 2 else 1)
         jobs.append(dict(params=p, balls=balls_for(N, L, rng, 1 + i % 2), nsteps=int(rng.integers(5, 60)),
                          seed=100 + i, N=N))
     return jobs
