@@ -23,6 +23,7 @@
  *                      scp_sim.c:2461-2556)
  *   scp_grid_gauss_project  init_gauss_project (scp_sim.c:2284-2400) on the GPU
  *   scp_grid_rescale_phi    rescale_phi_kernel (scp_sim.cu:2619-2624)
+ *   scp_grid_set_kappa      the self-tune kappa update (scp_sim.cu:5520-5541)
  *   scp_grid_vec_fit   fit_patch over the six fields of a vector frame
  *                      (scp_sim.c:2567-2640, 3106-3117)
  *
@@ -133,8 +134,14 @@ int  scp_grid_vec_fit(scp_grid *g, int block_size, float *coeffs, int64_t n_coef
 int  scp_grid_gauss_project(scp_grid *g, int *iters, double *gauss_max);
 int  scp_grid_set_g_offset(scp_grid *g, double g_offset);
 double scp_grid_g_offset(const scp_grid *g);
-/* phi, phi_vel (and imaginary parts) *= f — the fixed-charge projection. */
+/* phi, phi_vel (and imaginary parts) *= f between steps — the fixed-charge
+ * projection of the self-tune loop (rescale_phi_kernel, scp_sim.cu:2619-2624,
+ * scp_sim.cu:5512-5517). As there, the next half kick uses the acceleration of
+ * the state before the scaling; the next force evaluation sees the scaled one. */
 int  scp_grid_rescale_phi(scp_grid *g, double f);
+/* kappa of the potential for the following steps — the self-tune update
+ * (cudaMemcpyToSymbol(d_KAPPA), scp_sim.cu:5531-5541); kappa > 0. */
+int  scp_grid_set_kappa(scp_grid *g, double kappa);
 /* Steps completed since upload. */
 int64_t scp_grid_steps_done(const scp_grid *g);
 

@@ -3,6 +3,7 @@
 TEST INFRASTRUCTURE ONLY (the checker); never imported by scp_amd.
 """
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -184,3 +185,53 @@ def vec_fit(fields32, N, bs):
                     out[p, f] = vec_fit_patch(fields32[f], N, (bi * bs, bj * bs, bk * bs), bs)
                 p += 1
     return out
+
+
+# ---------------------------------------------------------------------------
+# Self-tune (the reference's GPU binary only: scp_sim.cu:5475-5547, real mode).
+# No CPU counterpart runs it, so nothing executable pins it; the tests check the
+# closed forms (after a projection Q equals the target; kappa follows the SOC /
+# homeostasis rule given P_max) and run the GPU path against this restatement.
+# ---------------------------------------------------------------------------
+
+def self_tune_run(p, state, n_steps, st_every, m2, eps=0.04, gamma=0.12, pcrit=2.0, project=1,
+                  ptarget=0.0, gain=0.10, charge=0.0):
+    """n_steps of the step with the self-tune update every st_every steps. At an
+    update: diagnostics of the state, Q = 2 E_mass / m2, target Q0 set once,
+    phi and its velocity scaled by sqrt(Q0 / Q) keeping the stored acceleration,
+    kappa moved by the SOC rule (or towards P_max = ptarget). Returns the final
+    state and the list of (t, kappa, P_max, Q, status or err)."""
+    st = np.array(state, dtype=np.float64).reshape(ncols(p.mode), -1).copy()
+    acc, events, Q0, done = None, [], -1.0, 0
+    nphi = 3 if p.mode == 0 else 6
+    vcol = {0: 6, 1: 6}[p.mode]          # real: phi_v at 6..8; complex: phi_vx..z at 6..8, phiim_v at 18..20
+    while done < n_steps:
+        n = min(st_every, n_steps - done)
+        st, acc = step(p, st, n, acc=acc)
+        done += n
+        if n < st_every:
+            break
+        d = diag(p, st)
+        Pmax, Emass = d["P_max"], d["E_mass"]
+        Q = 2.0 * Emass / m2 if m2 > 1e-12 else Emass
+        if Q0 < 0:
+            Q0 = charge if charge > 0 else Q
+        if project and ptarget <= 0 and Q > 1e-12:
+            f = math.sqrt(Q0 / Q)
+            cols = list(range(3)) + list(range(vcol, vcol + 3))
+            if p.mode == 1:
+                cols += list(range(12, 15)) + list(range(18, 21))
+            st[cols] *= f
+        nan = (Pmax != Pmax) or Pmax > 1e30
+        if ptarget > 0:
+            err = 1.0 if nan else (Pmax - ptarget) / ptarget
+            err = max(-1.0, min(1.0, err))
+            p.kappa *= (1.0 + gain * err)
+            p.kappa = max(p.kappa, 1e-3)
+            events.append((done * p.dt, p.kappa, Pmax, Q, err))
+        else:
+            collapse = nan or Pmax > pcrit
+            p.kappa *= (1.0 + gamma) if collapse else (1.0 - eps)
+            p.kappa = max(p.kappa, 1e-3)
+            events.append((done * p.dt, p.kappa, Pmax, Q, "COLLAPSE" if collapse else "stable"))
+    return st, events

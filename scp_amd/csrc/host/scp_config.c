@@ -207,7 +207,6 @@ int scp_cfg_unsupported(const scp_config *c, char *err, int n) {
     else if (c->complex_gauge && c->higgs_v > 0) bad = "higgs_v";
     else if (c->complex_gauge && c->g_gauge2 != 0) bad = "g_gauge2";
     else if (c->n_locks > 0) bad = "n_locks";
-    else if (c->self_tune) bad = "self_tune";
     else if (c->sweep) bad = "sweep";
     else if (c->tune_dt != 0) bad = "tune_dt";
     else if (c->bc_type < 0 || c->bc_type > 2) bad = "bc_type";

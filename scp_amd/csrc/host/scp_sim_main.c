@@ -352,6 +352,15 @@ int main(int argc, char **argv) {
     } while (0)
     ROW(0.0);
 
+    /* self-tune (the reference's GPU binary, scp_sim.cu:5475-5547; real mode): kappa
+       follows the stability edge at fixed charge, phi rescaled to the target charge */
+    const int st_every = (c.self_tune && c.st_dt > 0) ? (int)lround(c.st_dt / dt) : 0;
+    double st_Q0 = -1.0;
+    if (st_every > 0)
+        printf("Self-tune: kappa dynamical, every %d steps (%.1f t.u.); "
+               "eps=%.3f gamma=%.3f pcrit=%.2f project=%d  kappa0=%.3f\n\n",
+               st_every, c.st_dt, c.st_eps, c.st_gamma, c.st_pcrit, c.st_project, c.kappa);
+
     double wall0 = omp_get_wtime();
     int major = diag_every * 25; if (major < 1) major = 1;
     if (major > n_steps / 10) { major = (n_steps / 10 / diag_every) * diag_every; if (major < diag_every) major = diag_every; }
@@ -369,15 +378,46 @@ int main(int argc, char **argv) {
         const int do_snap = in_burst || step % snap_every == 0;
         const int do_diag = step % diag_every == 0;
         const int do_vec = vec_on && step % vec_every == 0;
+        const int do_st = st_every > 0 && step % st_every == 0;
         pending++;
-        if (!do_snap && !do_diag && !do_vec && step != n_steps) continue;
-        /* a diag step takes the diagnostics fused into its last launch */
-        if (do_diag) {
+        if (!do_snap && !do_diag && !do_vec && !do_st && step != n_steps) continue;
+        /* a diag step takes the diagnostics fused into its last launch (unless a
+           self-tune rescale comes between the step and the diag row) */
+        if (do_diag && !do_st) {
             if (scp_grid_step_diag(x.g, pending, d, SCP_D_COUNT)) die_abi("step");
         } else if (scp_grid_step(x.g, pending)) {
             die_abi("step");
         }
         pending = 0;
+        if (do_st) {                                          /* scp_sim.cu:5502-5547 */
+            double sd[SCP_D_COUNT];
+            if (scp_grid_diag(x.g, sd, SCP_D_COUNT)) die_abi("diag");
+            const double Pmax = sd[SCP_D_P_MAX], Emass = sd[SCP_D_E_MASS];
+            const double Q = (c.m2 > 1e-12) ? 2.0 * Emass / c.m2 : Emass;
+            if (st_Q0 < 0) st_Q0 = (c.st_charge > 0 ? c.st_charge : Q);   /* the target, set once */
+            if (c.st_project && c.st_ptarget <= 0 && Q > 1e-12)             /* charge projection */
+                if (scp_grid_rescale_phi(x.g, sqrt(st_Q0 / Q))) die_abi("rescale");
+            const int nanflag = (Pmax != Pmax) || (Pmax > 1e30);
+            if (c.st_ptarget > 0.0) {                                       /* density homeostasis */
+                double err = nanflag ? 1.0 : (Pmax - c.st_ptarget) / c.st_ptarget;
+                if (err > 1.0) err = 1.0;
+                if (err < -1.0) err = -1.0;
+                c.kappa *= (1.0 + c.st_gain * err);
+                if (c.kappa < 1e-3) c.kappa = 1e-3;
+                if (scp_grid_set_kappa(x.g, c.kappa)) die_abi("kappa");
+                printf("[self-tune] t=%6.1f kappa=%9.4f Pmax=%10.3e target=%.4f err=%+.3f Q=%9.3e\n",
+                       t, c.kappa, Pmax, c.st_ptarget, err, Q);
+            } else {                                                         /* SOC update */
+                const int collapse = nanflag || (Pmax > c.st_pcrit);
+                c.kappa *= collapse ? (1.0 + c.st_gamma) : (1.0 - c.st_eps);
+                if (c.kappa < 1e-3) c.kappa = 1e-3;
+                if (scp_grid_set_kappa(x.g, c.kappa)) die_abi("kappa");
+                printf("[self-tune] t=%6.1f kappa=%9.4f Pmax=%10.3e Q=%9.3e %s\n", t, c.kappa, Pmax, Q,
+                       collapse ? "COLLAPSE" : "stable");
+            }
+            fflush(stdout);
+            if (do_diag && scp_grid_diag(x.g, d, SCP_D_COUNT)) die_abi("diag");
+        }
         if (do_snap) snap(&x, t);
         if (do_vec) {                                         /* scp_sim.c:3098-3193 */
             if (scp_grid_vec_fit(x.g, vf.bs, vf.coeffs, (int64_t)vf.n_patches * vf.nc)) die_abi("vector fit");

@@ -307,14 +307,15 @@ static int launch_kind(scp_grid *g, int kind, bool emit) {
     return kerr(launch_step_f64(g->step_mode, kind, emit, g->kp, bufs_for(g, kind), g->stream), "step64");
 }
 
-// initial force + first half kick + first drift, from xo/vo
-int scp_internal::run_init(scp_grid *g) {
+// initial force + first half kick + first drift, from xo/vo; pin: bc 1 saves the
+// state as the pinned state (an initial state; not after a rescale between steps)
+int scp_internal::run_init(scp_grid *g, bool pin) {
     int rc = run_aux(g, g->xo, g->vo);
     if (rc) return rc;
     rc = launch_kind(g, K_INIT, false);
     if (rc) return rc;
     g->cur = 0;
-    if (g->pinx) {   // bc 1: the state at init is the pinned state (grid_save_pinned, C:381-400, 2845-2849)
+    if (g->pinx && pin) {   // bc 1: the state at init is the pinned state (grid_save_pinned, C:381-400, 2845-2849)
         const size_t nb = (size_t)g->NP * g->kp.fs * sizeof(double);
         HIPCHK(hipMemcpyAsync(g->pinx, g->xo, nb, hipMemcpyDeviceToDevice, g->stream));
         HIPCHK(hipMemcpyAsync(g->pinv, g->vo, nb, hipMemcpyDeviceToDevice, g->stream));
@@ -632,17 +633,39 @@ extern "C" int scp_grid_vec_fit(scp_grid *g, int block_size, float *coeffs, int6
 
 extern "C" int scp_grid_rescale_phi(scp_grid *g, double f) {
     if (!g) return set_err(-1, "null grid");
+    // Between steps the reference scales phi and its velocity in place and keeps
+    // the stored acceleration a_n = F(x_n) of the unscaled state: the next half
+    // kick uses it, the next force evaluation sees the scaled positions
+    // (rescale_phi_kernel, scp_sim.cu:2619-2624, called at scp_sim.cu:5512-5517).
+    // a_n is recomputed from x_n before the scaling (the forces kernel is the
+    // step's own, so it is the step's a_n; with sigma_freq the theta frequency
+    // is then taken from the completed velocities).
+    const size_t nacc = (size_t)g->NP * g->kp.fs * sizeof(double);
+    if (!g->acc) HIPCHK(hipMalloc(&g->acc, nacc));
+    int rc = run_aux(g, g->xo, g->vo);
+    if (!rc) rc = launch_kind(g, K_FORCES, false);
+    if (rc) return rc;
     const int nphi = g->mode == 0 ? 3 : 6;    // u (and v)
     for (int q = 0; q < nphi; q++) {
         int fld = q < 3 ? F_U + q : F_V + (q - 3);
-        int rc = kerr(launch_scale(g->xo + (size_t)fld * g->kp.fs, f, g->kp.n3, g->stream), "scale");
+        rc = kerr(launch_scale(g->xo + (size_t)fld * g->kp.fs, f, g->kp.n3, g->stream), "scale");
         if (rc) return rc;
         rc = kerr(launch_scale(g->vo + (size_t)fld * g->kp.fs, f, g->kp.n3, g->stream), "scale");
         if (rc) return rc;
     }
-    int rc = run_init(g);
+    g->kp.init_acc = 1;
+    rc = run_init(g, false);
+    g->kp.init_acc = 0;
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+extern "C" int scp_grid_set_kappa(scp_grid *g, double kappa) {
+    if (!g) return set_err(-1, "null grid");
+    if (!(kappa > 0)) return set_err(-1, "kappa=%g must be > 0", kappa);
+    g->kp.KAPPA = kappa;
+    g->p.kappa = kappa;
     return 0;
 }
 

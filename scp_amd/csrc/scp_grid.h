@@ -58,7 +58,7 @@ bool use_march(const scp_grid *g, int kind);
 scp::Bufs<double> bufs_for(scp_grid *g, int kind);
 scp::Bufs32 bufs32_for(scp_grid *g, int kind);
 // initial force + first half kick + drift from xo/vo (upload and device seeding)
-int run_init(scp_grid *g);
+int run_init(scp_grid *g, bool pin = true);
 }  // namespace scp_internal
 
 // return the HIP error of x as status -2 with a message

@@ -42,7 +42,7 @@ struct KParams {
     uint32_t plane;            // N * pitch
     uint32_t fsh;              // field stride of P/W: N * plane rounded up to 64
     int32_t gm;                // bc 1: gradient_margin
-    int32_t pad;
+    int32_t init_acc;          // K_INIT: the first half kick takes acc (a given a_n), not F(x)
 };
 
 // Device buffers. Each group is one allocation of nfields * fs elements.

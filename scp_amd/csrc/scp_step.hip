@@ -112,6 +112,7 @@ k_step64(KParams kp, Bufs<double> b, int DI) {
                 double x = c0[f];
                 if constexpr (KIND == K_INIT) {
                     v = b.vo[oc];
+                    if (kp.init_acc) a = b.acc[oc];        // a_n kept across a rescale of x_n, v_n
                 } else {
                     v = __builtin_nontemporal_load(&b.w[o]);
                     v += kp.hdt * a;                       // stage 4 (C:1680-1696)

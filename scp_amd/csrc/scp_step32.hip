@@ -124,7 +124,8 @@ k_step32(KParams kp, Bufs32 b, int DI) {
 #pragma unroll
             for (int f = 0; f < NP; f++) {
                 const size_t o = (size_t)f * fh + chh, oc = (size_t)f * fs + cc;
-                double w = b.vo[oc] + kp.hdt * (double)acc[f];
+                const double a = kp.init_acc ? b.acc[oc] : (double)acc[f];   // a given a_n, or F(x)
+                double w = b.vo[oc] + kp.hdt * a;
                 double x = b.xo[oc], xn;
                 if (MODE == 2 && f >= F_TH) {
                     double t = x + kp.gad * w;

@@ -512,3 +512,43 @@ def _compare_vec_frames(path, z, first_frame):
                 if len(v["indices"]):
                     d = z[pre + "deltas"]
                     assert np.abs(v["deltas"] - d).max() <= 1e-5 * max(np.abs(d).max(), 1e-3)
+
+
+# ---------------------------------------------------------------------------
+# self-tune restatement (scp_sim.cu:5475-5547)
+# ---------------------------------------------------------------------------
+
+def _st_state(N, L, seed):
+    rng = np.random.default_rng(seed)
+    dx = 2 * L / (N - 1)
+    x = -L + dx * np.arange(N)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    st = 0.05 * rng.standard_normal((12, N ** 3))
+    st[0] += 0.8 * np.exp(-(X ** 2 + Y ** 2 + Z ** 2) / 4.0).ravel()
+    return st
+
+
+@pytest.mark.parametrize("ptarget", [0.0, 0.01])
+def test_self_tune_restatement_closed_forms(ptarget):
+    """The charge after a projection is the target Q0 (the first update's Q), and
+    kappa moves by the stated factor at every update."""
+    N = 12
+    L = 0.25 * (N - 1)
+    p = oracle.make_params(N, 0, L, 0.1)
+    k0, every, pcrit, eps, gamma, gain = p.kappa, 6, 0.003, 0.04, 0.12, 0.10
+    fin, ev = oracle.self_tune_run(p, _st_state(N, L, 7), 36, every, p.m2, pcrit=pcrit, ptarget=ptarget)
+    assert len(ev) == 6
+    kappa = k0
+    for (t, k, Pmax, Q, tag) in ev:
+        if ptarget > 0:
+            err = max(-1.0, min(1.0, (Pmax - ptarget) / ptarget))
+            assert tag == err
+            kappa *= 1.0 + gain * err
+        else:
+            assert tag == ("COLLAPSE" if Pmax > pcrit else "stable")
+            kappa *= (1.0 + gamma) if Pmax > pcrit else (1.0 - eps)
+        assert k == max(kappa, 1e-3)
+    if ptarget <= 0:
+        assert {e[4] for e in ev} == {"COLLAPSE", "stable"}
+        d = oracle.diag(p, fin)
+        assert abs(2 * d["E_mass"] / p.m2 - ev[0][3]) <= 1e-12 * ev[0][3]
