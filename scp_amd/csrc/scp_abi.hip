@@ -21,13 +21,15 @@ using namespace scp;
 namespace scp_internal {
 
 
-// K_STEP of the complex and gauged steps runs the LDS-staged march unless the
-// caller asked for the direct-neighbour kernel; the march addresses a field
-// with 32-bit byte offsets, so one field must stay below 4 GiB (N <= ~800 in
-// FP64, N <= 1023 in FP32C); larger grids run the direct-neighbour kernel.
+// K_STEP runs the LDS-staged march unless the caller asked for the
+// direct-neighbour kernel. Real-mode grids with the pass-1 fields (alpha_cs,
+// beta_h, sigma_freq: g->aux, read at neighbours) and bc 1 (pinned slabs) run
+// the direct kernel. The march addresses a field with 32-bit byte offsets, so
+// one field must stay below 4 GiB (N <= ~800 in FP64, N <= 1023 in FP32C);
+// larger grids run the direct-neighbour kernel.
 bool use_march(const scp_grid *g, int kind) {
     const uint64_t field_bytes = (uint64_t)g->kp.fsh * (g->f32 ? 4u : 8u);
-    return kind == K_STEP && g->p.step_kernel == 0 && (g->step_mode == 1 || g->step_mode == 2) &&
+    return kind == K_STEP && g->p.step_kernel == 0 && (g->step_mode != 0 || !g->aux) &&
            g->p.bc_type != 1 && field_bytes < (1ull << 32);
 }
 
@@ -555,13 +557,13 @@ extern "C" int scp_grid_diag(scp_grid *g, double *out, int nout) {
 // launch: the march accumulates every pass-1 sum and maximum of the emitted
 // state per block (and the gauged charge density), a light pass adds the
 // Gauss-law statistics, which need E at neighbours other blocks compute. Grids
-// whose step does not run the march (real mode, direct-neighbour kernel,
-// gauged storage with g = 0) take the step and the stand-alone pass.
+// whose step does not run the march (direct-neighbour kernel, gauged storage
+// with g = 0) and real-mode grids take the step and the stand-alone pass.
 extern "C" int scp_grid_step_diag(scp_grid *g, int nsteps, double *out, int nout) {
     if (!g || !out) return set_err(-1, "null argument");
     if (nsteps < 1) return set_err(-1, "nsteps=%d must be >= 1", nsteps);
     if (nout < SCP_D_COUNT) return set_err(-1, "nout=%d < %d", nout, SCP_D_COUNT);
-    if (!use_march(g, K_STEP) || g->step_mode != g->mode) {
+    if (!use_march(g, K_STEP) || g->step_mode != g->mode || g->mode == 0) {
         int rc = scp_grid_step(g, nsteps);
         return rc ? rc : scp_grid_diag(g, out, nout);
     }

@@ -318,7 +318,7 @@ struct S32 {
 // ---------------------------------------------------------------------------
 
 template <int MODE>
-constexpr int march_nf() { return MODE == 2 ? 15 : 12; }   // staged fields (matter + links)
+constexpr int march_nf() { return MODE == 2 ? 15 : (MODE == 1 ? 12 : 6); }   // staged fields (matter + links)
 template <class S, int KW, int MODE>
 constexpr int march_lds_elems() { return MNBUF * march_nf<MODE>() * MG<KW>::MPL; }
 
@@ -339,7 +339,7 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
                                            const DiagArgs *da = nullptr, DiagState *ds = nullptr) {
     using T = typename S::T;
     constexpr int NF = march_nf<MODE>();
-    constexpr int NM = 12;                    // matter fields (the i-1 plane is matter only)
+    constexpr int NM = MODE == 2 ? 12 : NF;   // fields of the i-1 plane (gauged: matter only)
     constexpr int MPL = MG<KW>::MPL;
     constexpr int RQ = Halo<NF, KW>::RQ;
 
@@ -476,8 +476,10 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
             T acc[NF];
             if constexpr (MODE == 2)
                 forces_gauge(kp, c0, c1, cm, nb, cy, acc);
-            else
+            else if constexpr (MODE == 1)
                 forces_complex(kp, c0, c1, cm, nb, acc);
+            else   // real mode without the pass-1 fields (the caller's condition): AUX is never read
+                forces_real(kp, c0, c1, cm, nb, [](int, uint32_t) { return T(0); }, 0u, 0u, 0u, 0u, 0u, 0u, 0u, acc);
             double dmp = 1.0;
             double xe[NF], ve[NF];   // the emitted state of this site (used by DIAG)
             typename S::Lo lo{};     // packed compensation words out

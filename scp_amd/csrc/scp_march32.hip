@@ -10,8 +10,9 @@ namespace scp {
 
 template <int LOK, int KW>
 static int launch_lo(int mode, bool emit, const KParams &kp, const Bufs32 &b, int DI, hipStream_t s) {
-    return mode == 2 ? launch_march_t<S32<LOK>, KW, 2>(emit, kp, b, DI, s)
-                     : launch_march_t<S32<LOK>, KW, 1>(emit, kp, b, DI, s);
+    return mode == 2   ? launch_march_t<S32<LOK>, KW, 2>(emit, kp, b, DI, s)
+           : mode == 1 ? launch_march_t<S32<LOK>, KW, 1>(emit, kp, b, DI, s)
+                       : launch_march_t<S32<LOK>, KW, 0>(emit, kp, b, DI, s);
 }
 
 template <int KW>
@@ -25,7 +26,7 @@ static int launch_kw(int mode, bool emit, int lo_kind, const KParams &kp, const 
 
 int launch_march_f32c_kw(int kw, int mode, bool emit, int lo_kind, const KParams &kp, const Bufs32 &b, int DI,
                          hipStream_t s) {
-    if (mode != 1 && mode != 2) return (int)hipErrorInvalidValue;
+    if (mode < 0 || mode > 2) return (int)hipErrorInvalidValue;
     return kw == 1 ? launch_kw<1>(mode, emit, lo_kind, kp, b, DI, s) : launch_kw<2>(mode, emit, lo_kind, kp, b, DI, s);
 }
 
@@ -53,13 +54,14 @@ int launch_march_f32c_diag(int mode, int lo_kind, const KParams &kp, const Bufs3
 template <int LOK>
 static int batch_lo(int mode, const BatchGrid<Bufs32> *grids, const BatchTile *tiles, uint32_t ntiles,
                     uint32_t nblocks, uint32_t per_block, int q, hipStream_t s) {
-    return mode == 2 ? launch_march_batch_t<S32<LOK>, 2>(grids, tiles, ntiles, nblocks, per_block, q, s)
-                     : launch_march_batch_t<S32<LOK>, 1>(grids, tiles, ntiles, nblocks, per_block, q, s);
+    return mode == 2   ? launch_march_batch_t<S32<LOK>, 2>(grids, tiles, ntiles, nblocks, per_block, q, s)
+           : mode == 1 ? launch_march_batch_t<S32<LOK>, 1>(grids, tiles, ntiles, nblocks, per_block, q, s)
+                       : launch_march_batch_t<S32<LOK>, 0>(grids, tiles, ntiles, nblocks, per_block, q, s);
 }
 
 int launch_march_batch_f32c(int mode, int lo_kind, const BatchGrid<Bufs32> *grids, const BatchTile *tiles,
                             uint32_t ntiles, uint32_t nblocks, uint32_t per_block, int q, hipStream_t s) {
-    if (mode != 1 && mode != 2) return (int)hipErrorInvalidValue;
+    if (mode < 0 || mode > 2) return (int)hipErrorInvalidValue;
     if (lo_kind == LO_F32) return batch_lo<LO_F32>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
     if (lo_kind == LO_BF16) return batch_lo<LO_BF16>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
     if (lo_kind == LO_Q8) return batch_lo<LO_Q8>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);

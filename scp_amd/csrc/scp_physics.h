@@ -242,14 +242,15 @@ __device__ inline void forces_complex(const KParams &kp, const CV &c, const PV &
     }
 }
 
-// Real-mode accelerations (compute_forces, C:743-922). AUX(f, idx) loads the
+// Real-mode accelerations (compute_forces, C:743-922). c/ip/im: own column at
+// planes i, i+1, i-1; nb: in-plane neighbours. AUX(f, idx) loads the
 // pass-1 fields M[0..2] (f=0..2) and Q[0..2] (f=3..5) when alpha_cs/beta_h != 0,
 // and the local theta frequency (f=6) when sigma_freq != 0. With kp.XT the
 // terms of the reference's GPU step follow (compute_forces_kernel,
 // scp_sim.cu:1144-1248), added in its order onto the base accelerations.
-template <typename R, class LD, class LA>
-__device__ inline void forces_real(const KParams &kp, const R *c, const R *ip,
-                                   const R *im, LD X, LA AUX, uint32_t ci, uint32_t ipx,
+template <typename R, class CV, class PV, class MV, class NB, class LA>
+__device__ inline void forces_real(const KParams &kp, const CV &c, const PV &ip,
+                                   const MV &im, const NB &nb, LA AUX, uint32_t ci, uint32_t ipx,
                                    uint32_t imx, uint32_t jp, uint32_t jm, uint32_t kp_,
                                    uint32_t km, R *acc) {
     const R h1 = R(kp.h1), h2 = R(kp.h2);
@@ -258,8 +259,7 @@ __device__ inline void forces_real(const KParams &kp, const R *c, const R *ip,
     const int MODE = kp.rmode;
     auto nbv = [&](int f, int d, int sg) -> R {
         if (d == 0) return sg > 0 ? ip[f] : im[f];
-        if (d == 1) return X(f, sg > 0 ? jp : jm);
-        return X(f, sg > 0 ? kp_ : km);
+        return nb.face(f, d, sg);
     };
     auto nbi = [&](int d, int sg) -> uint32_t {
         if (d == 0) return sg > 0 ? ipx : imx;

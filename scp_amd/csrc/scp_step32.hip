@@ -111,7 +111,7 @@ k_step32(KParams kp, Bufs32 b, int DI) {
         else if constexpr (MODE == 1)
             forces_complex(kp, c0, cp, cm, nb, acc);
         else
-            forces_real(kp, c0, cp, cm, X, AUX, ci, IX(ip1, j, k), IX(im1, j, k), jp, jm,
+            forces_real(kp, c0, cp, cm, nb, AUX, ci, IX(ip1, j, k), IX(im1, j, k), jp, jm,
                         kpx, kmx, acc);
 
         if constexpr (KIND == K_FORCES) {

@@ -86,16 +86,37 @@ def test_batch_autotune_keeps_results():
         h.close()
 
 
+@pytest.mark.parametrize("prec,lo", [(G.PREC_F64, 0), (G.PREC_F32C, 4)])
+def test_real_batch_equals_each_grid_alone(prec, lo):
+    def mk(i, N):
+        L, st = ball(N, 400 + i)
+        g = G.Grid(G.make_params(N, G.MODE_REAL, L, 0.2, precision=prec, lo_words=lo))
+        g.upload(st[:12].copy())
+        return g
+    print(f"case real prec={prec} lo={lo} sizes={SIZES}", flush=True)
+    alone = [mk(i, N) for i, N in enumerate(SIZES)]
+    mixed = [mk(i, N) for i, N in enumerate(SIZES)]
+    with G.Batch(mixed) as b:
+        for rep in range(2):
+            for g in alone:
+                g.step(5)
+            b.step(5)
+            for g, h in zip(alone, mixed):
+                assert np.array_equal(g.download(), h.download()), (rep, g.params.N)
+    for g in alone + mixed:
+        g.close()
+
+
 def test_batch_membership_rules():
     a = make(16, 1, G.PREC_F32C, 1)
     b64 = make(16, 2, G.PREC_F64, 0)
     L = 0.25 * 15
-    real = G.Grid(G.make_params(16, G.MODE_REAL, L, 0.2))
+    real = G.Grid(G.make_params(16, G.MODE_REAL, L, 0.2, alpha_cs=0.1))
     with G.Batch([a]) as b:
         with pytest.raises(G.ScpError):
             b.add(b64)                 # other class
-        with pytest.raises(G.ScpError):
-            b.add(real)                # the real step is not batched
+        with pytest.raises(G.ScpError, match="cannot join"):
+            b.add(real)                # real mode with the pass-1 fields: direct kernel only
         with pytest.raises(G.ScpError):
             b.add(a)                   # already a member
         b.remove(a)

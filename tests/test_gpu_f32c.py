@@ -108,6 +108,26 @@ def test_f32c_march_and_direct_kernels_agree(lo):
     assert np.linalg.norm(outs[0] - outs[1]) / np.linalg.norm(a) <= 2e-5
 
 
+@pytest.mark.parametrize("lo", [1, 4])
+def test_f32c_real_march_and_direct_kernels_agree(lo):
+    """Real mode, FP32C: the march and the direct-neighbour kernel after 20
+    steps of a 24^3 state stay within the FP32C bar of FP64 and of each other."""
+    N = 24
+    L = 0.25 * (N - 1)
+    st = ball_state(N, L)[:12].copy()
+    a, _, d64 = run(G.make_params(N, G.MODE_REAL, L, 0.2), st, 20, project=False)
+    outs = []
+    for sk in (0, 1):
+        p = G.make_params(N, G.MODE_REAL, L, 0.2, precision=G.PREC_F32C, lo_words=lo, step_kernel=sk)
+        with G.Grid(p) as g:
+            assert g.step_kernel_name.startswith("k_march" if sk == 0 else "k_step32"), g.step_kernel_name
+        b, _, d = run(p, st, 20, project=False)
+        outs.append(b)
+        assert np.linalg.norm(b - a) / np.linalg.norm(a) <= 2e-5, sk
+        assert abs(d["E_total"] - d64["E_total"]) <= 1e-6 * abs(d64["E_total"]), sk
+    assert np.linalg.norm(outs[0] - outs[1]) / np.linalg.norm(a) <= 2e-5
+
+
 def q_halves(st, N):
     """Noether charge of Phi in the x < 0 and x >= 0 halves (FP64, host)."""
     q = sum(st[a] * st[18 + a] - st[12 + a] * st[6 + a] for a in range(3)).reshape(N, N, N)

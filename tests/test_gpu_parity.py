@@ -252,6 +252,51 @@ def test_march_kernel_equals_direct_kernel(mode, N):
     assert np.array_equal(outs[0], outs[1])
 
 
+REAL_MARCH_CASES = {
+    "base": {},
+    "rmode1": dict(rmode=1),
+    "rmode3": dict(rmode=3),
+    "kappa_h": dict(kappa_h=0.3),
+    "gpu_terms": dict(theta_sat=0.05, gamma_conv=0.8, sigma_grad=0.4, chi_chiral=1.5, sigma_cubic=0.7,
+                      sigma_cross=0.5, lambda_self=0.9),
+    "periodic": dict(bc_type=0),
+}
+
+
+@pytest.mark.parametrize("N", [10, 70])
+@pytest.mark.parametrize("case", list(REAL_MARCH_CASES))
+def test_real_march_equals_direct_kernel(case, N):
+    """Real mode on the LDS-staged march (no alpha_cs / beta_h / sigma_freq
+    fields) against the direct-neighbour kernel: bitwise equal in FP64 for every
+    real-mode variant the march runs, also on ragged tiles (N = 70)."""
+    L = 0.25 * (N - 1)
+    st = seeded(N, 1, L)[:12].copy()
+    rng = np.random.default_rng(N)
+    st[3:6] += 0.1 * rng.standard_normal((3, N ** 3))   # theta large enough for the saturation term
+    kw = REAL_MARCH_CASES[case]
+    print(f"case {case} N={N}", flush=True)
+    outs, names = [], []
+    for sk in (0, 1):
+        with G.Grid(G.make_params(N, 0, L, 0.1, step_kernel=sk, **kw)) as g:
+            names.append(g.step_kernel_name)
+            g.upload(st)
+            g.step(9)
+            g.step(11)
+            outs.append(g.download())
+    assert names[0].startswith("k_march<scp::S64, 1, 0") and names[1].startswith("k_step64<0"), names
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_real_grids_with_pass1_fields_run_the_direct_kernel():
+    """alpha_cs, beta_h and sigma_freq read the pass-1 fields at neighbours: such
+    real grids keep the direct-neighbour kernel."""
+    N = 12
+    L = 0.25 * (N - 1)
+    for kw in (dict(alpha_cs=0.1), dict(beta_h=0.2), dict(sigma_freq=0.3)):
+        with G.Grid(G.make_params(N, 0, L, 0.1, **kw)) as g:
+            assert g.step_kernel_name.startswith("k_step64<0"), kw
+
+
 GPU_TERMS = {
     "theta_sat": dict(theta_sat=0.05, gamma_conv=0.8),
     "sigma_grad": dict(sigma_grad=0.4, chi_chiral=1.5),
