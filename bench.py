@@ -23,8 +23,13 @@ from the counter summary profiles/hbm_traffic.txt when it holds a line for
 this kernel and size. cpu_baseline: the reference CPU binary built from source
 (oracle/_ref/scp_sim) on a bounded 64^3 gauged sample on this host.
 
+--mode real runs the same measurement on the real-mode step (6 fields: 12
+state scalars, 192 B FP64 / 96 B FP32C per site-update; the state is the real
+part of the same 64 seeded balls, theta noise 0.01) with the reference
+binary's real-mode 64^3 run as the CPU baseline. It is not the headline line.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--prec f64|f32c]
-                       [--lo-words 0|1|2|3|4] [--no-compare] [--no-cpu-baseline]
+                       [--lo-words 0|1|2|3|4] [--mode gauged|real] [--no-compare] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -45,19 +50,24 @@ HBM_PEAK_GBS = 8000.0
 LO_NAMES = {0: "f32", 1: "bf16", 2: "no", 3: "int8", 4: "int8 position"}
 
 
-def bytes_per_site(prec):
-    """Algorithmic: 30 gauged state scalars, each read once and written once per
-    step at the state's width (compensation words are not state)."""
-    return 2 * 30 * (8 if prec == "f64" else 4)
+NFIELDS = {"gauged": 15, "real": 6}   # fields per site (positions; as many velocities)
 
 
-def stored_bytes_per_site(prec, lo_words):
+def bytes_per_site(prec, mode="gauged"):
+    """Algorithmic: the state scalars (gauged 30, real 12), each read once and
+    written once per step at the state's width (compensation words are not state)."""
+    return 2 * 2 * NFIELDS[mode] * (8 if prec == "f64" else 4)
+
+
+def stored_bytes_per_site(prec, lo_words, mode="gauged"):
     """What the step streams: the state words plus the compensation words."""
+    nf = NFIELDS[mode]
     if prec == "f64":
-        return 2 * 30 * 8
-    # 15 position and 15 velocity scalars; int8 words packed four fields to a word
-    lo_p, lo_w = {0: (4, 4), 1: (2, 2), 2: (0, 0), 3: (16 / 15, 16 / 15), 4: (16 / 15, 0)}[lo_words]
-    return 2 * 15 * (8 + lo_p + lo_w)
+        return 2 * 2 * nf * 8
+    # nf position and nf velocity scalars; int8 words packed four fields to a 4-byte word
+    q8 = 4 * ((nf + 3) // 4) / nf
+    lo_p, lo_w = {0: (4, 4), 1: (2, 2), 2: (0, 0), 3: (q8, q8), 4: (q8, 0)}[lo_words]
+    return 2 * nf * (8 + lo_p + lo_w)
 
 
 def measured_traffic(kernel, N):
@@ -78,21 +88,22 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(seconds_hint=15.0):
+def cpu_baseline(mode="gauged"):
     """Time the reference CPU path (oracle/_ref/scp_sim, built from the reference
-    sources with its Makefile flags -O3 -fopenmp) on a 64^3 gauged sample."""
+    sources with its Makefile flags -O3 -fopenmp) on a 64^3 sample of the mode."""
     ref = os.path.join(ROOT, "oracle", "_ref", "scp_sim")
     if not os.path.exists(ref):
         return None
     cores = min(16, os.cpu_count() or 1)
     N, L, dtf = 64, 15.75, 0.2
     dt = dtf * (2 * L / (N - 1))
-    nsteps = 600
+    nsteps = 600 if mode == "gauged" else 1500
+    cfields = "complex_phi = 1\ncomplex_gauge = 1\ng_gauge = 0.05\n" if mode == "gauged" else ""
     with tempfile.TemporaryDirectory() as wd:
         cfg = os.path.join(wd, "b.cfg")
         with open(cfg, "w") as f:
-            f.write(f"N = {N}\nL = {L}\nT = {nsteps * dt}\ndt_factor = {dtf}\ncomplex_phi = 1\n"
-                    f"complex_gauge = 1\ng_gauge = 0.05\nbc_type = 2\ninit = oscillon\n"
+            f.write(f"N = {N}\nL = {L}\nT = {nsteps * dt}\ndt_factor = {dtf}\n{cfields}"
+                    f"bc_type = 2\ninit = oscillon\n"
                     f"precision = f32\nsnap_dt = {10 * nsteps * dt}\ndiag_dt = {nsteps * dt}\n"
                     f"output = {wd}/o.sfa\ndiag_file = {wd}/d.tsv\n")
         env = dict(os.environ, OMP_NUM_THREADS=str(cores))
@@ -106,7 +117,7 @@ def cpu_baseline(seconds_hint=15.0):
     ms = float(m.group(1))
     return {"value": N ** 3 / (ms * 1e-3) / 1e9, "unit": "Gsite-updates/s", "cores": cores,
             "kind": "reference",
-            "sample": f"reference scp_sim (gcc -O3 -fopenmp), gauged 64^3 periodic, init=oscillon, "
+            "sample": f"reference scp_sim (gcc -O3 -fopenmp), {mode} 64^3 bc_type=2, init=oscillon, "
                       f"{nsteps} steps, OMP_NUM_THREADS={cores}, 'Wall' ms/step={ms:.2f}, "
                       f"process wall {wall:.1f}s"}
 
@@ -128,17 +139,22 @@ def copy_calibration(gib, reps=3):
                       "GBps": 2 * n * 4 / dt / 1e9}), flush=True)
 
 
-def build_state(N, L, seed=1002):
+REAL_COLS = [0, 1, 2, 3, 4, 5, 12, 13, 14, 15, 16, 17]   # complex columns: Re Phi, Re Theta and their velocities
+
+
+def build_state(N, L, mode="gauged", seed=1002):
     from scp_amd import seeds
     balls = seeds.draw_balls(N, L, 64, seed)
-    return seeds.seed_balls(N, L, 2, balls, theta_sigma=0.01, link_sigma=0.0, seed=seed)
+    if mode == "gauged":
+        return seeds.seed_balls(N, L, 2, balls, theta_sigma=0.01, link_sigma=0.0, seed=seed)
+    return np.ascontiguousarray(seeds.seed_balls(N, L, 1, balls, theta_sigma=0.01, seed=seed)[REAL_COLS])
 
 
-def run_one(G, N, L, st, prec, lo_words, steps, warmup):
+def run_one(G, N, L, st, prec, lo_words, steps, warmup, mode="gauged"):
     """Upload the state, warm up, time `steps` launches with HIP events on the
     grid's stream; returns the timing and the roofline figures."""
-    params = G.make_params(N, G.MODE_GAUGED, L, 0.2, g=0.05, lo_words=lo_words,
-                           precision=G.PREC_F64 if prec == "f64" else G.PREC_F32C)
+    params = G.make_params(N, G.MODE_GAUGED if mode == "gauged" else G.MODE_REAL, L, 0.2, g=0.05,
+                           lo_words=lo_words, precision=G.PREC_F64 if prec == "f64" else G.PREC_F32C)
     log(f"{prec} lo_words={lo_words}: device bytes {G.grid_bytes(params) / 1e9:.1f} GB")
     with G.Grid(params) as g:
         g.upload(st)
@@ -152,16 +168,16 @@ def run_one(G, N, L, st, prec, lo_words, steps, warmup):
         diag_ms = diag_step_timing(g)
     ms_step = ms / steps
     sites = float(N) ** 3
-    bps = bytes_per_site(prec)
-    achieved = bps * sites / (ms_step * 1e-3) / 1e9
-    stored = stored_bytes_per_site(prec, lo_words) * sites / (ms_step * 1e-3) / 1e9
     log(f"{prec}: {steps} steps {ms:.2f} ms (wall {wall * 1e3:.1f} ms) -> {ms_step:.3f} ms/step; "
         f"E_total={d['E_total']:.9e} Q_total={d['Q_total']:.6e}")
+    bps, sbps = bytes_per_site(prec, mode), stored_bytes_per_site(prec, lo_words, mode)
+    achieved = bps * sites / (ms_step * 1e-3) / 1e9
+    stored = sbps * sites / (ms_step * 1e-3) / 1e9
     log(f"{prec}: diag step (wall, ms): step alone {diag_ms['step']:.2f}, step + stand-alone diagnostics "
         f"{diag_ms['separate']:.2f}, diagnostics fused into the step {diag_ms['fused']:.2f}")
     return {"ms_per_step": ms_step, "value": sites / (ms_step * 1e-3) / 1e9, "kernel": kernel,
             "bytes_per_site": bps, "achieved": achieved, "traffic": measured_traffic(kernel, N),
-            "stored_bytes_per_site": stored_bytes_per_site(prec, lo_words), "stored_GBps": stored,
+            "stored_bytes_per_site": sbps, "stored_GBps": stored,
             "diag_step_ms": diag_ms}
 
 
@@ -342,6 +358,8 @@ def main():
     ap.add_argument("--lo-words", type=int, default=4,
                     help="FP32C compensation words: 0 FP32, 1 bfloat16, 2 none (plain FP32), "
                          "3 signed byte in units of ulp/256, 4 such bytes on the positions only (default)")
+    ap.add_argument("--mode", default="gauged", choices=["gauged", "real"],
+                    help="step workload: gauged (the headline) or real (6-field real-mode step)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="step", choices=["step", "sweep", "serve"],
@@ -380,24 +398,26 @@ def main():
 
     N = a.n
     L = 0.25 * (N - 1)            # dx = 0.5
-    log(f"seeding {N}^3 gauged state on the host")
+    log(f"seeding {N}^3 {a.mode} state on the host")
     t0 = time.time()
-    st = build_state(N, L)
+    st = build_state(N, L, a.mode)
     log(f"seeded in {time.time() - t0:.1f}s")
-    head = run_one(G, N, L, st, a.prec, a.lo_words, a.steps, a.warmup)
+    head = run_one(G, N, L, st, a.prec, a.lo_words, a.steps, a.warmup, a.mode)
     cmp64 = None
     if a.prec == "f32c" and not a.no_compare:
-        cmp64 = run_one(G, N, L, st, "f64", 0, a.steps, a.warmup)
+        cmp64 = run_one(G, N, L, st, "f64", 0, a.steps, a.warmup, a.mode)
     del st
-    cpu = None if a.no_cpu_baseline else cpu_baseline()
+    cpu = None if a.no_cpu_baseline else cpu_baseline(a.mode)
     dtype = "f64" if a.prec == "f64" else f"f32c ({LO_NAMES[a.lo_words]} compensation words)"
     out = {
-        "metric": "Gsite-updates/s & %HBM peak, 512^3 gauged, FP32c vs FP64; p50/p99 job latency",
+        "metric": ("Gsite-updates/s & %HBM peak, 512^3 gauged, FP32c vs FP64; p50/p99 job latency"
+                   if a.mode == "gauged" else f"Gsite-updates/s & %HBM peak, {N}^3 real-mode step"),
         "value": head["value"], "unit": "Gsite-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": dtype, "data": "synthetic (seeded Gaussian Q-balls, generated on the host)",
-        "config": {"workload": f"gauged {N}^3 periodic leapfrog step, 64 Q-balls, dx=0.5 dt=0.1",
-                   "N": N, "mode": "gauged", "precision": a.prec, "lo_words": a.lo_words, "g": 0.05},
+        "config": {"workload": f"{a.mode} {N}^3 leapfrog step, 64 Q-balls, dx=0.5 dt=0.1",
+                   "N": N, "mode": a.mode, "precision": a.prec, "lo_words": a.lo_words,
+                   "g": 0.05 if a.mode == "gauged" else None},
         "roofline": {"bound": "hbm", "achieved": head["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": head["achieved"] / HBM_PEAK_GBS, "traffic": head["traffic"],
                      "kernel": head["kernel"], "bytes_per_site": head["bytes_per_site"],
