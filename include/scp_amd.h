@@ -110,9 +110,9 @@ int  scp_grid_download(scp_grid *g, double *const *cols, int ncols);
 int  scp_grid_step(scp_grid *g, int nsteps);
 int  scp_grid_diag(scp_grid *g, double *out, int nout);
 /* scp_grid_step followed by scp_grid_diag of the resulting state, with the
- * diagnostics fused into the step's last launch (complex and gauged modes on
- * the LDS-staged march: every energy, charge and moment sum is reduced by the
- * step kernel itself; the Gauss-law statistics take one light pass over E).
+ * diagnostics fused into the step's last launch (grids on the LDS-staged
+ * march: every energy, charge and moment sum is reduced by the step kernel
+ * itself; gauged, the Gauss-law statistics take one light pass over E).
  * Replaces the main loop's verlet_step + diagnostics pair on a diag step
  * (scp_sim.c:3082 and 2993-3065; scp_sim.cu:5492, 3697-4041). Other grids
  * run the step and the stand-alone pass. The first call on a gauged grid

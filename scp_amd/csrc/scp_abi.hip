@@ -558,12 +558,12 @@ extern "C" int scp_grid_diag(scp_grid *g, double *out, int nout) {
 // state per block (and the gauged charge density), a light pass adds the
 // Gauss-law statistics, which need E at neighbours other blocks compute. Grids
 // whose step does not run the march (direct-neighbour kernel, gauged storage
-// with g = 0) and real-mode grids take the step and the stand-alone pass.
+// with g = 0) take the step and the stand-alone pass.
 extern "C" int scp_grid_step_diag(scp_grid *g, int nsteps, double *out, int nout) {
     if (!g || !out) return set_err(-1, "null argument");
     if (nsteps < 1) return set_err(-1, "nsteps=%d must be >= 1", nsteps);
     if (nout < SCP_D_COUNT) return set_err(-1, "nout=%d < %d", nout, SCP_D_COUNT);
-    if (!use_march(g, K_STEP) || g->step_mode != g->mode || g->mode == 0) {
+    if (!use_march(g, K_STEP) || g->step_mode != g->mode) {
         int rc = scp_grid_step(g, nsteps);
         return rc ? rc : scp_grid_diag(g, out, nout);
     }

@@ -62,7 +62,6 @@ k_diag1(KParams kp, DiagParams dp, const double *__restrict__ xo, const double *
         double *partials) {
     const int N = kp.N;
     const uint32_t fs = kp.fs, n3 = kp.n3;
-    const double dx = kp.dx, dV = dp.dV, h1 = kp.h1, L = kp.L;
     auto X = [&](int f, uint32_t c) -> double { return xo[(size_t)f * fs + c]; };
     auto V = [&](int f, uint32_t c) -> double { return vo[(size_t)f * fs + c]; };
     double S[DS_NSUM], M[DM_NMAX];
@@ -70,75 +69,19 @@ k_diag1(KParams kp, DiagParams dp, const double *__restrict__ xo, const double *
     for (int q = 0; q < DS_NSUM; q++) S[q] = 0.0;
 #pragma unroll
     for (int q = 0; q < DM_NMAX; q++) M[q] = 0.0;
-    constexpr int NFD = MODE == 2 ? 15 : 12;
+    constexpr int NFD = MODE == 2 ? 15 : (MODE == 1 ? 12 : 6);
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n3; c += gridDim.x * blockDim.x) {
         const int k = c % N, j = (c / N) % N, i = c / ((uint32_t)N * N);
         uint32_t np[3] = {sidx3(N, wpd(i, N), j, k), sidx3(N, i, wpd(j, N), k), sidx3(N, i, j, wpd(k, N))};
         uint32_t nm[3] = {sidx3(N, wmd(i, N), j, k), sidx3(N, i, wmd(j, N), k), sidx3(N, i, j, wmd(k, N))};
-        const double x = -L + i * dx, y = -L + j * dx, z = -L + k * dx;
-        if constexpr (MODE == 0) {              // compute_energy, C:1734-1796 (+ C:1803-1816)
-            const int TU = 3;
-            double p0 = X(0, c), p1 = X(1, c), p2 = X(2, c);
-            double sig = p0 * p0 + p1 * p1 + p2 * p2;
-            double me2 = (kp.rmode == 1) ? kp.IA / (1.0 + kp.IB * sig) : kp.MASS2;
-            double keff = (kp.rmode == 3) ? kp.KAPPA / (1.0 + kp.KG * sig) : kp.KAPPA;
-            for (int a = 0; a < 3; a++) {
-                S[DS_EPK] += 0.5 * V(a, c) * V(a, c) * dV;
-                S[DS_ETK] += 0.5 * V(TU + a, c) * V(TU + a, c) * dV;
-                double gx = (X(a, np[0]) - X(a, nm[0])) * h1, gy = (X(a, np[1]) - X(a, nm[1])) * h1;
-                double gz = (X(a, np[2]) - X(a, nm[2])) * h1;
-                S[DS_EG] += 0.5 * (gx * gx + gy * gy + gz * gz) * dV;
-                S[DS_EM] += 0.5 * me2 * X(a, c) * X(a, c) * dV;
-                double tx = (X(TU + a, np[0]) - X(TU + a, nm[0])) * h1, ty = (X(TU + a, np[1]) - X(TU + a, nm[1])) * h1;
-                double tz = (X(TU + a, np[2]) - X(TU + a, nm[2])) * h1;
-                S[DS_ETG] += 0.5 * (tx * tx + ty * ty + tz * tz) * dV;
-                S[DS_ETM] += 0.5 * kp.MT2 * X(TU + a, c) * X(TU + a, c) * dV;
-                M[DM_PHI] = fmax(M[DM_PHI], fabs(X(a, c)));
-                S[DS_TH2] += X(TU + a, c) * X(TU + a, c);
-            }
-            double P = p0 * p1 * p2, P2 = P * P;
-            if (kp.rmode == 3) {
-                double D = 1.0 + kp.KG * sig + kp.KAPPA * P2;
-                S[DS_EP] += (kp.MU / 2.0) * P2 * (1.0 + kp.KG * sig) / D * dV;
-            } else {
-                S[DS_EP] += (kp.MU / 2.0) * P2 / (1.0 + keff * P2) * dV;
-            }
-            M[DM_P] = fmax(M[DM_P], fabs(P));
-            if (kp.ACS != 0 || kp.BH != 0 || kp.KH != 0) {
-                double cx[3];
-                cx[0] = (X(2, np[1]) - X(2, nm[1]) - X(1, np[2]) + X(1, nm[2])) * h1;
-                cx[1] = (X(0, np[2]) - X(0, nm[2]) - X(2, np[0]) + X(2, nm[0])) * h1;
-                cx[2] = (X(1, np[0]) - X(1, nm[0]) - X(0, np[1]) + X(0, nm[1])) * h1;
-                if (kp.ACS != 0) {
-                    double m0 = cx[0] * 0.5 - X(3, c), m1 = cx[1] * 0.5 - X(4, c), m2 = cx[2] * 0.5 - X(5, c);
-                    S[DS_EP] += kp.ACS * (m0 * m0 + m1 * m1 + m2 * m2) * dV;
-                }
-                if (kp.BH != 0) {
-                    double csq = cx[0] * cx[0] + cx[1] * cx[1] + cx[2] * cx[2];
-                    double T2 = X(3, c) * X(3, c) + X(4, c) * X(4, c) + X(5, c) * X(5, c);
-                    S[DS_EP] += 0.5 * kp.BH * T2 * csq * dV;
-                }
-                if (kp.KH != 0) {
-                    double hel = p0 * cx[0] + p1 * cx[1] + p2 * cx[2];
-                    S[DS_EP] += kp.KH * P2 * hel * dV;
-                }
-            }
-            double ct[3];
-            ct[0] = (X(TU + 2, np[1]) - X(TU + 2, nm[1]) - X(TU + 1, np[2]) + X(TU + 1, nm[2])) * h1;
-            ct[1] = (X(TU + 0, np[2]) - X(TU + 0, nm[2]) - X(TU + 2, np[0]) + X(TU + 2, nm[0])) * h1;
-            ct[2] = (X(TU + 1, np[0]) - X(TU + 1, nm[0]) - X(TU + 0, np[1]) + X(TU + 0, nm[1])) * h1;
-            for (int a = 0; a < 3; a++) S[DS_EC] -= kp.ETA * X(a, c) * ct[a] * dV;
-            S[DS_PINT] += fabs(p0 * p1 * p2) * dV;
-        } else {
-            const auto xn = [&](int f, int d, int sg) -> double { return X(f, sg > 0 ? np[d] : nm[d]); };
-            double xc[15] = {}, vc[15] = {};
+        const auto xn = [&](int f, int d, int sg) -> double { return X(f, sg > 0 ? np[d] : nm[d]); };
+        double xc[15] = {}, vc[15] = {};
 #pragma unroll
-            for (int f = 0; f < NFD; f++) { xc[f] = X(f, c); vc[f] = V(f, c); }
-            const double rho = diag_site<MODE>(kp, dp, xc, vc, xn, i, j, k, S, M);
-            if (MODE == 2) {
-                const double Em[3] = {V(F_TH + 0, nm[0]), V(F_TH + 1, nm[1]), V(F_TH + 2, nm[2])};
-                gauss_site(kp, dp, vc + F_TH, Em, rho, i, j, k, S, M);
-            }
+        for (int f = 0; f < NFD; f++) { xc[f] = X(f, c); vc[f] = V(f, c); }
+        const double rho = diag_site<MODE>(kp, dp, xc, vc, xn, i, j, k, S, M);
+        if (MODE == 2) {
+            const double Em[3] = {V(F_TH + 0, nm[0]), V(F_TH + 1, nm[1]), V(F_TH + 2, nm[2])};
+            gauss_site(kp, dp, vc + F_TH, Em, rho, i, j, k, S, M);
         }
     }
     double *out = partials + (size_t)blockIdx.x * (DS_NSUM + DM_NMAX);

@@ -1,4 +1,4 @@
-// scp_diag_site.h — per-site diagnostics of the complex and gauged modes,
+// scp_diag_site.h — per-site diagnostics of the three modes,
 // shared by the stand-alone diagnostics pass (scp_diag.hip, neighbours from the
 // emitted FP64 state) and the diagnostics fused into the step's emitting
 // launch (scp_march.h, neighbours from the staged planes).
@@ -17,13 +17,81 @@
 
 namespace scp {
 
+// Real-mode terms of one site (compute_energy, C:1734-1796, with the alpha_cs,
+// beta_h, kappa_h and coupling terms of C:1803-1816; P_int and theta_rms of
+// C:2039-2059): xc/vc the six fields and velocities (phi 0..2, theta 3..5), xn
+// as below.
+template <class XN>
+__device__ inline void diag_site_real(const KParams &kp, const DiagParams &dp, const double *xc, const double *vc,
+                                      const XN &xn, double *S, double *M) {
+#pragma clang fp contract(off)
+    const double dV = dp.dV, h1 = kp.h1;
+    const int TU = 3;
+    double p0 = xc[0], p1 = xc[1], p2 = xc[2];
+    double sig = p0 * p0 + p1 * p1 + p2 * p2;
+    double me2 = (kp.rmode == 1) ? kp.IA / (1.0 + kp.IB * sig) : kp.MASS2;
+    double keff = (kp.rmode == 3) ? kp.KAPPA / (1.0 + kp.KG * sig) : kp.KAPPA;
+    for (int a = 0; a < 3; a++) {
+        S[DS_EPK] += 0.5 * vc[a] * vc[a] * dV;
+        S[DS_ETK] += 0.5 * vc[TU + a] * vc[TU + a] * dV;
+        double gx = (xn(a, 0, 1) - xn(a, 0, -1)) * h1, gy = (xn(a, 1, 1) - xn(a, 1, -1)) * h1;
+        double gz = (xn(a, 2, 1) - xn(a, 2, -1)) * h1;
+        S[DS_EG] += 0.5 * (gx * gx + gy * gy + gz * gz) * dV;
+        S[DS_EM] += 0.5 * me2 * xc[a] * xc[a] * dV;
+        double tx = (xn(TU + a, 0, 1) - xn(TU + a, 0, -1)) * h1, ty = (xn(TU + a, 1, 1) - xn(TU + a, 1, -1)) * h1;
+        double tz = (xn(TU + a, 2, 1) - xn(TU + a, 2, -1)) * h1;
+        S[DS_ETG] += 0.5 * (tx * tx + ty * ty + tz * tz) * dV;
+        S[DS_ETM] += 0.5 * kp.MT2 * xc[TU + a] * xc[TU + a] * dV;
+        M[DM_PHI] = fmax(M[DM_PHI], fabs(xc[a]));
+        S[DS_TH2] += xc[TU + a] * xc[TU + a];
+    }
+    double P = p0 * p1 * p2, P2 = P * P;
+    if (kp.rmode == 3) {
+        double D = 1.0 + kp.KG * sig + kp.KAPPA * P2;
+        S[DS_EP] += (kp.MU / 2.0) * P2 * (1.0 + kp.KG * sig) / D * dV;
+    } else {
+        S[DS_EP] += (kp.MU / 2.0) * P2 / (1.0 + keff * P2) * dV;
+    }
+    M[DM_P] = fmax(M[DM_P], fabs(P));
+    if (kp.ACS != 0 || kp.BH != 0 || kp.KH != 0) {
+        double cx[3];
+        cx[0] = (xn(2, 1, 1) - xn(2, 1, -1) - xn(1, 2, 1) + xn(1, 2, -1)) * h1;
+        cx[1] = (xn(0, 2, 1) - xn(0, 2, -1) - xn(2, 0, 1) + xn(2, 0, -1)) * h1;
+        cx[2] = (xn(1, 0, 1) - xn(1, 0, -1) - xn(0, 1, 1) + xn(0, 1, -1)) * h1;
+        if (kp.ACS != 0) {
+            double m0 = cx[0] * 0.5 - xc[3], m1 = cx[1] * 0.5 - xc[4], m2 = cx[2] * 0.5 - xc[5];
+            S[DS_EP] += kp.ACS * (m0 * m0 + m1 * m1 + m2 * m2) * dV;
+        }
+        if (kp.BH != 0) {
+            double csq = cx[0] * cx[0] + cx[1] * cx[1] + cx[2] * cx[2];
+            double T2 = xc[3] * xc[3] + xc[4] * xc[4] + xc[5] * xc[5];
+            S[DS_EP] += 0.5 * kp.BH * T2 * csq * dV;
+        }
+        if (kp.KH != 0) {
+            double hel = p0 * cx[0] + p1 * cx[1] + p2 * cx[2];
+            S[DS_EP] += kp.KH * P2 * hel * dV;
+        }
+    }
+    double ct[3];
+    ct[0] = (xn(TU + 2, 1, 1) - xn(TU + 2, 1, -1) - xn(TU + 1, 2, 1) + xn(TU + 1, 2, -1)) * h1;
+    ct[1] = (xn(TU + 0, 2, 1) - xn(TU + 0, 2, -1) - xn(TU + 2, 0, 1) + xn(TU + 2, 0, -1)) * h1;
+    ct[2] = (xn(TU + 1, 0, 1) - xn(TU + 1, 0, -1) - xn(TU + 0, 1, 1) + xn(TU + 0, 1, -1)) * h1;
+    for (int a = 0; a < 3; a++) S[DS_EC] -= kp.ETA * xc[a] * ct[a] * dV;
+    S[DS_PINT] += fabs(p0 * p1 * p2) * dV;
+}
+
 // xc, vc: the site's fields and velocities (device field order, FP64);
 // xn(f, d, sg): field f at the neighbour one step along d (0 = i, 1 = j,
-// 2 = k) in direction sg = +1 / -1, as a double.
+// 2 = k) in direction sg = +1 / -1, as a double. Returns the Gauss-law charge
+// density (gauged mode; 0 otherwise).
 template <int MODE, class XN>
 __device__ inline double diag_site(const KParams &kp, const DiagParams &dp, const double *xc, const double *vc,
                                    const XN &xn, int i, int j, int k, double *S, double *M) {
 #pragma clang fp contract(off)
+    if constexpr (MODE == 0) {
+        diag_site_real(kp, dp, xc, vc, xn, S, M);
+        return 0.0;
+    }
     const double dx = kp.dx, dV = dp.dV, h1 = kp.h1, L = kp.L;
     const double x = -L + i * dx, y = -L + j * dx, z = -L + k * dx;
     const bool gauged = (MODE == 2) && dp.gauged;

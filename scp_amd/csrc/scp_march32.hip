@@ -37,13 +37,14 @@ int launch_march_f32c(int mode, bool emit, int lo_kind, const KParams &kp, const
 
 template <int LOK>
 static int diag_lo(int mode, const KParams &kp, const Bufs32 &b, int DI, const DiagArgs &da, hipStream_t s) {
-    return mode == 2 ? launch_march_diag_t<S32<LOK>, 2>(kp, b, DI, da, s)
-                     : launch_march_diag_t<S32<LOK>, 1>(kp, b, DI, da, s);
+    return mode == 2   ? launch_march_diag_t<S32<LOK>, 2>(kp, b, DI, da, s)
+           : mode == 1 ? launch_march_diag_t<S32<LOK>, 1>(kp, b, DI, da, s)
+                       : launch_march_diag_t<S32<LOK>, 0>(kp, b, DI, da, s);
 }
 
 int launch_march_f32c_diag(int mode, int lo_kind, const KParams &kp, const Bufs32 &b, int DI, const DiagArgs &da,
                            hipStream_t s) {
-    if (mode != 1 && mode != 2) return (int)hipErrorInvalidValue;
+    if (mode < 0 || mode > 2) return (int)hipErrorInvalidValue;
     if (lo_kind == LO_F32) return diag_lo<LO_F32>(mode, kp, b, DI, da, s);
     if (lo_kind == LO_BF16) return diag_lo<LO_BF16>(mode, kp, b, DI, da, s);
     if (lo_kind == LO_Q8) return diag_lo<LO_Q8>(mode, kp, b, DI, da, s);

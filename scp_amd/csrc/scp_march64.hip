@@ -17,6 +17,7 @@ int launch_march_f64_diag(int mode, const KParams &kp, const Bufs<double> &b, in
                           hipStream_t s) {
     if (mode == 2) return launch_march_diag_t<S64D, 2>(kp, b, DI, da, s);
     if (mode == 1) return launch_march_diag_t<S64D, 1>(kp, b, DI, da, s);
+    if (mode == 0) return launch_march_diag_t<S64D, 0>(kp, b, DI, da, s);
     return (int)hipErrorInvalidValue;
 }
 

@@ -51,14 +51,15 @@ def _pair(params, st, steps):
 
 def _case(mode, N, bc):
     L = 0.25 * (N - 1)
-    st = seeded(N, mode, L, link_sigma=0.05)[: 24 if mode == 1 else 30]
+    st = seeded(N, max(mode, 1), L, link_sigma=0.05)[: {0: 12, 1: 24, 2: 30}[mode]].copy()
     kw = dict(g=0.1) if mode == 2 else {}
     if bc == 0:
         kw.update(bc_type=0, damp_width=0.2 * L, damp_rate=0.05)
     return L, st, kw
 
 
-@pytest.mark.parametrize("mode,N,bc", [(1, 10, 2), (2, 10, 2), (1, 37, 2), (2, 70, 2), (2, 33, 0), (1, 24, 0)])
+@pytest.mark.parametrize("mode,N,bc", [(1, 10, 2), (2, 10, 2), (1, 37, 2), (2, 70, 2), (2, 33, 0), (1, 24, 0),
+                                       (0, 10, 2), (0, 37, 2), (0, 24, 0)])
 def test_fused_diag_equals_standalone_f64(mode, N, bc):
     L, st, kw = _case(mode, N, bc)
     print(f"case f64 mode={mode} N={N} bc={bc}", flush=True)
@@ -72,7 +73,8 @@ def test_fused_diag_equals_standalone_f64(mode, N, bc):
             assert abs(d_f[k] - d_s[k]) <= tol, f"{k}: fused {d_f[k]!r} stand-alone {d_s[k]!r}"
 
 
-@pytest.mark.parametrize("mode,N,bc,lo", [(2, 33, 2, 1), (1, 37, 2, 1), (2, 48, 0, 1), (2, 40, 0, 4)])
+@pytest.mark.parametrize("mode,N,bc,lo", [(2, 33, 2, 1), (1, 37, 2, 1), (2, 48, 0, 1), (2, 40, 0, 4),
+                                          (0, 37, 2, 4), (0, 33, 0, 1)])
 def test_fused_diag_f32c(mode, N, bc, lo):
     L, st, kw = _case(mode, N, bc)
     print(f"case f32c mode={mode} N={N} bc={bc} lo={lo}", flush=True)
@@ -94,9 +96,10 @@ def test_fused_diag_f32c(mode, N, bc, lo):
     print("relative differences of the neighbour terms:", {k: f"{v:.1e}" for k, v in worst.items()})
 
 
-@pytest.mark.parametrize("name", ["complex_n8", "complex_damp_n10", "gauged_n8", "gauged_damp_n10"])
+@pytest.mark.parametrize("name", ["complex_n8", "complex_damp_n10", "gauged_n8", "gauged_damp_n10", "real_n8",
+                                  "real_mode1_n8", "real_terms_n8"])
 def test_fused_diag_matches_oracle(name):
-    """The fused diagnostics against the oracle restatement of compute_energy_complex*,
+    """The fused diagnostics against the oracle restatement of compute_energy*,
     compute_charges and compute_gauss on the state the step emits."""
     fx = fixtures.load(name)
     n, p = fixtures.nsteps(fx)
@@ -114,13 +117,14 @@ def test_fused_diag_matches_oracle(name):
 
 
 def test_fused_diag_other_grids_take_the_standalone_pass():
-    """Real mode and the direct-neighbour kernel: step_diag = step + diag."""
+    """Grids on the direct-neighbour kernel (real mode with the pass-1 fields,
+    step_kernel 1): step_diag = step + diag."""
     for mode, sk in ((0, 0), (2, 1)):
         N = 12
         L = 0.25 * (N - 1)
         st = seeded(N, max(mode, 1), L, link_sigma=0.05)
         st = st[:12] if mode == 0 else st
-        kw = dict(g=0.1) if mode == 2 else {}
+        kw = dict(g=0.1) if mode == 2 else dict(alpha_cs=0.1)
         d_f, s_f, d_s, s_s = _pair(G.make_params(N, mode, L, 0.1, step_kernel=sk, **kw), st, 4)
         assert np.array_equal(s_f, s_s)
         for k in G.DIAG_NAMES:
