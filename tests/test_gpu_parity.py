@@ -233,11 +233,12 @@ def test_no_cpu_fallback_symbols():
     assert L.scp_abi_version() == 1
 
 
-@pytest.mark.parametrize("mode,N", [(1, 10), (2, 10), (1, 70), (2, 70)])
+@pytest.mark.parametrize("mode,N", [(1, 10), (2, 10), (1, 70), (2, 70), (1, 4), (2, 4)])
 def test_march_kernel_equals_direct_kernel(mode, N):
     """The LDS-staged march (step_kernel 0) and the direct-neighbour march
     (step_kernel 1) run the same per-site arithmetic: FP64 results are bitwise
-    equal, also on ragged tiles (N = 70: a 6-wide k tile and a 2-row j tile)."""
+    equal, also on ragged tiles (N = 70: a 6-wide k tile and a 2-row j tile)
+    and on the smallest grid (N = 4: the halo wraps onto the tile itself)."""
     L = 0.25 * (N - 1)
     st = seeded(N, mode, L, link_sigma=0.05)[: 24 if mode == 1 else 30]
     kw = dict(g=0.1) if mode == 2 else {}
@@ -263,7 +264,7 @@ REAL_MARCH_CASES = {
 }
 
 
-@pytest.mark.parametrize("N", [10, 70])
+@pytest.mark.parametrize("N", [4, 10, 70])
 @pytest.mark.parametrize("case", list(REAL_MARCH_CASES))
 def test_real_march_equals_direct_kernel(case, N):
     """Real mode on the LDS-staged march (no alpha_cs / beta_h / sigma_freq
