@@ -196,6 +196,57 @@ def diag_step_timing(g, reps=3):
             "fused": med(lambda: g.step_diag(1))}
 
 
+def drift_bench(a):
+    """FP32C against FP64 over many steps on the headline state: gauged 512^3,
+    64 Q-balls (alternating charge signs), Gauss-projected at t = 0, both grids
+    resident and stepped in chunks with the diagnostics fused into the last
+    step of each chunk. Reports each path's energy drift and the FP32C-FP64
+    differences of E_total and Q_total; the charge difference is relative to
+    the sum of the balls' |Q| (sum over sites of |Phi's charge density| x dV of
+    the initial state), since the net charge of alternating balls is near 0."""
+    from scp_amd import grid as G
+    N = a.n
+    L = 0.25 * (N - 1)
+    dx = 2 * L / (N - 1)
+    log(f"seeding {N}^3 gauged state on the host")
+    st = build_state(N, L)
+    qabs = dx ** 3 * float(np.abs(sum(st[c] * st[18 + c] - st[12 + c] * st[6 + c] for c in range(3))).sum())
+    p64 = G.make_params(N, G.MODE_GAUGED, L, 0.2, g=0.05)
+    p32 = G.make_params(N, G.MODE_GAUGED, L, 0.2, g=0.05, precision=G.PREC_F32C, lo_words=a.lo_words)
+    rows = []
+    with G.Grid(p64) as g64, G.Grid(p32) as g32:
+        for g in (g64, g32):
+            g.upload(st)
+            g.gauss_project()
+        del st
+        d64, d32 = g64.diag(), g32.diag()
+        e0, q0 = d64["E_total"], d64["Q_total"]
+        log(f"t=0: E {e0:.12e} Q {q0:.6e} sum|Q| {qabs:.6e} gauss_max {d64['gauss_max']:.3e} / {d32['gauss_max']:.3e}")
+        done, t0 = 0, time.time()
+        while done < a.drift_steps:
+            n = min(a.chunk, a.drift_steps - done)
+            d64, d32 = g64.step_diag(n), g32.step_diag(n)
+            done += n
+            row = {"step": done, "dE_f64": (d64["E_total"] - e0) / abs(e0), "dE_f32c": (d32["E_total"] - e0) / abs(e0),
+                   "dE_f32c_vs_f64": (d32["E_total"] - d64["E_total"]) / abs(d64["E_total"]),
+                   "dQ_f32c_vs_f64": (d32["Q_total"] - d64["Q_total"]) / qabs,
+                   "gauss_max_f64": d64["gauss_max"], "gauss_max_f32c": d32["gauss_max"]}
+            rows.append(row)
+            log(f"step {done}: " + " ".join(f"{k} {v:.3e}" for k, v in row.items() if k != "step")
+                + f" ({time.time() - t0:.0f}s)")
+    last = rows[-1]
+    out = {"metric": "FP32C vs FP64 energy / charge difference over a long run, 512^3 gauged",
+           "value": max(abs(last["dE_f32c_vs_f64"]), abs(last["dQ_f32c_vs_f64"])), "unit": "relative",
+           "n_gpus": 1, "steps": a.drift_steps, "higher_is_better": False,
+           "dtype": f"f32c ({LO_NAMES[a.lo_words]} compensation words) vs f64",
+           "data": "synthetic (seeded Gaussian Q-balls, generated on the host, Gauss-projected)",
+           "config": {"workload": f"gauged {N}^3 periodic, 64 Q-balls, dx=0.5 dt=0.1, {a.drift_steps} steps",
+                      "N": N, "chunk": a.chunk, "lo_words": a.lo_words, "sum_abs_Q": qabs, "E0": e0, "Q0": q0},
+           "max_abs": {k: max(abs(r[k]) for r in rows) for k in rows[0] if k != "step"},
+           "rows": rows}
+    print(json.dumps(out), flush=True)
+
+
 def sweep_bench(a):
     """Batched parameter sweep: independent gauged grids of edge drawn from
     {32,48,64,80,96} (seeded), one Q-ball each (amp U[0.6,0.9], width U[2,4]
@@ -362,9 +413,12 @@ def main():
                     help="step workload: gauged (the headline) or real (6-field real-mode step)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="step", choices=["step", "sweep", "serve"],
+    ap.add_argument("--drift-steps", type=int, default=10000, help="drift: steps of the long run")
+    ap.add_argument("--chunk", type=int, default=1000, help="drift: steps between diagnostics")
+    ap.add_argument("--workload", default="step", choices=["step", "sweep", "serve", "drift"],
                     help="step: the 512^3 gauged step (default); sweep: batched independent grids; "
-                         "serve: Poisson job arrivals through the request loop")
+                         "serve: Poisson job arrivals through the request loop; drift: FP32C against "
+                         "FP64 over --drift-steps steps")
     ap.add_argument("--rate", type=float, default=4.0, help="serve: job arrivals per second")
     ap.add_argument("--duration", type=float, default=600.0, help="serve: arrival window (s)")
     ap.add_argument("--horizon", type=float, default=60.0, help="serve: time after the window to finish jobs (s)")
@@ -390,6 +444,9 @@ def main():
         sys.exit(2)
     if a.workload == "sweep":
         sweep_bench(a)
+        return
+    if a.workload == "drift":
+        drift_bench(a)
         return
     if a.workload == "serve":
         serve_bench(a)
