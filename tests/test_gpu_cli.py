@@ -195,3 +195,37 @@ def test_cli_two_balls_128_against_reference_binary(tmp_path):
     for a, b in zip(rr, rm):
         assert abs(a[iE] - b[iE]) <= 1e-6 * abs(a[iE]), (a[iE], b[iE])
         assert abs(a[iQ] - b[iQ]) <= 1e-6 * qscale, (a[iQ], b[iQ])
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="the reference binary (oracle/_ref) is not built")
+def test_cli_complex_64_500_steps_bitwise_against_reference_binary(tmp_path):
+    """Complex mode, 64^3 periodic, 500 steps, FP64: one centred ball (amplitude
+    0.8, width 3, omega 0.8, winding 1), theta N(0, 0.01^2): the drop-in
+    binary's frames are bitwise the reference binary's and every diag.tsv
+    value matches to the printed precision."""
+    from scp_amd import seeds
+    N, steps = 64, 500
+    L = 0.25 * (N - 1)
+    dt = 0.2 * 2 * L / (N - 1)
+    balls = [dict(x0=0.0, y0=0.0, z0=0.0, amp=0.8, sigma=3.0, omega=0.8, phase=0.0, winding=1)]
+    st = seeds.seed_balls(N, L, 1, balls, theta_sigma=0.01, seed=11)
+    seed = tmp_path / "seed.sfa"
+    sfa.write_sim_sfa(seed, N, L, dt, [(0.0, st)])
+    runs = {}
+    for name, binary in (("ref", REF), ("gpu", BIN)):
+        wd = tmp_path / name
+        wd.mkdir()
+        cfg = {"N": N, "L": L, "T": steps * dt, "dt_factor": 0.2, "precision": "f64", "bc_type": 2,
+               "snap_dt": steps * dt, "diag_dt": 100 * dt, "complex_phi": 1, "init": "sfa",
+               "init_sfa": str(seed), "output": str(wd / "out.sfa"), "diag_file": str(wd / "diag.tsv")}
+        (wd / "run.cfg").write_text("".join(f"{k} = {v}\n" for k, v in cfg.items()))
+        print(f"case complex 64^3: {name} run", flush=True)
+        res = subprocess.run([binary, str(wd / "run.cfg")], capture_output=True, text=True, cwd=wd, timeout=600)
+        print(res.stdout[-400:], res.stderr[-400:], flush=True)
+        assert res.returncode == 0, name
+        with sfa.SFAFile(wd / "out.sfa") as f:
+            runs[name] = ([f.read_frame(i) for i in range(f.total_frames)], (wd / "diag.tsv").read_text())
+    assert len(runs["ref"][0]) == len(runs["gpu"][0]) == 2
+    for a, b in zip(runs["ref"][0], runs["gpu"][0]):
+        assert np.array_equal(a, b)
+    compare_diag(runs["ref"][1], runs["gpu"][1])
