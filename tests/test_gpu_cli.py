@@ -229,3 +229,32 @@ def test_cli_complex_64_500_steps_bitwise_against_reference_binary(tmp_path):
     for a, b in zip(runs["ref"][0], runs["gpu"][0]):
         assert np.array_equal(a, b)
     compare_diag(runs["ref"][1], runs["gpu"][1])
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="the reference binary (oracle/_ref) is not built")
+def test_cli_real_oscillon_64_bitwise_against_reference_binary(tmp_path):
+    """Real mode with the reference's own oscillon init and absorbing sphere
+    (bc_type 0: the damping sponge), 64^3, 1000 steps, FP64, on the LDS-staged
+    march: the frames are bitwise the reference binary's and every diag.tsv
+    value matches to the printed precision."""
+    N, steps = 64, 1000
+    L = 0.25 * (N - 1)
+    dt = 0.2 * 2 * L / (N - 1)
+    runs = {}
+    for name, binary in (("ref", REF), ("gpu", BIN)):
+        wd = tmp_path / name
+        wd.mkdir()
+        cfg = {"N": N, "L": L, "T": steps * dt, "dt_factor": 0.2, "precision": "f64", "bc_type": 0,
+               "snap_dt": steps * dt / 2, "diag_dt": 100 * dt, "init": "oscillon",
+               "output": str(wd / "out.sfa"), "diag_file": str(wd / "diag.tsv")}
+        (wd / "run.cfg").write_text("".join(f"{k} = {v}\n" for k, v in cfg.items()))
+        print(f"case real oscillon 64^3: {name} run", flush=True)
+        res = subprocess.run([binary, str(wd / "run.cfg")], capture_output=True, text=True, cwd=wd, timeout=600)
+        print(res.stdout[-400:], res.stderr[-400:], flush=True)
+        assert res.returncode == 0, name
+        with sfa.SFAFile(wd / "out.sfa") as f:
+            runs[name] = ([f.read_frame(i) for i in range(f.total_frames)], (wd / "diag.tsv").read_text())
+    assert len(runs["ref"][0]) == len(runs["gpu"][0]) == 3
+    for a, b in zip(runs["ref"][0], runs["gpu"][0]):
+        assert np.array_equal(a, b)
+    compare_diag(runs["ref"][1], runs["gpu"][1])
