@@ -59,9 +59,9 @@ typedef struct {
                              1 bfloat16 (6 B/scalar), 2 none (plain FP32, 4 B/scalar),
                              3 signed byte in units of ulp(hi)/256 (5 B/scalar),
                              4 such bytes on the positions only (4.5 B/scalar) */
-    int32_t step_kernel;  /* 0: LDS-staged march (default, every mode; real-mode grids with
-                                alpha_cs, beta_h or sigma_freq, bc_type 1 and one field above
-                                4 GiB run the direct kernel), 1: direct-neighbour march */
+    int32_t step_kernel;  /* 0: LDS-staged march (default, every mode; bc_type 1 and one
+                                field above 4 GiB run the direct kernel), 1: direct-neighbour
+                                march */
     int32_t gradient_margin;  /* bc_type 1: x-slabs pinned on each face (gradient_margin,
                                  scp_config.h:110); must be 0 otherwise */
     double L, dx, dt;     /* dx = 2L/(N-1), dt = dt_factor*dx (scp_sim.c:204-205) */

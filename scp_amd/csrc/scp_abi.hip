@@ -22,16 +22,18 @@ namespace scp_internal {
 
 
 // K_STEP runs the LDS-staged march unless the caller asked for the
-// direct-neighbour kernel. Real-mode grids with the pass-1 fields (alpha_cs,
-// beta_h, sigma_freq: g->aux, read at neighbours) and bc 1 (pinned slabs) run
-// the direct kernel. The march addresses a field with 32-bit byte offsets, so
-// one field must stay below 4 GiB (N <= ~800 in FP64, N <= 1023 in FP32C);
-// larger grids run the direct-neighbour kernel.
+// direct-neighbour kernel; bc 1 (pinned slabs) runs the direct kernel. The
+// march addresses a field with 32-bit byte offsets, so one field must stay
+// below 4 GiB (N <= ~800 in FP64, N <= 1023 in FP32C); larger grids run the
+// direct-neighbour kernel.
 bool use_march(const scp_grid *g, int kind) {
     const uint64_t field_bytes = (uint64_t)g->kp.fsh * (g->f32 ? 4u : 8u);
-    return kind == K_STEP && g->p.step_kernel == 0 && (g->step_mode != 0 || !g->aux) &&
-           g->p.bc_type != 1 && field_bytes < (1ull << 32);
+    return kind == K_STEP && g->p.step_kernel == 0 && g->p.bc_type != 1 && field_bytes < (1ull << 32);
 }
+
+// the march instance of a grid: its step mode, or 3 for real-mode grids that
+// carry the pass-1 fields (the instance that reads them)
+static int march_mode(const scp_grid *g) { return g->step_mode == 0 && g->aux ? 3 : g->step_mode; }
 
 static thread_local char g_err[512];
 
@@ -243,9 +245,9 @@ extern "C" int scp_grid_create(const scp_params *p, scp_grid **out) {
     }
     if (use_march(g, K_STEP)) {
         if (g->f32)
-            snprintf(g->kname, sizeof g->kname, "k_march<scp::S32<%d>, 1, %d, false>", g->lo_kind, g->step_mode);
+            snprintf(g->kname, sizeof g->kname, "k_march<scp::S32<%d>, 1, %d, false>", g->lo_kind, march_mode(g));
         else
-            snprintf(g->kname, sizeof g->kname, "k_march<scp::S64, 1, %d, false>", g->step_mode);
+            snprintf(g->kname, sizeof g->kname, "k_march<scp::S64, 1, %d, false>", march_mode(g));
     } else if (g->f32)
         snprintf(g->kname, sizeof g->kname, "k_step32<%d, 1, false, %d>", g->step_mode, g->lo_kind);
     else
@@ -300,9 +302,9 @@ Bufs32 scp_internal::bufs32_for(scp_grid *g, int kind) {
 static int launch_kind(scp_grid *g, int kind, bool emit) {
     if (use_march(g, kind)) {
         if (g->f32)
-            return kerr(launch_march_f32c(g->step_mode, emit, g->lo_kind, g->kp, bufs32_for(g, kind), 0, g->stream),
+            return kerr(launch_march_f32c(march_mode(g), emit, g->lo_kind, g->kp, bufs32_for(g, kind), 0, g->stream),
                         "march32");
-        return kerr(launch_march_f64(g->step_mode, emit, g->kp, bufs_for(g, kind), 0, g->stream), "march64");
+        return kerr(launch_march_f64(march_mode(g), emit, g->kp, bufs_for(g, kind), 0, g->stream), "march64");
     }
     if (g->f32)
         return kerr(launch_step_f32c(g->step_mode, kind, emit, g->lo_kind, g->kp, bufs32_for(g, kind), g->stream), "step32");
@@ -387,9 +389,9 @@ static int enqueue_steps(scp_grid *g, int nsteps, const DiagArgs *da = nullptr) 
         }
         const bool emit = (s == nsteps - 1);
         if (emit && da)
-            rc = g->f32 ? kerr(launch_march_f32c_diag(g->step_mode, g->lo_kind, g->kp, bufs32_for(g, K_STEP), 0, *da,
+            rc = g->f32 ? kerr(launch_march_f32c_diag(march_mode(g), g->lo_kind, g->kp, bufs32_for(g, K_STEP), 0, *da,
                                                       g->stream), "march32 diag")
-                        : kerr(launch_march_f64_diag(g->step_mode, g->kp, bufs_for(g, K_STEP), 0, *da, g->stream),
+                        : kerr(launch_march_f64_diag(march_mode(g), g->kp, bufs_for(g, K_STEP), 0, *da, g->stream),
                                "march64 diag");
         else
             rc = launch_kind(g, K_STEP, emit);

@@ -237,10 +237,10 @@ extern "C" int scp_batch_destroy(scp_batch *b) {
 
 extern "C" int scp_batch_add(scp_batch *b, scp_grid *g) {
     if (!b || !g) return set_err(-1, "null argument");
-    if (!use_march(g, K_STEP))
+    if (!use_march(g, K_STEP) || g->aux)
         return set_err(-1, "grid cannot join a batch: the batched step is the LDS-staged march "
-                           "(step_kernel 0, bc_type != 1, real mode without alpha_cs / beta_h / sigma_freq, "
-                           "one field below 4 GiB)");
+                           "(step_kernel 0, bc_type != 1, one field below 4 GiB) without the real mode's "
+                           "pass-1 fields (alpha_cs, beta_h, sigma_freq)");
     if (g->batch) return set_err(-1, "grid is already a member of a batch");
     if (b->members.empty()) {
         b->step_mode = g->step_mode;

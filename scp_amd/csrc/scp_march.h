@@ -319,6 +319,9 @@ struct S32 {
 
 template <int MODE>
 constexpr int march_nf() { return MODE == 2 ? 15 : (MODE == 1 ? 12 : 6); }   // staged fields (matter + links)
+// MODE: 0 real, 1 complex, 2 gauged, 3 real with the pass-1 fields (alpha_cs,
+// beta_h, sigma_freq: an auxiliary array read from global memory; a separate
+// instance so that the plain real step keeps its registers)
 template <class S, int KW, int MODE>
 constexpr int march_lds_elems() { return MNBUF * march_nf<MODE>() * MG<KW>::MPL; }
 
@@ -478,8 +481,25 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
                 forces_gauge(kp, c0, c1, cm, nb, cy, acc);
             else if constexpr (MODE == 1)
                 forces_complex(kp, c0, c1, cm, nb, acc);
-            else   // real mode without the pass-1 fields (the caller's condition): AUX is never read
-                forces_real(kp, c0, c1, cm, nb, [](int, uint32_t) { return T(0); }, 0u, 0u, 0u, 0u, 0u, 0u, 0u, acc);
+            else if constexpr (MODE == 0)
+                forces_real(kp, c0, c1, cm, nb, [](int, uint32_t) { return T(0); },
+                            [](int, int) { return 0u; }, acc);
+            else {
+                // real mode with the pass-1 fields: they come from global memory at the unpadded
+                // site index, which equals the step-state index here (real-mode rows are not
+                // padded: pitch = N)
+                const double *__restrict__ aux = b.aux;
+                const auto AUX = [&](int f, uint32_t idx) -> T { return T(aux[(size_t)f * kp.fs + idx]); };
+                // neighbour indices formed where a pass-1 term reads them (not held otherwise)
+                const auto nbi = [&](int d, int sg) -> uint32_t {
+                    if (d == 3) return e;
+                    if (d == 0) return (uint32_t)(sg > 0 ? wp(i, N) : wm(i, N)) * NN + col;
+                    const uint32_t pl = (uint32_t)i * NN;
+                    if (d == 1) return pl + (uint32_t)(sg > 0 ? wp(t.j, N) : wm(t.j, N)) * kp.pitch + t.k;
+                    return pl + (uint32_t)t.j * kp.pitch + (sg > 0 ? wp(t.k, N) : wm(t.k, N));
+                };
+                forces_real(kp, c0, c1, cm, nb, AUX, nbi, acc);
+            }
             double dmp = 1.0;
             double xe[NF], ve[NF];   // the emitted state of this site (used by DIAG)
             typename S::Lo lo{};     // packed compensation words out
@@ -514,7 +534,7 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
                     }
                     return (double)nb.face(f, d, sg);
                 };
-                const double rho = diag_site<MODE>(kp, da->dp, xe, ve, xn, i, t.j, t.k, ds->S, ds->M);
+                const double rho = diag_site<MODE == 3 ? 0 : MODE>(kp, da->dp, xe, ve, xn, i, t.j, t.k, ds->S, ds->M);
                 if constexpr (MODE == 2) __builtin_nontemporal_store(rho, &da->rho[ec]);
             }
         };

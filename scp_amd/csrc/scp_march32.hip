@@ -12,6 +12,7 @@ template <int LOK, int KW>
 static int launch_lo(int mode, bool emit, const KParams &kp, const Bufs32 &b, int DI, hipStream_t s) {
     return mode == 2   ? launch_march_t<S32<LOK>, KW, 2>(emit, kp, b, DI, s)
            : mode == 1 ? launch_march_t<S32<LOK>, KW, 1>(emit, kp, b, DI, s)
+           : mode == 3 ? launch_march_t<S32<LOK>, KW, 3>(emit, kp, b, DI, s)
                        : launch_march_t<S32<LOK>, KW, 0>(emit, kp, b, DI, s);
 }
 
@@ -26,7 +27,7 @@ static int launch_kw(int mode, bool emit, int lo_kind, const KParams &kp, const 
 
 int launch_march_f32c_kw(int kw, int mode, bool emit, int lo_kind, const KParams &kp, const Bufs32 &b, int DI,
                          hipStream_t s) {
-    if (mode < 0 || mode > 2) return (int)hipErrorInvalidValue;
+    if (mode < 0 || mode > 3) return (int)hipErrorInvalidValue;
     return kw == 1 ? launch_kw<1>(mode, emit, lo_kind, kp, b, DI, s) : launch_kw<2>(mode, emit, lo_kind, kp, b, DI, s);
 }
 
@@ -39,12 +40,13 @@ template <int LOK>
 static int diag_lo(int mode, const KParams &kp, const Bufs32 &b, int DI, const DiagArgs &da, hipStream_t s) {
     return mode == 2   ? launch_march_diag_t<S32<LOK>, 2>(kp, b, DI, da, s)
            : mode == 1 ? launch_march_diag_t<S32<LOK>, 1>(kp, b, DI, da, s)
+           : mode == 3 ? launch_march_diag_t<S32<LOK>, 3>(kp, b, DI, da, s)
                        : launch_march_diag_t<S32<LOK>, 0>(kp, b, DI, da, s);
 }
 
 int launch_march_f32c_diag(int mode, int lo_kind, const KParams &kp, const Bufs32 &b, int DI, const DiagArgs &da,
                            hipStream_t s) {
-    if (mode < 0 || mode > 2) return (int)hipErrorInvalidValue;
+    if (mode < 0 || mode > 3) return (int)hipErrorInvalidValue;
     if (lo_kind == LO_F32) return diag_lo<LO_F32>(mode, kp, b, DI, da, s);
     if (lo_kind == LO_BF16) return diag_lo<LO_BF16>(mode, kp, b, DI, da, s);
     if (lo_kind == LO_Q8) return diag_lo<LO_Q8>(mode, kp, b, DI, da, s);

@@ -10,6 +10,7 @@ int launch_march_f64(int mode, bool emit, const KParams &kp, const Bufs<double> 
     if (mode == 2) return launch_march_t<S64, 1, 2>(emit, kp, b, DI, s);
     if (mode == 1) return launch_march_t<S64, 1, 1>(emit, kp, b, DI, s);
     if (mode == 0) return launch_march_t<S64, 1, 0>(emit, kp, b, DI, s);
+    if (mode == 3) return launch_march_t<S64, 1, 3>(emit, kp, b, DI, s);
     return (int)hipErrorInvalidValue;
 }
 
@@ -18,6 +19,7 @@ int launch_march_f64_diag(int mode, const KParams &kp, const Bufs<double> &b, in
     if (mode == 2) return launch_march_diag_t<S64D, 2>(kp, b, DI, da, s);
     if (mode == 1) return launch_march_diag_t<S64D, 1>(kp, b, DI, da, s);
     if (mode == 0) return launch_march_diag_t<S64D, 0>(kp, b, DI, da, s);
+    if (mode == 3) return launch_march_diag_t<S64D, 3>(kp, b, DI, da, s);
     return (int)hipErrorInvalidValue;
 }
 

@@ -243,16 +243,16 @@ __device__ inline void forces_complex(const KParams &kp, const CV &c, const PV &
 }
 
 // Real-mode accelerations (compute_forces, C:743-922). c/ip/im: own column at
-// planes i, i+1, i-1; nb: in-plane neighbours. AUX(f, idx) loads the
+// planes i, i+1, i-1; nb: in-plane neighbours; nbi(d, sg): the index of the
+// neighbour along d (0 = i, 1 = j, 2 = k) in direction sg, nbi(3, 0) the
+// site's own (called only where a pass-1 term is on). AUX(f, idx) loads the
 // pass-1 fields M[0..2] (f=0..2) and Q[0..2] (f=3..5) when alpha_cs/beta_h != 0,
 // and the local theta frequency (f=6) when sigma_freq != 0. With kp.XT the
 // terms of the reference's GPU step follow (compute_forces_kernel,
 // scp_sim.cu:1144-1248), added in its order onto the base accelerations.
-template <typename R, class CV, class PV, class MV, class NB, class LA>
+template <typename R, class CV, class PV, class MV, class NB, class LA, class IX>
 __device__ inline void forces_real(const KParams &kp, const CV &c, const PV &ip,
-                                   const MV &im, const NB &nb, LA AUX, uint32_t ci, uint32_t ipx,
-                                   uint32_t imx, uint32_t jp, uint32_t jm, uint32_t kp_,
-                                   uint32_t km, R *acc) {
+                                   const MV &im, const NB &nb, LA AUX, IX nbi, R *acc) {
     const R h1 = R(kp.h1), h2 = R(kp.h2);
     const R MU = R(kp.MU), KAPPA = R(kp.KAPPA), MASS2 = R(kp.MASS2), ETA = R(kp.ETA);
     const R KH = R(kp.KH), ACS = R(kp.ACS), BH = R(kp.BH), KG = R(kp.KG);
@@ -260,11 +260,6 @@ __device__ inline void forces_real(const KParams &kp, const CV &c, const PV &ip,
     auto nbv = [&](int f, int d, int sg) -> R {
         if (d == 0) return sg > 0 ? ip[f] : im[f];
         return nb.face(f, d, sg);
-    };
-    auto nbi = [&](int d, int sg) -> uint32_t {
-        if (d == 0) return sg > 0 ? ipx : imx;
-        if (d == 1) return sg > 0 ? jp : jm;
-        return sg > 0 ? kp_ : km;
     };
     auto curlF = [&](int f0, int a) -> R {
         if (a == 0)
@@ -349,7 +344,7 @@ __device__ inline void forces_real(const KParams &kp, const CV &c, const PV &ip,
         }
         R cp = curlF(0, a);
         R cos_th = 0;
-        if (ACS != 0) cos_th = R(2.0) * ACS * AUX(a, ci);
+        if (ACS != 0) cos_th = R(2.0) * ACS * AUX(a, nbi(3, 0));
         R hard_th = -BH * csq * c[TU + a];
         acc[TU + a] = lt - R(kp.MT2) * c[TU + a] + ETA * cp + cos_th + hard_th;
     }
@@ -393,7 +388,7 @@ __device__ inline void forces_real(const KParams &kp, const CV &c, const PV &ip,
             }
         }
         if (kp.SFREQ > 0) {                         // frequency mismatch (CU:1201-1229)
-            const R oc = AUX(6, ci);
+            const R oc = AUX(6, nbi(3, 0));
             R dw = 0;
 #pragma unroll
             for (int d = 0; d < 3; d++)

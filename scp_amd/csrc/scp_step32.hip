@@ -110,9 +110,16 @@ k_step32(KParams kp, Bufs32 b, int DI) {
             forces_gauge(kp, c0, cp, cm, nb, cy, acc);
         else if constexpr (MODE == 1)
             forces_complex(kp, c0, cp, cm, nb, acc);
-        else
-            forces_real(kp, c0, cp, cm, nb, AUX, ci, IX(ip1, j, k), IX(im1, j, k), jp, jm,
-                        kpx, kmx, acc);
+        else {
+            const uint32_t ipx = IX(ip1, j, k), imx = IX(im1, j, k);
+            const auto nbi = [&](int d, int sg) -> uint32_t {
+                if (d == 3) return ci;
+                if (d == 0) return sg > 0 ? ipx : imx;
+                if (d == 1) return sg > 0 ? jp : jm;
+                return sg > 0 ? kpx : kmx;
+            };
+            forces_real(kp, c0, cp, cm, nb, AUX, nbi, acc);
+        }
 
         if constexpr (KIND == K_FORCES) {
 #pragma unroll

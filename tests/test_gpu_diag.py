@@ -117,14 +117,13 @@ def test_fused_diag_matches_oracle(name):
 
 
 def test_fused_diag_other_grids_take_the_standalone_pass():
-    """Grids on the direct-neighbour kernel (real mode with the pass-1 fields,
-    step_kernel 1): step_diag = step + diag."""
-    for mode, sk in ((0, 0), (2, 1)):
+    """Grids on the direct-neighbour kernel (step_kernel 1): step_diag = step + diag."""
+    for mode, sk in ((0, 1), (2, 1)):
         N = 12
         L = 0.25 * (N - 1)
         st = seeded(N, max(mode, 1), L, link_sigma=0.05)
         st = st[:12] if mode == 0 else st
-        kw = dict(g=0.1) if mode == 2 else dict(alpha_cs=0.1)
+        kw = dict(g=0.1) if mode == 2 else {}
         d_f, s_f, d_s, s_s = _pair(G.make_params(N, mode, L, 0.1, step_kernel=sk, **kw), st, 4)
         assert np.array_equal(s_f, s_s)
         for k in G.DIAG_NAMES:
