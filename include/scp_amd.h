@@ -24,6 +24,8 @@
  *   scp_grid_gauss_project  init_gauss_project (scp_sim.c:2284-2400) on the GPU
  *   scp_grid_rescale_phi    rescale_phi_kernel (scp_sim.cu:2619-2624)
  *   scp_grid_set_kappa      the self-tune kappa update (scp_sim.cu:5520-5541)
+ *   scp_grid_set_damp_rate  the absorbing -> periodic switch at bc_switch_time
+ *                           (scp_sim.c:3073-3081, scp_sim.cu:5464-5490)
  *   scp_grid_vec_fit   fit_patch over the six fields of a vector frame
  *                      (scp_sim.c:2567-2640, 3106-3117)
  *
@@ -143,6 +145,12 @@ int  scp_grid_rescale_phi(scp_grid *g, double f);
 /* kappa of the potential for the following steps — the self-tune update
  * (cudaMemcpyToSymbol(d_KAPPA), scp_sim.cu:5531-5541); kappa > 0. */
 int  scp_grid_set_kappa(scp_grid *g, double kappa);
+/* damp_rate of the absorbing sphere (bc_type 0) for the following steps; rate
+ * >= 0, 0 turns the sponge off. The reference's bc_switch_time sets damp_rate
+ * = 0 before the step lround(bc_switch_time / dt) (c.damp_rate = 0.0,
+ * scp_sim.c:3077-3081; cudaMemcpyToSymbol(d_DAMP_RATE), scp_sim.cu:5485-5490):
+ * a caller switching there calls this after the previous step completed. */
+int  scp_grid_set_damp_rate(scp_grid *g, double rate);
 /* Steps completed since upload. */
 int64_t scp_grid_steps_done(const scp_grid *g);
 

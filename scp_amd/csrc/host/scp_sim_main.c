@@ -365,8 +365,20 @@ int main(int argc, char **argv) {
     int major = diag_every * 25; if (major < 1) major = 1;
     if (major > n_steps / 10) { major = (n_steps / 10 / diag_every) * diag_every; if (major < diag_every) major = diag_every; }
     int pending = 0;
+    /* absorbing -> periodic at bc_switch_time: damp_rate = 0 from that step on
+       (scp_sim.c:3073-3081, scp_sim.cu:5464-5490) */
+    const int bc_switch_step = (c.bc_switch_time > 0) ? (int)lround(c.bc_switch_time / dt) : 0;
+    int bc_switched = 0;
     for (int step = 1; step <= n_steps; step++) {
         double t = step * dt;
+        if (bc_switch_step > 0 && step == bc_switch_step && !bc_switched) {
+            if (pending && scp_grid_step(x.g, pending)) die_abi("step");   /* steps before the switch keep the sponge */
+            pending = 0;
+            if (scp_grid_set_damp_rate(x.g, 0.0)) die_abi("damp rate");
+            c.damp_rate = 0.0;
+            printf("\n*** BC SWITCH at t=%.1f: absorbing -> periodic ***\n\n", t);
+            bc_switched = 1;
+        }
         int in_burst = 0;
         if (burst_dur > 0) {
             double tr = t - c.burst_start;

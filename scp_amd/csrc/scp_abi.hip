@@ -673,6 +673,16 @@ extern "C" int scp_grid_set_kappa(scp_grid *g, double kappa) {
     return 0;
 }
 
+extern "C" int scp_grid_set_damp_rate(scp_grid *g, double rate) {
+    if (!g) return set_err(-1, "null grid");
+    if (!(rate >= 0)) return set_err(-1, "damp_rate=%g must be >= 0", rate);
+    g->p.damp_rate = rate;
+    g->kp.DR = rate;
+    // the sponge stage runs where make_kparams enables it: bc 0 with a positive width and rate
+    g->kp.damp = (g->p.bc_type == 0 && g->p.damp_width > 0 && rate > 0) ? 1 : 0;
+    return 0;
+}
+
 // Gauss projection on the device, same algorithm and stopping rule as the
 // reference (CG on -lap chi = G - Gbar, chi kept zero-mean, tol 1e-13*max(1,|b|max),
 // at most 20000 iterations), in FP64.

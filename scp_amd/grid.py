@@ -41,7 +41,7 @@ ABI_SYMBOLS = ["scp_abi_version", "scp_last_error", "scp_grid_bytes", "scp_grid_
                "scp_batch_tune", "scp_batch_set_config", "scp_batch_config", "scp_grid_seed_balls",
                "scp_server_create", "scp_server_destroy", "scp_server_submit", "scp_server_round",
                "scp_server_job", "scp_server_take", "scp_server_stats_get", "scp_server_now",
-               "scp_grid_vec_fit", "scp_grid_set_kappa"]
+               "scp_grid_vec_fit", "scp_grid_set_kappa", "scp_grid_set_damp_rate"]
 
 
 class ScpError(RuntimeError):
@@ -153,6 +153,7 @@ def lib():
         L.scp_grid_g_offset.argtypes = [VP]
         L.scp_grid_rescale_phi.argtypes = [VP, C.c_double]
         L.scp_grid_set_kappa.argtypes = [VP, C.c_double]
+        L.scp_grid_set_damp_rate.argtypes = [VP, C.c_double]
         L.scp_grid_steps_done.restype = C.c_int64
         L.scp_grid_steps_done.argtypes = [VP]
         L.scp_grid_time_steps.argtypes = [VP, C.c_int, C.POINTER(C.c_float)]
@@ -294,6 +295,9 @@ class Grid:
 
     def set_kappa(self, kappa):
         _check(lib().scp_grid_set_kappa(self._h, float(kappa)), "scp_grid_set_kappa")
+
+    def set_damp_rate(self, rate):
+        _check(lib().scp_grid_set_damp_rate(self._h, float(rate)), "scp_grid_set_damp_rate")
 
     @property
     def steps_done(self):

@@ -258,3 +258,46 @@ def test_cli_real_oscillon_64_bitwise_against_reference_binary(tmp_path):
     for a, b in zip(runs["ref"][0], runs["gpu"][0]):
         assert np.array_equal(a, b)
     compare_diag(runs["ref"][1], runs["gpu"][1])
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="the reference binary (oracle/_ref) is not built")
+@pytest.mark.parametrize("case", ["real_oscillon", "complex_ball"])
+def test_cli_bc_switch_against_reference_binary(tmp_path, case):
+    """bc_switch_time: the absorbing sphere (bc_type 0) is switched off before step
+    lround(bc_switch_time / dt) = 120 of 300 (scp_sim.c:3073-3081). 32^3 FP64,
+    real mode from the oscillon init or complex mode from one seeded ball: the
+    drop-in binary prints the reference's switch line, its three frames are
+    bitwise the reference binary's and every diag.tsv value matches to the
+    printed precision."""
+    from scp_amd import seeds
+    N, steps = 32, 300
+    L = 0.25 * (N - 1)
+    dt = 0.2 * 2 * L / (N - 1)
+    cfg0 = {"N": N, "L": L, "T": steps * dt, "dt_factor": 0.2, "precision": "f64", "bc_type": 0,
+            "damp_width": 4.0, "damp_rate": 0.05, "bc_switch_time": 120.3 * dt, "snap_dt": steps * dt / 2,
+            "diag_dt": 20 * dt}
+    if case == "real_oscillon":
+        cfg0["init"] = "oscillon"
+    else:
+        balls = [dict(x0=1.0, y0=0.0, z0=-0.5, amp=0.8, sigma=2.5, omega=0.9, phase=0.2, winding=1)]
+        st = seeds.seed_balls(N, L, 1, balls, theta_sigma=0.01, seed=5)
+        seed = tmp_path / "seed.sfa"
+        sfa.write_sim_sfa(seed, N, L, dt, [(0.0, st)])
+        cfg0.update({"complex_phi": 1, "init": "sfa", "init_sfa": str(seed)})
+    runs = {}
+    for name, binary in (("ref", REF), ("gpu", BIN)):
+        wd = tmp_path / name
+        wd.mkdir()
+        cfg = dict(cfg0, output=str(wd / "out.sfa"), diag_file=str(wd / "diag.tsv"))
+        (wd / "run.cfg").write_text("".join(f"{k} = {v}\n" for k, v in cfg.items()))
+        print(f"case bc switch {case}: {name} run", flush=True)
+        res = subprocess.run([binary, str(wd / "run.cfg")], capture_output=True, text=True, cwd=wd, timeout=300)
+        print(res.stdout[-400:], res.stderr[-400:], flush=True)
+        assert res.returncode == 0, name
+        assert f"*** BC SWITCH at t={120 * dt:.1f}: absorbing -> periodic ***" in res.stdout, name
+        with sfa.SFAFile(wd / "out.sfa") as f:
+            runs[name] = ([f.read_frame(i) for i in range(f.total_frames)], (wd / "diag.tsv").read_text())
+    assert len(runs["ref"][0]) == len(runs["gpu"][0]) == 3
+    for a, b in zip(runs["ref"][0], runs["gpu"][0]):
+        assert np.array_equal(a, b)
+    compare_diag(runs["ref"][1], runs["gpu"][1])

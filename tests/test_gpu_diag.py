@@ -96,6 +96,30 @@ def test_fused_diag_f32c(mode, N, bc, lo):
     print("relative differences of the neighbour terms:", {k: f"{v:.1e}" for k, v in worst.items()})
 
 
+@pytest.mark.parametrize("lo", [1, 4])
+def test_fused_diag_f32c_real_pass1(lo):
+    """Real mode with the pass-1 fields (alpha_cs, beta_h, sigma_freq: the MODE 3
+    march instance), FP32C: the fused diagnostics against the stand-alone pass
+    on the same emitted state, to the FP32C fused-diagnostics bar."""
+    N = 33
+    L, st, kw = _case(0, N, 2)
+    kw.update(alpha_cs=0.1, beta_h=0.2, sigma_freq=0.3, kappa_h=0.3)
+    print(f"case f32c real pass-1 N={N} lo={lo}", flush=True)
+    p = G.make_params(N, 0, L, 0.1, precision=G.PREC_F32C, lo_words=lo, **kw)
+    with G.Grid(p) as g:
+        assert g.step_kernel_name == f"k_march<scp::S32<{lo}>, 1, 3, false>", g.step_kernel_name
+    d_f, s_f, d_s, s_s = _pair(p, st, 9)
+    assert np.array_equal(s_f, s_s)
+    for k in G.DIAG_NAMES:
+        err = abs(d_f[k] - d_s[k])
+        # with alpha_cs, beta_h or kappa_h, E_pot holds curl terms of the neighbours
+        if (k in OWN_SITE and k != "E_pot") or k in EXACT:
+            assert err <= 1e-13 * max(1.0, abs(d_s[k])), f"{k}: fused {d_f[k]!r} stand-alone {d_s[k]!r}"
+        else:
+            assert err <= 2e-7 * max(abs(d_s[k]), 1e-3 * abs(d_s["E_total"])), \
+                f"{k}: fused {d_f[k]!r} stand-alone {d_s[k]!r}"
+
+
 @pytest.mark.parametrize("name", ["complex_n8", "complex_damp_n10", "gauged_n8", "gauged_damp_n10", "real_n8",
                                   "real_mode1_n8", "real_terms_n8"])
 def test_fused_diag_matches_oracle(name):

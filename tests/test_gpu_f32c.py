@@ -108,19 +108,30 @@ def test_f32c_march_and_direct_kernels_agree(lo):
     assert np.linalg.norm(outs[0] - outs[1]) / np.linalg.norm(a) <= 2e-5
 
 
+# real-mode terms of the pass-1 instance (MODE 3: alpha_cs, beta_h, sigma_freq read from an
+# auxiliary array) together with the register-only terms
+REAL_PASS1_ALL = dict(alpha_cs=0.1, beta_h=0.2, kappa_h=0.3, sigma_freq=0.3, theta_sat=0.05, gamma_conv=0.8,
+                      sigma_grad=0.4, chi_chiral=1.5)
+
+
+@pytest.mark.parametrize("case", ["base", "pass1_all"])
 @pytest.mark.parametrize("lo", [1, 4])
-def test_f32c_real_march_and_direct_kernels_agree(lo):
+def test_f32c_real_march_and_direct_kernels_agree(lo, case):
     """Real mode, FP32C: the march and the direct-neighbour kernel after 20
-    steps of a 24^3 state stay within the FP32C bar of FP64 and of each other."""
+    steps of a 24^3 state stay within the FP32C bar of FP64 and of each other;
+    with the pass-1 fields the march runs its MODE 3 instance."""
     N = 24
     L = 0.25 * (N - 1)
     st = ball_state(N, L)[:12].copy()
-    a, _, d64 = run(G.make_params(N, G.MODE_REAL, L, 0.2), st, 20, project=False)
+    kw = REAL_PASS1_ALL if case == "pass1_all" else {}
+    print(f"case real f32c lo={lo} {case}", flush=True)
+    a, _, d64 = run(G.make_params(N, G.MODE_REAL, L, 0.2, **kw), st, 20, project=False)
     outs = []
     for sk in (0, 1):
-        p = G.make_params(N, G.MODE_REAL, L, 0.2, precision=G.PREC_F32C, lo_words=lo, step_kernel=sk)
+        p = G.make_params(N, G.MODE_REAL, L, 0.2, precision=G.PREC_F32C, lo_words=lo, step_kernel=sk, **kw)
         with G.Grid(p) as g:
-            assert g.step_kernel_name.startswith("k_march" if sk == 0 else "k_step32"), g.step_kernel_name
+            want = f"k_march<scp::S32<{lo}>, 1, {3 if kw else 0}, false>" if sk == 0 else "k_step32<0"
+            assert g.step_kernel_name.startswith(want), g.step_kernel_name
         b, _, d = run(p, st, 20, project=False)
         outs.append(b)
         assert np.linalg.norm(b - a) / np.linalg.norm(a) <= 2e-5, sk

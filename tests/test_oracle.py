@@ -104,6 +104,49 @@ def test_oracle_vs_live_reference_binary(mode):
     assert np.array_equal(out, final)
 
 
+def _bc_switch_case(wd, extra=""):
+    """Complex 10^3 seed with a strong absorbing sphere, run through oracle/_ref
+    for 40 steps; returns (params, frame0, final, stdout)."""
+    N, L, steps = 10, 4.5, 40
+    rng = np.random.default_rng(77)
+    st = 0.3 * rng.standard_normal((24, N ** 3))
+    dx = 2 * L / (N - 1)
+    seed = os.path.join(wd, "seed.sfa")
+    sfa.write_sim_sfa(seed, N, L, 0.1 * dx, [(0.0, st)])
+    cfg = os.path.join(wd, "c.cfg")
+    T = steps * 0.1 * dx
+    with open(cfg, "w") as f:
+        f.write(f"N={N}\nL={L}\nT={T}\ndt_factor=0.1\ncomplex_phi=1\nbc_type=0\ndamp_width=2.0\n"
+                f"damp_rate=0.05\nprecision=f64\ninit=sfa\ninit_sfa={seed}\nsnap_dt={T}\ndiag_dt={T}\n"
+                f"output={wd}/o.sfa\ndiag_file={wd}/d.tsv\n{extra}")
+    res = subprocess.run([REF_BIN, cfg], capture_output=True, text=True, cwd=wd,
+                         env=dict(os.environ, OMP_NUM_THREADS="2"))
+    assert res.returncode == 0, res.stderr
+    with sfa.SFAFile(os.path.join(wd, "o.sfa")) as fo:
+        frame0, final = fo.read_frame(0), fo.read_frame(fo.total_frames - 1)
+    p = oracle.make_params(N, 1, L, 0.1, bc_type=0, damp_width=2.0, damp_rate=0.05)
+    return p, frame0, final, res.stdout
+
+
+@pytest.mark.skipif(not os.path.exists(REF_BIN), reason="reference binary not built here")
+def test_oracle_bc_switch_vs_live_reference_binary():
+    """bc_switch_time (scp_sim.c:3073-3081): the reference sets damp_rate = 0 before
+    step lround(bc_switch_time / dt) = 17. The oracle's 16 damped steps followed by
+    24 undamped ones (the stored acceleration carried across) are bitwise the
+    reference binary's 40 steps; without the switch the two runs differ."""
+    with tempfile.TemporaryDirectory() as wd:
+        p, frame0, final, out = _bc_switch_case(wd, "bc_switch_time=%r\n" % (17.4 * 0.1 * 2 * 4.5 / 9))
+    assert "*** BC SWITCH at t=" in out
+    st, a = oracle.step(p, frame0, 16)
+    p.damp_rate = 0.0
+    st, _ = oracle.step(p, st, 24, acc=a)
+    assert np.array_equal(st, final)
+    with tempfile.TemporaryDirectory() as wd:
+        _, _, final_damped, out = _bc_switch_case(wd)
+    assert "BC SWITCH" not in out
+    assert np.abs(final_damped - final).max() > 1e-6
+
+
 def test_q_conservation_eta0_periodic():
     """Noether charge is conserved to round-off at eta=0 (v66 qcons: 7e-14 relative)."""
     N, L = 16, 7.5
