@@ -120,6 +120,16 @@ int  scp_grid_diag(scp_grid *g, double *out, int nout);
  * run the step and the stand-alone pass. The first call on a gauged grid
  * allocates N^3 doubles of charge density. */
 int  scp_grid_step_diag(scp_grid *g, int nsteps, double *out, int nout);
+/* scp_grid_step_diag without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The work space of the fused diagnostics (per-block partial rows and, gauged,
+ * N^3 doubles of charge density) lives in a static 64 MiB device array that
+ * every GPU the library runs on holds until the process ends; grids needing
+ * more (gauged N > ~202) are rejected with an error. Results equal
+ * scp_grid_step_diag's. Every call on one GPU shares that GPU's array, so
+ * calls that overlap on one GPU (different host threads) race: serialise them. */
+int  scp_grid_step_diag_noalloc(scp_grid *g, int nsteps, double *out, int nout);
 /* Accelerations of the current positions: 6/12/15 columns in the order
  * phi_acc, theta_acc, [phi_im_acc, theta_im_acc], [E_acc]. */
 int  scp_grid_forces(scp_grid *g, double *const *acc, int nacc);

@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 using namespace scp;
@@ -561,16 +562,50 @@ extern "C" int scp_grid_diag(scp_grid *g, double *out, int nout) {
 // Gauss-law statistics, which need E at neighbours other blocks compute. Grids
 // whose step does not run the march (direct-neighbour kernel, gauged storage
 // with g = 0) take the step and the stand-alone pass.
+
+// the fused path's work space: partial rows (nbm march blocks + nbt Gauss-tail
+// blocks, DS_NSUM + DM_NMAX doubles each) and, gauged, N^3 doubles of charge density
+static bool fused_diag(const scp_grid *g) { return use_march(g, K_STEP) && g->step_mode == g->mode; }
+static void fused_rows(const scp_grid *g, int *nbm, int *nbt) {
+    *nbm = march_blocks(g->kp.N, 0);
+    *nbt = g->mode == 2 ? g->nblocks_diag : 0;
+}
+
+// the fused step + diagnostics over caller-provided work space (fpart: the
+// partial rows, every row written by its block before it is read; rho: N^3,
+// gauged, every site written by the march before the Gauss tail reads it)
+static int step_diag_run(scp_grid *g, int nsteps, double *out, int nout, double *fpart, double *rho) {
+    constexpr int NQ = DS_NSUM + DM_NMAX;
+    int nbm, nbt;
+    fused_rows(g, &nbm, &nbt);
+    DiagArgs da;
+    da.dp = diag_params(g);
+    da.rho = rho;
+    da.partials = fpart;
+    int rc = enqueue_steps(g, nsteps, &da);
+    if (rc) return rc;
+    if (nbt)
+        rc = kerr(launch_gauss_tail(g->kp, da.dp, g->vo, rho, fpart + (size_t)nbm * NQ, nbt, g->stream),
+                  "gauss tail");
+    if (rc) return rc;
+    rc = kerr(launch_reduce_wide(fpart, nbm + nbt, DS_NSUM, DM_NMAX, g->dout, g->stream), "reduce");
+    if (rc) return rc;
+    double r[NQ];
+    if ((rc = fetch(g, r, g->dout, NQ))) return rc;
+    return diag_finish(g, da.dp, r, r + DS_NSUM, out, nout);
+}
+
 extern "C" int scp_grid_step_diag(scp_grid *g, int nsteps, double *out, int nout) {
     if (!g || !out) return set_err(-1, "null argument");
     if (nsteps < 1) return set_err(-1, "nsteps=%d must be >= 1", nsteps);
     if (nout < SCP_D_COUNT) return set_err(-1, "nout=%d < %d", nout, SCP_D_COUNT);
-    if (!use_march(g, K_STEP) || g->step_mode != g->mode) {
+    if (!fused_diag(g)) {
         int rc = scp_grid_step(g, nsteps);
         return rc ? rc : scp_grid_diag(g, out, nout);
     }
     constexpr int NQ = DS_NSUM + DM_NMAX;
-    const int nbm = march_blocks(g->kp.N, 0), nbt = g->mode == 2 ? g->nblocks_diag : 0;
+    int nbm, nbt;
+    fused_rows(g, &nbm, &nbt);
     if (g->fpart_rows < nbm + nbt) {
         hipFree(g->fpart);
         g->fpart = nullptr;
@@ -579,21 +614,63 @@ extern "C" int scp_grid_step_diag(scp_grid *g, int nsteps, double *out, int nout
         g->fpart_rows = nbm + nbt;
     }
     if (g->mode == 2 && !g->rho) HIPCHK(hipMalloc(&g->rho, (size_t)g->kp.n3 * sizeof(double)));
-    DiagArgs da;
-    da.dp = diag_params(g);
-    da.rho = g->rho;
-    da.partials = g->fpart;
-    int rc = enqueue_steps(g, nsteps, &da);
+    return step_diag_run(g, nsteps, out, nout, g->fpart, g->rho);
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free variant: the same work space in a static device array.
+// Each device the library runs on holds one copy (SCRATCH_BYTES) until the
+// process ends; its address is looked up once per device, with that device
+// current, and only that address is used from the host.
+// ---------------------------------------------------------------------------
+constexpr size_t SCRATCH_BYTES = (size_t)64 << 20;
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+static std::mutex scratch_mu;
+static void *scratch_dev[SCRATCH_MAX_DEVICES];
+
+static int scratch_for(int dev, void **ptr) {
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES)
+        return set_err(-1, "device %d: the scratch table holds devices 0..%d", dev, SCRATCH_MAX_DEVICES - 1);
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        int cur = -1;
+        HIPCHK(hipGetDevice(&cur));
+        if (cur != dev) HIPCHK(hipSetDevice(dev));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) HIPCHK(hipSetDevice(cur));
+        if (e != hipSuccess || !p)
+            return set_err(-2, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev, hipGetErrorString(e));
+        scratch_dev[dev] = p;
+    }
+    *ptr = scratch_dev[dev];
+    return 0;
+}
+
+extern "C" int scp_grid_step_diag_noalloc(scp_grid *g, int nsteps, double *out, int nout) {
+    if (!g || !out) return set_err(-1, "null argument");
+    if (nsteps < 1) return set_err(-1, "nsteps=%d must be >= 1", nsteps);
+    if (nout < SCP_D_COUNT) return set_err(-1, "nout=%d < %d", nout, SCP_D_COUNT);
+    if (!fused_diag(g)) {   // step and stand-alone pass: grid-owned buffers only
+        int rc = scp_grid_step(g, nsteps);
+        return rc ? rc : scp_grid_diag(g, out, nout);
+    }
+    constexpr int NQ = DS_NSUM + DM_NMAX;
+    int nbm, nbt;
+    fused_rows(g, &nbm, &nbt);
+    const size_t rows = ((size_t)(nbm + nbt) * NQ * sizeof(double) + 255) / 256 * 256;
+    const size_t need = rows + (g->mode == 2 ? (size_t)g->kp.n3 * sizeof(double) : 0);
+    if (need > SCRATCH_BYTES)
+        return set_err(-1, "N=%d needs %zu bytes of diagnostics work space, above the %zu of the static "
+                           "scratch (gauged grids up to N^3 ~ 8.3e6 sites, N <= 202)",
+                       g->p.N, need, SCRATCH_BYTES);
+    void *base = nullptr;
+    int rc = scratch_for(g->device, &base);
     if (rc) return rc;
-    if (nbt)
-        rc = kerr(launch_gauss_tail(g->kp, da.dp, g->vo, g->rho, g->fpart + (size_t)nbm * NQ, nbt, g->stream),
-                  "gauss tail");
-    if (rc) return rc;
-    rc = kerr(launch_reduce_wide(g->fpart, nbm + nbt, DS_NSUM, DM_NMAX, g->dout, g->stream), "reduce");
-    if (rc) return rc;
-    double r[NQ];
-    if ((rc = fetch(g, r, g->dout, NQ))) return rc;
-    return diag_finish(g, da.dp, r, r + DS_NSUM, out, nout);
+    double *fpart = (double *)base;
+    double *rho = g->mode == 2 ? (double *)((char *)base + rows) : nullptr;
+    return step_diag_run(g, nsteps, out, nout, fpart, rho);
 }
 
 extern "C" int scp_grid_forces(scp_grid *g, double *const *acc, int nacc) {

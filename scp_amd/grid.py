@@ -41,7 +41,8 @@ ABI_SYMBOLS = ["scp_abi_version", "scp_last_error", "scp_grid_bytes", "scp_grid_
                "scp_batch_tune", "scp_batch_set_config", "scp_batch_config", "scp_grid_seed_balls",
                "scp_server_create", "scp_server_destroy", "scp_server_submit", "scp_server_round",
                "scp_server_job", "scp_server_take", "scp_server_stats_get", "scp_server_now",
-               "scp_grid_vec_fit", "scp_grid_set_kappa", "scp_grid_set_damp_rate"]
+               "scp_grid_vec_fit", "scp_grid_set_kappa", "scp_grid_set_damp_rate",
+               "scp_grid_step_diag_noalloc"]
 
 
 class ScpError(RuntimeError):
@@ -145,6 +146,7 @@ def lib():
         L.scp_grid_step.argtypes = [VP, C.c_int]
         L.scp_grid_diag.argtypes = [VP, C.POINTER(C.c_double), C.c_int]
         L.scp_grid_step_diag.argtypes = [VP, C.c_int, C.POINTER(C.c_double), C.c_int]
+        L.scp_grid_step_diag_noalloc.argtypes = [VP, C.c_int, C.POINTER(C.c_double), C.c_int]
         L.scp_grid_forces.argtypes = [VP, DPP, C.c_int]
         L.scp_grid_vec_fit.argtypes = [VP, C.c_int, C.POINTER(C.c_float), C.c_int64]
         L.scp_grid_gauss_project.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_double)]
@@ -262,6 +264,14 @@ class Grid:
         step launch where the grid runs the march (see scp_grid_step_diag)."""
         out = (C.c_double * len(DIAG_NAMES))()
         _check(lib().scp_grid_step_diag(self._h, int(n), out, len(DIAG_NAMES)), "scp_grid_step_diag")
+        return dict(zip(DIAG_NAMES, list(out)))
+
+    def step_diag_noalloc(self, n=1):
+        """step_diag with its work space in the library's static device scratch
+        (scp_grid_step_diag_noalloc: no device allocation)."""
+        out = (C.c_double * len(DIAG_NAMES))()
+        _check(lib().scp_grid_step_diag_noalloc(self._h, int(n), out, len(DIAG_NAMES)),
+               "scp_grid_step_diag_noalloc")
         return dict(zip(DIAG_NAMES, list(out)))
 
     def vec_fit(self, block_size=8):
