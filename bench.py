@@ -21,7 +21,8 @@ it as stored_bytes_per_site and stored_frac. traffic: measured HBM bytes per lau
 (rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a plain copy's counters), read
 from the counter summary profiles/hbm_traffic.txt when it holds a line for
 this kernel and size. cpu_baseline: the reference CPU binary built from source
-(oracle/_ref/scp_sim) on a bounded 64^3 gauged sample on this host.
+(oracle/_ref/scp_sim) on every core of this host, on bounded samples of the
+CPU-runnable shapes (gauged 128^3 two balls, complex 64^3 one ball).
 
 --mode real runs the same measurement on the real-mode step (6 fields: 12
 state scalars, 192 B FP64 / 96 B FP32C per site-update; the state is the real
@@ -88,38 +89,104 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(mode="gauged"):
+def host_cpu():
+    """(threads this process may run on, physical cores of the host, CPU model)."""
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    model, phys = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            cur = {}
+            for line in f.read().splitlines() + [""]:
+                if not line.strip():
+                    if "physical id" in cur and "core id" in cur:
+                        phys.add((cur["physical id"], cur["core id"]))
+                    cur = {}
+                    continue
+                k, _, v = line.partition(":")
+                cur[k.strip()] = v.strip()
+                if k.strip() == "model name":
+                    model = v.strip()
+    except OSError:
+        pass
+    return threads, len(phys) or threads, model
+
+
+def _ref_run(ref, wd, name, cfg, threads):
+    """One reference run; returns (ms/step from its 'Wall' line, process wall s)."""
+    path = os.path.join(wd, name + ".cfg")
+    cfg = dict(cfg, output=os.path.join(wd, name + ".sfa"), diag_file=os.path.join(wd, name + ".tsv"))
+    with open(path, "w") as f:
+        f.write("".join(f"{k} = {v}\n" for k, v in cfg.items()))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    t0 = time.time()
+    res = subprocess.run([ref, path], capture_output=True, text=True, env=env, cwd=wd, timeout=900)
+    wall = time.time() - t0
+    m = re.search(r"Wall: [\d.]+s \([\d.]+ min\) ([\d.]+)ms/step", res.stdout)
+    if res.returncode != 0 or not m:
+        log(f"cpu baseline {name} failed rc={res.returncode}: {res.stderr[-300:]}")
+        return None, wall
+    return float(m.group(1)), wall
+
+
+def cpu_baseline(mode="gauged", gauged_steps=300):
     """Time the reference CPU path (oracle/_ref/scp_sim, built from the reference
-    sources with its Makefile flags -O3 -fopenmp) on a 64^3 sample of the mode."""
+    sources with its Makefile flags -O3 -fopenmp) on every core this process may
+    use, on the CPU-runnable measurement shapes: the gauged 128^3 two-ball run
+    (periodic, g = 0.05, links N(0, 1e-3), each run's own Gauss-projecting init;
+    gauged_steps of its 2000 steps) and the complex 64^3 one-ball run (all 500
+    steps), with snap_dt >= T and diag_dt = T. value: the gauged rate (the
+    headline's mode); --mode real times the real-mode oscillon 64^3 instead."""
+    from scp_amd import sfa
+    from scp_amd import seeds
     ref = os.path.join(ROOT, "oracle", "_ref", "scp_sim")
     if not os.path.exists(ref):
         return None
-    cores = min(16, os.cpu_count() or 1)
-    N, L, dtf = 64, 15.75, 0.2
-    dt = dtf * (2 * L / (N - 1))
-    nsteps = 600 if mode == "gauged" else 1500
-    cfields = "complex_phi = 1\ncomplex_gauge = 1\ng_gauge = 0.05\n" if mode == "gauged" else ""
+    threads, phys, model = host_cpu()
+    base = {"dt_factor": 0.2, "precision": "f64", "bc_type": 2}
+    out = {"unit": "Gsite-updates/s", "cores": threads, "kind": "reference"}
+    host = f"{model}, {phys} physical cores, OMP_NUM_THREADS={threads}"
     with tempfile.TemporaryDirectory() as wd:
-        cfg = os.path.join(wd, "b.cfg")
-        with open(cfg, "w") as f:
-            f.write(f"N = {N}\nL = {L}\nT = {nsteps * dt}\ndt_factor = {dtf}\n{cfields}"
-                    f"bc_type = 2\ninit = oscillon\n"
-                    f"precision = f32\nsnap_dt = {10 * nsteps * dt}\ndiag_dt = {nsteps * dt}\n"
-                    f"output = {wd}/o.sfa\ndiag_file = {wd}/d.tsv\n")
-        env = dict(os.environ, OMP_NUM_THREADS=str(cores))
-        t0 = time.time()
-        res = subprocess.run([ref, cfg], capture_output=True, text=True, env=env, cwd=wd, timeout=600)
-        wall = time.time() - t0
-    m = re.search(r"Wall: [\d.]+s \([\d.]+ min\) ([\d.]+)ms/step", res.stdout)
-    if res.returncode != 0 or not m:
-        log(f"cpu baseline failed rc={res.returncode}")
-        return None
-    ms = float(m.group(1))
-    return {"value": N ** 3 / (ms * 1e-3) / 1e9, "unit": "Gsite-updates/s", "cores": cores,
-            "kind": "reference",
-            "sample": f"reference scp_sim (gcc -O3 -fopenmp), {mode} 64^3 bc_type=2, init=oscillon, "
-                      f"{nsteps} steps, OMP_NUM_THREADS={cores}, 'Wall' ms/step={ms:.2f}, "
-                      f"process wall {wall:.1f}s"}
+        if mode == "real":
+            N, steps = 64, 1500
+            L = 0.25 * (N - 1)
+            dt = 0.2 * 2 * L / (N - 1)
+            ms, wall = _ref_run(ref, wd, "real64", dict(base, N=N, L=L, T=steps * dt, init="oscillon",
+                                                        snap_dt=2 * steps * dt, diag_dt=steps * dt), threads)
+            if ms is None:
+                return None
+            out.update(value=N ** 3 / (ms * 1e-3) / 1e9,
+                       sample=f"reference scp_sim (gcc -O3 -fopenmp), real 64^3 oscillon, {steps} steps, "
+                              f"{host}, 'Wall' {ms:.2f} ms/step, process wall {wall:.1f}s")
+            return out
+        runs = {}
+        for name, N, steps, nb in (("gauged128", 128, gauged_steps, 2), ("complex64", 64, 500, 1)):
+            L = 0.25 * (N - 1)
+            dt = 0.2 * 2 * L / (N - 1)
+            if nb == 2:
+                balls = [dict(x0=-6.0, y0=0.0, z0=0.0, amp=0.8, sigma=3.0, omega=0.8, phase=0.0, winding=1),
+                         dict(x0=6.0, y0=0.0, z0=0.0, amp=0.8, sigma=3.0, omega=-0.8, phase=0.0, winding=-1)]
+                st = seeds.seed_balls(N, L, 2, balls, theta_sigma=0.01, link_sigma=1e-3, seed=1001)
+                extra = {"complex_phi": 1, "complex_gauge": 1, "g_gauge": 0.05}
+            else:
+                balls = [dict(x0=0.0, y0=0.0, z0=0.0, amp=0.8, sigma=3.0, omega=0.8, phase=0.0, winding=1)]
+                st = seeds.seed_balls(N, L, 1, balls, theta_sigma=0.01, seed=1000)
+                extra = {"complex_phi": 1}
+            seed = os.path.join(wd, name + "_seed.sfa")
+            sfa.write_sim_sfa(seed, N, L, dt, [(0.0, st)])
+            del st
+            ms, wall = _ref_run(ref, wd, name, dict(base, N=N, L=L, T=steps * dt, init="sfa", init_sfa=seed,
+                                                    snap_dt=2 * steps * dt, diag_dt=steps * dt, **extra), threads)
+            if ms is None:
+                return None
+            runs[name] = (N ** 3 / (ms * 1e-3) / 1e9, ms, steps, wall)
+    g, c = runs["gauged128"], runs["complex64"]
+    out.update(value=g[0],
+               sample=f"reference scp_sim (gcc -O3 -fopenmp) on {host}: gauged 128^3 two opposite balls, "
+                      f"periodic, g=0.05, {g[2]} of 2000 steps, 'Wall' {g[1]:.2f} ms/step (process {g[3]:.1f}s "
+                      f"incl. its CG init); complex 64^3 one ball, {c[2]} steps, 'Wall' {c[1]:.2f} ms/step "
+                      f"= {c[0]:.4f} Gsite-updates/s (process {c[3]:.1f}s)",
+               complex_64=c[0])
+    return out
 
 
 def copy_calibration(gib, reps=3):

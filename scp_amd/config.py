@@ -105,6 +105,20 @@ class Config:
         return v.decode() if isinstance(v, bytes) else v
 
 
+def do_init(cfg, mode):
+    """The initial state of an init mode (scp_do_init of libscp_io, the reference's
+    do_init, scp_init.h:489-499): (12/24/30, N^3) float64 in snapshot column order."""
+    import numpy as np
+    L = _lib()
+    L.scp_do_init.argtypes = [C.POINTER(ScpConfig), C.c_int, C.POINTER(C.c_double), C.c_char_p, C.c_int]
+    ncols = {0: 12, 1: 24, 2: 30}[mode]
+    st = np.zeros((ncols, cfg.N ** 3))
+    err = C.create_string_buffer(512)
+    if L.scp_do_init(C.byref(cfg.c), mode, st.ctypes.data_as(C.POINTER(C.c_double)), err, 512):
+        raise ValueError(err.value.decode())
+    return st
+
+
 def f64_to_f16(x):
     return _lib().scp_f64_to_f16(float(x))
 

@@ -301,3 +301,80 @@ def test_cli_bc_switch_against_reference_binary(tmp_path, case):
     for a, b in zip(runs["ref"][0], runs["gpu"][0]):
         assert np.array_equal(a, b)
     compare_diag(runs["ref"][1], runs["gpu"][1])
+
+
+# sfa/sim/verify_parity.cfg, the reference's own CPU/GPU parity config (output paths replaced)
+VERIFY_PARITY = dict(N=64, L=15.0, T=10.0, dt_factor=0.025, m=1.5, m_theta=0.0, eta=0.5, mu=-41.345, kappa=50.0,
+                     mode=0, damp_width=3.0, damp_rate=0.01, init="braid", A=0.8, A_bg=0.1, R_tube=3.0,
+                     ellip=0.3325, delta="0,3.0005,4.4325", precision="f64", snap_dt=10.0, diag_dt=2.0)
+
+
+def _both_binaries(tmp_path, cfg0, case):
+    runs = {}
+    for name, binary in (("ref", REF), ("gpu", BIN)):
+        wd = tmp_path / name
+        wd.mkdir()
+        cfg = dict(cfg0, output=str(wd / "out.sfa"), diag_file=str(wd / "diag.tsv"))
+        (wd / "run.cfg").write_text("".join(f"{k} = {v}\n" for k, v in cfg.items()))
+        print(f"case {case}: {name} run", flush=True)
+        res = subprocess.run([binary, str(wd / "run.cfg")], capture_output=True, text=True, cwd=wd, timeout=300)
+        print(res.stdout[-400:], res.stderr[-400:], flush=True)
+        assert res.returncode == 0, name
+        with sfa.SFAFile(wd / "out.sfa") as f:
+            runs[name] = ([f.read_frame(i) for i in range(f.total_frames)], (wd / "diag.tsv").read_text())
+    return runs
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="the reference binary (oracle/_ref) is not built")
+def test_cli_verify_parity_cfg_against_reference_binary(tmp_path):
+    """The reference's CPU/GPU parity config (real mode, braid init, 64^3, absorbing
+    sphere, 840 steps, f64): frames bitwise the reference binary's, every diag.tsv
+    value to the printed precision."""
+    runs = _both_binaries(tmp_path, VERIFY_PARITY, "verify_parity.cfg")
+    assert len(runs["ref"][0]) == len(runs["gpu"][0]) == 2
+    for a, b in zip(runs["ref"][0], runs["gpu"][0]):
+        assert np.array_equal(a, b)
+    compare_diag(runs["ref"][1], runs["gpu"][1])
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="the reference binary (oracle/_ref) is not built")
+@pytest.mark.parametrize("case", ["qball2_complex", "qball3_gauged_bc0", "qball4_gauged_pair", "template", "exec"])
+def test_cli_init_modes_against_reference_binary(tmp_path, case):
+    """init=qball (2/3/4-column profiles), template and exec, 24^3, 60 steps through
+    both binaries: frames bitwise (real and complex) or within 1e-12 relative
+    (gauged: this build's own link sincos), diag.tsv to the printed precision."""
+    from tests.test_init import write_profile
+    N, steps = 24, 60
+    L = 0.25 * (N - 1)
+    dt = 0.1 * 2 * L / (N - 1)
+    cfg = dict(N=N, L=L, T=steps * dt, dt_factor=0.1, precision="f64", snap_dt=steps * dt / 2, diag_dt=20 * dt)
+    gauged = "gauged" in case
+    if case.startswith("qball"):
+        prof = tmp_path / "prof.txt"
+        write_profile(prof, int(case[5]))
+        cfg.update(complex_phi=1, init="qball", qball_profile=str(prof), qball_omega=1.35, qball_x0=0.7,
+                   qball_y0=-0.4, qball_z0=0.2)
+        if "pair" in case:
+            cfg.update(qball2_x0=-2.1, qball2_y0=1.0, qball2_z0=-0.3, qball2_sign=-1, qball2_phase=0.9, bc_type=2)
+        if gauged:
+            cfg.update(complex_gauge=1, g_gauge=0.05)
+        if "bc0" in case:
+            cfg["bc_type"] = 0
+    elif case == "template":
+        rng = np.random.default_rng(3)
+        tpath = tmp_path / "tmpl.sfa"
+        sfa.write_sim_sfa(tpath, 12, 3.0, 0.05, [(0.0, 0.2 * rng.standard_normal((12, 12 ** 3)))])
+        cfg.update(init="template", init_sfa=str(tpath), A_bg=0.1, bc_type=2)
+    else:
+        st = 0.3 * np.random.default_rng(4).standard_normal((24, N ** 3))
+        seed = tmp_path / "gen.sfa"
+        sfa.write_sim_sfa(seed, N, L, dt, [(0.0, st)])
+        cfg.update(complex_phi=1, init="exec", init_exec=f"cat {seed}", bc_type=2)
+    runs = _both_binaries(tmp_path, cfg, case)
+    assert len(runs["ref"][0]) == len(runs["gpu"][0]) == 3
+    for a, b in zip(runs["ref"][0], runs["gpu"][0]):
+        if gauged:
+            assert np.linalg.norm(a - b) / np.linalg.norm(a) <= 1e-12
+        else:
+            assert np.array_equal(a, b)
+    compare_diag(runs["ref"][1], runs["gpu"][1])
