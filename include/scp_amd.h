@@ -28,6 +28,8 @@
  *                           (scp_sim.c:3073-3081, scp_sim.cu:5464-5490)
  *   scp_grid_vec_fit   fit_patch over the six fields of a vector frame
  *                      (scp_sim.c:2567-2640, 3106-3117)
+ *   scp_qflow_*        the fixed-charge gradient flow of the seed relaxer
+ *                      sfa/seed/eta_qflow.c:242-375 (fixed-Q mode)
  *
  * Column order of every host state buffer = the reference snapshot order
  * (scp_sim.c:2862-2897): phi_x..z, theta_x..z, phi_vx..vz, theta_vx..vz,
@@ -273,6 +275,44 @@ int  scp_server_job(const scp_server *s, int64_t id, scp_job_info *info);
 int  scp_server_take(scp_server *s, int64_t id, double *const *cols, int ncols, double *diag, int ndiag);
 int  scp_server_stats_get(const scp_server *s, scp_server_stats *st);
 double scp_server_now(const scp_server *s);
+
+/* Fixed-charge gradient flow: the relaxer sfa/seed/eta_qflow.c (its fixed-Q
+ * mode, eta_qflow.c:242-375) on the device. A complex-mode state of 12 fields
+ * (u_a, v_a = Re, Im Phi_a; tu_a, tv_a = Re, Im Theta_a; a = 0..2) flows by
+ * x <- x + dtau F with F the fixed-charge forces (7-point Laplacian, central
+ * curl, periodic), omega = Q / N recomputed from the current state every
+ * iteration (N = int sum_a |Phi_a|^2 + |Theta_a|^2). The seed (profile, Theta
+ * partner, winding) is built on the host (libscp_io: qflow_seed.h). */
+typedef struct scp_qflow scp_qflow;
+
+typedef struct {
+    int32_t N;            /* sites per edge (periodic cube) */
+    int32_t reserved;     /* 0 */
+    double L;             /* dx = 2L/(N-1) */
+    double eta, m, mth, mu, kappa;
+} scp_qflow_params;
+
+/* One printed row of the relaxer (eta_qflow.c:370): iteration, multiplier
+ * omega = Q/N of that iteration, E_Q of the updated state, max |F| over the
+ * u and tu forces, s_max of the updated state, N of that iteration. */
+typedef struct {
+    int32_t iter;
+    int32_t reserved;
+    double omega, E_Q, maxF, s_max, N;
+} scp_qflow_row;
+
+int  scp_qflow_create(const scp_qflow_params *p, scp_qflow **out);
+int  scp_qflow_destroy(scp_qflow *f);
+/* ncols = 12: u0..2, v0..2, tu0..2, tv0..2 (site i*N*N + j*N + k) */
+int  scp_qflow_upload(scp_qflow *f, const double *const *cols, int ncols);
+int  scp_qflow_download(scp_qflow *f, double *const *cols, int ncols);
+/* int sum_a |Phi_a|^2 and int sum_a |Theta_a|^2 of the current state (times dV). */
+int  scp_qflow_norm(scp_qflow *f, double *n_phi, double *n_theta);
+/* Iterations it = 0..iters at charge Q, step dtau; rows for it % printev == 0
+ * and it == iters (the first maxrows kept, *nrows = all); *omega_last: the
+ * multiplier of the last iteration (the rotating seed's velocities use it). */
+int  scp_qflow_run(scp_qflow *f, double Q, double dtau, int iters, int printev, scp_qflow_row *rows, int maxrows,
+                   int *nrows, double *omega_last);
 
 #ifdef __cplusplus
 }

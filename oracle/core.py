@@ -35,6 +35,11 @@ class OrParams(C.Structure):
                 ("sigma_cross", C.c_double), ("lambda_self", C.c_double)]
 
 
+class OrQflow(C.Structure):
+    _fields_ = [("N", C.c_int), ("L", C.c_double), ("eta", C.c_double), ("m", C.c_double),
+                ("mth", C.c_double), ("mu", C.c_double), ("kappa", C.c_double)]
+
+
 def make_params(N, mode, L, dt_factor, **kw):
     """Parameters with the reference defaults (scp_config.h:165-215)."""
     dx = 2.0 * L / (N - 1)
@@ -59,6 +64,10 @@ def _lib():
     lib.or_net_charge.argtypes = [P, D, D]
     lib.or_gauss_project.argtypes = [P, D, D]
     lib.or_gauge_transform.argtypes = [P, D]
+    Q = C.POINTER(OrQflow)
+    lib.or_qflow_norm.restype = C.c_double
+    lib.or_qflow_norm.argtypes = [Q, D]
+    lib.or_qflow.argtypes = [Q, D, C.c_double, C.c_double, C.c_int, C.c_int, D, C.c_int, D]
     return lib
 
 
@@ -235,3 +244,24 @@ def self_tune_run(p, state, n_steps, st_every, m2, eps=0.04, gamma=0.12, pcrit=2
             p.kappa = max(p.kappa, 1e-3)
             events.append((done * p.dt, p.kappa, Pmax, Q, "COLLAPSE" if collapse else "stable"))
     return st, events
+
+
+def qflow_params(N, L, eta=0.5, m=1.5, mth=0.0, mu=-41.345, kappa=50.0):
+    return OrQflow(N=N, L=L, eta=eta, m=m, mth=mth, mu=mu, kappa=kappa)
+
+
+def qflow_norm(p, state):
+    st = np.ascontiguousarray(state, dtype=np.float64).reshape(12, -1)
+    return lib().or_qflow_norm(C.byref(p), _ptr(st))
+
+
+def qflow(p, state, Q, dtau, iters, printev):
+    """Fixed-charge flow (sfa/seed/eta_qflow.c): returns (state, rows, omega_last);
+    rows: list of (it, omega, E_Q, maxF, s_max, N)."""
+    st = np.array(state, dtype=np.float64).reshape(12, -1).copy()
+    maxrows = iters // max(1, printev) + 2
+    rows = np.zeros((maxrows, 6))
+    om = C.c_double()
+    n = lib().or_qflow(C.byref(p), _ptr(st), float(Q), float(dtau), int(iters), int(printev), _ptr(rows),
+                       maxrows, C.byref(om))
+    return st, [tuple(r) for r in rows[:n]], om.value

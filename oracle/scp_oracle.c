@@ -19,6 +19,10 @@
  *   or_gauss_project   — post-condition max|G-Gbar| <= 1e-12 and the
  *                        reference's frame-0 E field.
  *   or_gauge_transform — gauge invariance of |acc|^2, E_acc, G (<= 1e-12).
+ *   or_qflow           — the fixed-charge flow of sfa/seed/eta_qflow.c: its f32
+ *                        output and every printed row, bitwise, against that
+ *                        program built from source (oracle/_ref/eta_qflow,
+ *                        one thread, so its reductions run in site order).
  */
 #include "scp_oracle.h"
 
@@ -896,4 +900,129 @@ void or_gauge_transform(const OrParams *p, double *state) {
         }
     }
     free(al);
+}
+
+
+/* ------------------------------------------------------------------------ */
+/* fixed-charge gradient flow, sfa/seed/eta_qflow.c (plain fixed-Q mode)     */
+/* ------------------------------------------------------------------------ */
+
+double or_qflow_norm(const OrQflow *p, const double *st) {
+    const long N3 = (long)p->N * p->N * p->N;
+    const double dx = 2.0 * p->L / (p->N - 1), dV = dx * dx * dx;
+    const double *u = st, *v = st + 3 * N3, *tu = st + 6 * N3, *tv = st + 9 * N3;
+    double Nn = 0;
+    for (long ii = 0; ii < N3; ii++)                       /* eta_qflow.c:193-197 */
+        for (int a = 0; a < 3; a++) {
+            const long q = a * N3 + ii;
+            Nn += u[q] * u[q] + v[q] * v[q] + tu[q] * tu[q] + tv[q] * tv[q];
+        }
+    return Nn * dV;
+}
+
+int or_qflow(const OrQflow *p, double *st, double Q, double dtau, int iters, int printev, double *rows,
+             int maxrows, double *omega_last) {
+    const int N = p->N;
+    const long N3 = (long)N * N * N, NN = (long)N * N;
+    const double L = p->L, dx = 2.0 * L / (N - 1), dx2 = dx * dx, dV = dx * dx * dx;
+    const double invdx2 = 1.0 / dx2, inv2dx = 1.0 / (2.0 * dx);
+    const double eta = p->eta, m = p->m, mth = p->mth, mu = p->mu, kappa = p->kappa;
+    double *nx = malloc(sizeof(double) * 12 * (size_t)N3);
+    double *cur = st, *nxt = nx;
+    int nrows = 0;
+    double omega = 0;
+#define WR(a) (((a) + N) % N)
+#define IX3(i, j, k) (((long)(i) * N + (j)) * N + (k))
+    for (int it = 0; it <= iters; it++) {
+        const double *u = cur, *v = cur + 3 * N3, *tu = cur + 6 * N3, *tv = cur + 9 * N3;
+        double Nn = 0;                                      /* eta_qflow.c:269-274 */
+        for (long ii = 0; ii < N3; ii++)
+            for (int a = 0; a < 3; a++) {
+                const long q = a * N3 + ii;
+                Nn += u[q] * u[q] + v[q] * v[q] + tu[q] * tu[q] + tv[q] * tv[q];
+            }
+        Nn *= dV;
+        omega = Q / Nn;
+        const double m2w = m * m - omega * omega, mt2w = mth * mth - omega * omega;
+        double maxF = 0;
+        for (long ii = 0; ii < N3; ii++) {                 /* eta_qflow.c:278-330 */
+            const int i = (int)(ii / NN), j = (int)((ii / N) % N), k = (int)(ii % N);
+            const long X[2][3] = {{IX3(WR(i + 1), j, k), IX3(i, WR(j + 1), k), IX3(i, j, WR(k + 1))},
+                                  {IX3(WR(i - 1), j, k), IX3(i, WR(j - 1), k), IX3(i, j, WR(k - 1))}};
+            const double r0 = u[ii] * u[ii] + v[ii] * v[ii];
+            const double r1 = u[N3 + ii] * u[N3 + ii] + v[N3 + ii] * v[N3 + ii];
+            const double r2 = u[2 * N3 + ii] * u[2 * N3 + ii] + v[2 * N3 + ii] * v[2 * N3 + ii];
+            const double s = r0 * r1 * r2, den = 1.0 + kappa * s;
+            const double Vtp = 0.5 * mu / (den * den);
+            const double Pa[3] = {r1 * r2, r0 * r2, r0 * r1};
+            for (int a = 0; a < 3; a++) {
+                const double *ua = u + a * N3, *va = v + a * N3, *ta = tu + a * N3, *sa = tv + a * N3;
+#define LAP(f) (((f)[X[0][0]] + (f)[X[1][0]] + (f)[X[0][1]] + (f)[X[1][1]] + (f)[X[0][2]] + (f)[X[1][2]] - 6.0 * (f)[ii]) * invdx2)
+#define DRV(f, ax, comp) ((f)[(comp) * N3 + X[0][ax]] - (f)[(comp) * N3 + X[1][ax]])
+                const double lu = LAP(ua), lv = LAP(va), ltu = LAP(ta), ltv = LAP(sa);
+                const int b = (a + 1) % 3, c = (a + 2) % 3;
+                const double curl_tu = (DRV(tu, b, c) - DRV(tu, c, b)) * inv2dx;
+                const double curl_tv = (DRV(tv, b, c) - DRV(tv, c, b)) * inv2dx;
+                const double curl_u = (DRV(u, b, c) - DRV(u, c, b)) * inv2dx;
+                const double curl_v = (DRV(v, b, c) - DRV(v, c, b)) * inv2dx;
+                const double Fu = lu - m2w * ua[ii] - 2.0 * Vtp * ua[ii] * Pa[a] + eta * curl_tu;
+                const double Fv = lv - m2w * va[ii] - 2.0 * Vtp * va[ii] * Pa[a] + eta * curl_tv;
+                const double Ftu = ltu - mt2w * ta[ii] + eta * curl_u;
+                const double Ftv = ltv - mt2w * sa[ii] + eta * curl_v;
+                nxt[a * N3 + ii] = ua[ii] + dtau * Fu;
+                nxt[(3 + a) * N3 + ii] = va[ii] + dtau * Fv;
+                nxt[(6 + a) * N3 + ii] = ta[ii] + dtau * Ftu;
+                nxt[(9 + a) * N3 + ii] = sa[ii] + dtau * Ftv;
+                double af = fabs(Fu);
+                if (af > maxF) maxF = af;
+                af = fabs(Ftu);
+                if (af > maxF) maxF = af;
+            }
+        }
+        double *t = cur; cur = nxt; nxt = t;
+        if (it % printev == 0 || it == iters) {               /* eta_qflow.c:336-373 */
+            const double *U = cur, *V = cur + 3 * N3, *TU = cur + 6 * N3, *TV = cur + 9 * N3;
+            double Eg = 0, Em = 0, Ep = 0, Ex = 0, smax = 0;
+            for (long ii = 0; ii < N3; ii++) {
+                const int i = (int)(ii / NN), j = (int)((ii / N) % N), k = (int)(ii % N);
+                const long X[2][3] = {{IX3(WR(i + 1), j, k), IX3(i, WR(j + 1), k), IX3(i, j, WR(k + 1))},
+                                      {IX3(WR(i - 1), j, k), IX3(i, WR(j - 1), k), IX3(i, j, WR(k - 1))}};
+                const double r0 = U[ii] * U[ii] + V[ii] * V[ii];
+                const double r1 = U[N3 + ii] * U[N3 + ii] + V[N3 + ii] * V[N3 + ii];
+                const double r2 = U[2 * N3 + ii] * U[2 * N3 + ii] + V[2 * N3 + ii] * V[2 * N3 + ii];
+                const double s = r0 * r1 * r2;
+                if (s > smax) smax = s;
+                Ep += 0.5 * mu * s / (1.0 + kappa * s);
+                for (int a = 0; a < 3; a++) {
+                    const double *F4[4] = {U + a * N3, V + a * N3, TU + a * N3, TV + a * N3};
+                    for (int q = 0; q < 4; q++) {
+                        const double *f = F4[q];
+                        const double gx = (f[X[0][0]] - f[X[1][0]]) * inv2dx, gy = (f[X[0][1]] - f[X[1][1]]) * inv2dx;
+                        const double gz = (f[X[0][2]] - f[X[1][2]]) * inv2dx;
+                        Eg += 0.5 * (gx * gx + gy * gy + gz * gz);
+                    }
+                    Em += 0.5 * m * m * (U[a * N3 + ii] * U[a * N3 + ii] + V[a * N3 + ii] * V[a * N3 + ii])
+                        + 0.5 * mth * mth * (TU[a * N3 + ii] * TU[a * N3 + ii] + TV[a * N3 + ii] * TV[a * N3 + ii]);
+                    const int b = (a + 1) % 3, c = (a + 2) % 3;
+                    const double curl_u = (DRV(U, b, c) - DRV(U, c, b)) * inv2dx;
+                    const double curl_v = (DRV(V, b, c) - DRV(V, c, b)) * inv2dx;
+                    Ex += -eta * (TU[a * N3 + ii] * curl_u + TV[a * N3 + ii] * curl_v);
+                }
+            }
+            const double EQ = (Eg + Em + Ep + Ex) * dV + Q * Q / (2.0 * Nn);
+            if (nrows < maxrows) {
+                double *r = rows + 6 * nrows;
+                r[0] = it; r[1] = omega; r[2] = EQ; r[3] = maxF; r[4] = smax; r[5] = Nn;
+            }
+            nrows++;
+        }
+#undef LAP
+#undef DRV
+    }
+#undef WR
+#undef IX3
+    if (cur != st) memcpy(st, cur, sizeof(double) * 12 * (size_t)N3);
+    free(nx);
+    if (omega_last) *omega_last = omega;
+    return nrows;
 }

@@ -69,6 +69,21 @@ double or_net_charge(const OrParams *p, const double *state, double *q_abs);
 int    or_gauss_project(OrParams *p, double *state, double *gauss_max0);
 void   or_gauge_transform(const OrParams *p, double *state);
 
+/* Fixed-charge gradient flow (sfa/seed/eta_qflow.c, plain fixed-Q mode).
+ * state: 12 N^3 doubles u0..2, v0..2, tu0..2, tv0..2 (site i*N*N + j*N + k). */
+typedef struct {
+    int    N;
+    double L, eta, m, mth, mu, kappa;
+} OrQflow;
+/* N = int sum_a (|phi_a|^2 + |theta_a|^2), summed in site order (eta_qflow.c:193-197) */
+double or_qflow_norm(const OrQflow *p, const double *state);
+/* Iterations it = 0..iters (eta_qflow.c:242-375): omega = Q/N, Jacobi step
+ * x += dtau F. A row {it, omega, E_Q, maxF, s_max, N} for it % printev == 0 and
+ * it == iters, at most maxrows; returns the row count; *omega_last = the
+ * multiplier of the last iteration (the one the output velocities use). */
+int    or_qflow(const OrQflow *p, double *state, double Q, double dtau, int iters, int printev, double *rows,
+                int maxrows, double *omega_last);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,7 +42,8 @@ ABI_SYMBOLS = ["scp_abi_version", "scp_last_error", "scp_grid_bytes", "scp_grid_
                "scp_server_create", "scp_server_destroy", "scp_server_submit", "scp_server_round",
                "scp_server_job", "scp_server_take", "scp_server_stats_get", "scp_server_now",
                "scp_grid_vec_fit", "scp_grid_set_kappa", "scp_grid_set_damp_rate",
-               "scp_grid_step_diag_noalloc"]
+               "scp_grid_step_diag_noalloc", "scp_qflow_create", "scp_qflow_destroy", "scp_qflow_upload",
+               "scp_qflow_download", "scp_qflow_norm", "scp_qflow_run"]
 
 
 class ScpError(RuntimeError):
@@ -100,6 +101,17 @@ class ServerStats(C.Structure):
 
 
 JOB_FAILED, JOB_QUEUED, JOB_ACTIVE, JOB_DONE = -1, 0, 1, 2
+
+
+class QflowParams(C.Structure):
+    """scp_qflow_params of include/scp_amd.h."""
+    _fields_ = [("N", C.c_int32), ("reserved", C.c_int32), ("L", C.c_double), ("eta", C.c_double),
+                ("m", C.c_double), ("mth", C.c_double), ("mu", C.c_double), ("kappa", C.c_double)]
+
+
+class QflowRow(C.Structure):
+    _fields_ = [("iter", C.c_int32), ("reserved", C.c_int32), ("omega", C.c_double), ("E_Q", C.c_double),
+                ("maxF", C.c_double), ("s_max", C.c_double), ("N", C.c_double)]
 
 
 def make_params(N, mode, L, dt_factor, precision=PREC_F64, **kw):
@@ -183,6 +195,13 @@ def lib():
         L.scp_server_stats_get.argtypes = [VP, C.POINTER(ServerStats)]
         L.scp_server_now.restype = C.c_double
         L.scp_server_now.argtypes = [VP]
+        L.scp_qflow_create.argtypes = [C.POINTER(QflowParams), C.POINTER(VP)]
+        L.scp_qflow_destroy.argtypes = [VP]
+        L.scp_qflow_upload.argtypes = [VP, DPP, C.c_int]
+        L.scp_qflow_download.argtypes = [VP, DPP, C.c_int]
+        L.scp_qflow_norm.argtypes = [VP, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.scp_qflow_run.argtypes = [VP, C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(QflowRow), C.c_int,
+                                    IP, C.POINTER(C.c_double)]
         _typed = True
     return L
 
@@ -450,3 +469,55 @@ class Server:
 
     def __exit__(self, *a):
         self.close()
+
+
+class QFlow:
+    """The fixed-charge gradient flow on the device (scp_qflow_* of include/scp_amd.h):
+    the relaxer sfa/seed/eta_qflow.c in its fixed-Q mode. State: (12, N^3) rows
+    u0..2, v0..2, tu0..2, tv0..2."""
+
+    def __init__(self, N, L, eta=0.5, m=1.5, mth=0.0, mu=-41.345, kappa=50.0):
+        self.N = N
+        self._h = C.c_void_p()
+        p = QflowParams(N=N, reserved=0, L=L, eta=eta, m=m, mth=mth, mu=mu, kappa=kappa)
+        _check(lib().scp_qflow_create(C.byref(p), C.byref(self._h)), "scp_qflow_create")
+
+    def close(self):
+        if self._h:
+            lib().scp_qflow_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, state):
+        st = np.ascontiguousarray(np.asarray(state, dtype=np.float64).reshape(12, self.N ** 3))
+        _check(lib().scp_qflow_upload(self._h, _col_ptrs(st), 12), "scp_qflow_upload")
+
+    def download(self):
+        st = np.empty((12, self.N ** 3))
+        _check(lib().scp_qflow_download(self._h, _col_ptrs(st), 12), "scp_qflow_download")
+        return st
+
+    def norm(self):
+        a, b = C.c_double(), C.c_double()
+        _check(lib().scp_qflow_norm(self._h, C.byref(a), C.byref(b)), "scp_qflow_norm")
+        return a.value, b.value
+
+    def run(self, Q, dtau, iters, printev):
+        """Returns (rows [(it, omega, E_Q, maxF, s_max, N)], omega of the last iteration)."""
+        maxrows = iters // max(1, printev) + 2
+        rows = (QflowRow * maxrows)()
+        n, om = C.c_int(), C.c_double()
+        _check(lib().scp_qflow_run(self._h, float(Q), float(dtau), int(iters), int(printev), rows, maxrows,
+                                   C.byref(n), C.byref(om)), "scp_qflow_run")
+        return [(r.iter, r.omega, r.E_Q, r.maxF, r.s_max, r.N) for r in rows[:n.value]], om.value

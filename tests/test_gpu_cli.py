@@ -355,7 +355,7 @@ def test_cli_init_modes_against_reference_binary(tmp_path, case):
         cfg.update(complex_phi=1, init="qball", qball_profile=str(prof), qball_omega=1.35, qball_x0=0.7,
                    qball_y0=-0.4, qball_z0=0.2)
         if "pair" in case:
-            cfg.update(qball2_x0=-2.1, qball2_y0=1.0, qball2_z0=-0.3, qball2_sign=-1, qball2_phase=0.9, bc_type=2)
+            cfg.update(qball2_x0=-2.1, qball2_y0=1.0, qball2_z0=-0.3, qball2_sign=-1, qball2_phase=0.9, bc_type=0)
         if gauged:
             cfg.update(complex_gauge=1, g_gauge=0.05)
         if "bc0" in case:

@@ -595,3 +595,37 @@ def test_self_tune_restatement_closed_forms(ptarget):
         assert {e[4] for e in ev} == {"COLLAPSE", "stable"}
         d = oracle.diag(p, fin)
         assert abs(2 * d["E_mass"] / p.m2 - ev[0][3]) <= 1e-12 * ev[0][3]
+
+
+QFLOW_REF = os.path.join(os.path.dirname(REF_BIN), "eta_qflow")
+
+
+@pytest.mark.skipif(not os.path.exists(QFLOW_REF), reason="reference eta_qflow not built here")
+@pytest.mark.parametrize("case", ["ball_bvp", "ring_wind1", "ball_zero_theta"])
+def test_oracle_qflow_vs_reference_program(case):
+    """The fixed-charge flow: this build's host seed (libscp_io scp_qflow_seed) and
+    the oracle's flow against sfa/seed/eta_qflow.c built from source, run on one
+    thread (its reductions then run in site order, as the oracle's): every printed
+    row (omega, E_Q, maxF, s_max, N) is the same text, and the f32 output seed is
+    bitwise the oracle state's columns."""
+    from scp_amd import seeds
+    from tests import qflow_cases as qc
+    N, L, om0, iters, pev = 20, 5.0, 1.35, 90, 30
+    kw = dict(ring_wind1=dict(wind=1, wtor=2.0), ball_zero_theta=dict(thinit="zero")).get(case, {})
+    with tempfile.TemporaryDirectory() as wd:
+        prof = os.path.join(wd, "prof.txt")
+        qc.write_profile(prof)
+        out = os.path.join(wd, "seed.sfa")
+        text, rows = qc.run(qc.argv(QFLOW_REF, N, L, prof, om0, out, iters, pev, **kw))
+        with sfa.SFAFile(out) as f:
+            ref_cols = f.read_frame(0)
+        st, gmax = seeds.qflow_seed(N, L, prof, om0, qc.FLOW["eta"], qc.FLOW["mth"], **kw)
+    p = oracle.qflow_params(N, L, **qc.FLOW)
+    n0 = oracle.qflow_norm(p, st)
+    assert f"N_init={n0:.4f}  Q={om0 * n0:.4f}" in text and f"g_max={gmax:.4e}" in text
+    dx = 2 * L / (N - 1)
+    fin, orows, om_last = oracle.qflow(p, st, om0 * n0, 0.04 * dx * dx, iters, pev)
+    assert [r[0] for r in orows] == [0, 30, 60, 90]
+    assert [list(r) for r in rows] == [qc.fmt_row(r) for r in orows]
+    assert np.array_equal(ref_cols.astype(np.float32), qc.output_columns(fin, om_last))
+    assert np.abs(fin - st).max() > 1e-6          # the flow moved the seed
