@@ -228,10 +228,16 @@ int  scp_batch_config(const scp_batch *b, int *di, int *persistent, int *tuned);
  * concurrency slot is acquired :253, executeRun :239 execs one scp_sim process
  * per job, RunStatus :483, Download :514), rebuilt as one process whose jobs
  * share the GPU: jobs are (parameters, Q-balls, seed, steps); each round
- * admits queued jobs in arrival order while their scp_grid_bytes fit the HBM
- * budget (seeded on the device, Gauss-projected if asked), advances all active
- * jobs by up to `quantum` steps in one batched chain per class, and retires
- * finished jobs, whose grids stay resident until scp_server_take. */
+ * admits queued jobs in arrival order while their device bytes (scp_grid_bytes
+ * plus the projection's work arrays) fit the HBM budget (seeded on the device,
+ * Gauss-projected if asked), advances all active jobs by up to `quantum` steps
+ * in one batched chain per class, and retires finished jobs, whose grids stay
+ * resident until scp_server_take. A queued job that does not fit is passed
+ * over by later jobs that do for at most `aging` rounds (scp_server_set_aging,
+ * default 8; 0 = strict arrival order). Device memory comes from a pool the
+ * server grows in slabs of >= 2 GiB and reuses; a job holds its FP64 emit
+ * buffers only at admission and from its last chain on, its step state only
+ * until its last step. Take finished results: they hold memory until then. */
 typedef struct scp_server scp_server;
 
 typedef struct {
@@ -250,7 +256,7 @@ enum { SCP_JOB_FAILED = -1, SCP_JOB_QUEUED = 0, SCP_JOB_ACTIVE = 1, SCP_JOB_DONE
 typedef struct {
     int32_t state;             /* SCP_JOB_* */
     int32_t nsteps;
-    int64_t steps_done, bytes;
+    int64_t steps_done, bytes; /* bytes: device memory the job holds now */
     double t_submit, t_admit, t_done;   /* seconds on the server's clock (scp_server_now) */
 } scp_job_info;
 
@@ -275,6 +281,8 @@ int  scp_server_job(const scp_server *s, int64_t id, scp_job_info *info);
 int  scp_server_take(scp_server *s, int64_t id, double *const *cols, int ncols, double *diag, int ndiag);
 int  scp_server_stats_get(const scp_server *s, scp_server_stats *st);
 double scp_server_now(const scp_server *s);
+/* Rounds a queued job that does not fit may be passed over by later jobs (>= 0). */
+int  scp_server_set_aging(scp_server *s, int rounds);
 
 /* Fixed-charge gradient flow: the relaxer sfa/seed/eta_qflow.c (its fixed-Q
  * mode, eta_qflow.c:242-375) on the device. A complex-mode state of 12 fields

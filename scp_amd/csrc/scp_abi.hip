@@ -68,6 +68,17 @@ extern "C" int scp_last_error(char *buf, int n) {
 
 static int ncols_of(int mode) { return mode == 2 ? 30 : (mode == 1 ? 24 : 12); }
 
+// buffers an entry point needs: the FP64 emit buffers / the step state (a
+// pooled grid of the request loop holds the first only while it must emit, the
+// second only until its last step)
+static int need_emit(const scp_grid *g) {
+    return (g->xo && g->vo) ? 0 : set_err(-1, "grid holds no FP64 emit buffers (released by its owner)");
+}
+static int need_state(const scp_grid *g) {
+    return (g->P[0] || g->P32[0]) ? 0 : set_err(-1, "grid holds no step state (released after its last step)");
+}
+
+
 // device field f (position, or velocity when vel) -> SFA snapshot column
 static int col_of(int mode, int f, bool vel) {
     if (mode == 0) return vel ? 6 + f : f;        // u0..2 -> 0..2, tu0..2 -> 3..5
@@ -169,21 +180,78 @@ static KParams make_kparams(const scp_params &p, int step_mode) {
     return k;
 }
 
+hipError_t scp_internal::gmalloc(scp_grid *g, void **p, size_t bytes) {
+    *p = nullptr;
+    return g->mem ? g->mem->alloc(g->mem->ctx, p, bytes) : hipMalloc(p, bytes);
+}
+
+void scp_internal::gfree(scp_grid *g, void *p) {
+    if (!p) return;
+    if (g->mem) g->mem->release(g->mem->ctx, p);
+    else hipFree(p);
+}
+
+template <typename T>
+static hipError_t galloc(scp_grid *g, T **p, size_t bytes) {
+    return gmalloc(g, reinterpret_cast<void **>(p), bytes);
+}
+
+template <typename T>
+static void grelease(scp_grid *g, T *&p) {
+    gfree(g, p);
+    p = nullptr;
+}
+
+static void release_step_state(scp_grid *g) {
+    grelease(g, g->P[0]); grelease(g, g->P[1]); grelease(g, g->W);
+    grelease(g, g->P32[0]); grelease(g, g->P32[1]); grelease(g, g->W32);
+    grelease(g, g->Plo[0]); grelease(g, g->Plo[1]); grelease(g, g->Wlo);
+}
+
 static void free_grid(scp_grid *g) {
     if (!g) return;
-    hipFree(g->P[0]); hipFree(g->P[1]); hipFree(g->W); hipFree(g->xo); hipFree(g->vo);
-    hipFree(g->P32[0]); hipFree(g->P32[1]); hipFree(g->W32);
-    hipFree(g->Plo[0]); hipFree(g->Plo[1]); hipFree(g->Wlo);
-    hipFree(g->aux); hipFree(g->acc); hipFree(g->partials); hipFree(g->dout); hipFree(g->argmin);
-    hipFree(g->rho); hipFree(g->fpart);
-    hipFree(g->pinx); hipFree(g->pinv); hipFree(g->side);
+    release_step_state(g);
+    grelease(g, g->xo); grelease(g, g->vo);
+    grelease(g, g->aux); grelease(g, g->acc); grelease(g, g->partials); grelease(g, g->dout);
+    grelease(g, g->argmin); grelease(g, g->rho); grelease(g, g->fpart);
+    grelease(g, g->pinx); grelease(g, g->pinv); grelease(g, g->side);
     if (g->ev0) hipEventDestroy(g->ev0);
     if (g->ev1) hipEventDestroy(g->ev1);
     if (g->stream) hipStreamDestroy(g->stream);
     delete g;
 }
 
-extern "C" int scp_grid_create(const scp_params *p, scp_grid **out) {
+int64_t scp_internal::grid_emit_bytes(const scp_params *p) {
+    const int64_t n3 = (int64_t)p->N * p->N * p->N;
+    return 2LL * np_of_mode(p->mode) * field_stride(n3) * 8;
+}
+
+int scp_internal::grid_emit_buffers(scp_grid *g, bool on) {
+    if (!on) {
+        HIPCHK(hipStreamSynchronize(g->stream));
+        grelease(g, g->xo);
+        grelease(g, g->vo);
+        return 0;
+    }
+    if (g->xo && g->vo) return 0;
+    const size_t grp = (size_t)g->kp.fs * sizeof(double) * g->NP;
+    hipError_t e = g->xo ? hipSuccess : galloc(g, &g->xo, grp);
+    if (e == hipSuccess && !g->vo) e = galloc(g, &g->vo, grp);
+    if (e != hipSuccess) {
+        grelease(g, g->xo);
+        grelease(g, g->vo);
+        return set_err(-2, "emit buffer allocation failed: %s", hipGetErrorString(e));
+    }
+    return 0;
+}
+
+int scp_internal::grid_release_step_state(scp_grid *g) {
+    HIPCHK(hipStreamSynchronize(g->stream));
+    release_step_state(g);
+    return 0;
+}
+
+int scp_internal::grid_create_with(const scp_params *p, const GridMem *mem, bool emit, scp_grid **out) {
     if (!out) return set_err(-1, "null out");
     *out = nullptr;
     int rc = check_params(p);
@@ -192,6 +260,7 @@ extern "C" int scp_grid_create(const scp_params *p, scp_grid **out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(-4, "no HIP device");
     scp_grid *g = new scp_grid;
     memset(g, 0, sizeof *g);
+    g->mem = mem;
     g->p = *p;
     g->mode = p->mode;
     g->step_mode = (p->mode == 2 && p->g == 0.0) ? 1 : p->mode;
@@ -206,37 +275,39 @@ extern "C" int scp_grid_create(const scp_params *p, scp_grid **out) {
     g->f32 = p->precision == SCP_PREC_F32C;
     g->lo_kind = p->lo_words;
     if (!g->f32) {
-        e = e ? e : hipMalloc(&g->P[0], grph);
-        e = e ? e : hipMalloc(&g->P[1], grph);
-        e = e ? e : hipMalloc(&g->W, grph);
+        e = e ? e : galloc(g, &g->P[0], grph);
+        e = e ? e : galloc(g, &g->P[1], grph);
+        e = e ? e : galloc(g, &g->W, grph);
     } else {
         const size_t g32 = (size_t)g->kp.fsh * 4 * g->NP;
         const size_t glw = (size_t)g->kp.fsh * lo_site_bytes(lo_kind_w(g->lo_kind), g->NP);
         const size_t glp = (size_t)g->kp.fsh * lo_site_bytes(lo_kind_p(g->lo_kind), g->NP);
-        e = e ? e : hipMalloc(&g->P32[0], g32);
-        e = e ? e : hipMalloc(&g->P32[1], g32);
-        e = e ? e : hipMalloc(&g->W32, g32);
+        e = e ? e : galloc(g, &g->P32[0], g32);
+        e = e ? e : galloc(g, &g->P32[1], g32);
+        e = e ? e : galloc(g, &g->W32, g32);
         if (glp) {
-            e = e ? e : hipMalloc(&g->Plo[0], glp);
-            e = e ? e : hipMalloc(&g->Plo[1], glp);
+            e = e ? e : galloc(g, &g->Plo[0], glp);
+            e = e ? e : galloc(g, &g->Plo[1], glp);
         }
-        if (glw) e = e ? e : hipMalloc(&g->Wlo, glw);
+        if (glw) e = e ? e : galloc(g, &g->Wlo, glw);
     }
-    e = e ? e : hipMalloc(&g->xo, grp);
-    e = e ? e : hipMalloc(&g->vo, grp);
+    if (emit) {
+        e = e ? e : galloc(g, &g->xo, grp);
+        e = e ? e : galloc(g, &g->vo, grp);
+    }
     if (p->bc_type == 1) {
-        e = e ? e : hipMalloc(&g->pinx, grp);
-        e = e ? e : hipMalloc(&g->pinv, grp);
-        e = e ? e : hipMalloc(&g->side, (size_t)6 * g->NP * p->N * p->N * sizeof(double));
+        e = e ? e : galloc(g, &g->pinx, grp);
+        e = e ? e : galloc(g, &g->pinv, grp);
+        e = e ? e : galloc(g, &g->side, (size_t)6 * g->NP * p->N * p->N * sizeof(double));
     }
     if (p->mode == 0 && (p->alpha_cs != 0 || p->beta_h != 0 || p->sigma_freq > 0)) {
-        e = e ? e : hipMalloc(&g->aux, 7 * fsb);
+        e = e ? e : galloc(g, &g->aux, 7 * fsb);
         e = e ? e : hipMemsetAsync(g->aux, 0, 7 * fsb, g->stream);
     }
     g->nblocks_diag = (int)std::min<int64_t>(1024, ((int64_t)g->kp.n3 + DIAG_BLOCK - 1) / DIAG_BLOCK);
-    e = e ? e : hipMalloc(&g->partials, (size_t)g->nblocks_diag * 32 * sizeof(double));
-    e = e ? e : hipMalloc(&g->dout, 64 * sizeof(double));
-    e = e ? e : hipMalloc(&g->argmin, sizeof(unsigned long long));
+    e = e ? e : galloc(g, &g->partials, (size_t)g->nblocks_diag * 32 * sizeof(double));
+    e = e ? e : galloc(g, &g->dout, 64 * sizeof(double));
+    e = e ? e : galloc(g, &g->argmin, sizeof(unsigned long long));
     e = e ? e : hipEventCreate(&g->ev0);
     e = e ? e : hipEventCreate(&g->ev1);
     if (e != hipSuccess) {
@@ -256,6 +327,8 @@ extern "C" int scp_grid_create(const scp_params *p, scp_grid **out) {
     *out = g;
     return 0;
 }
+
+extern "C" int scp_grid_create(const scp_params *p, scp_grid **out) { return grid_create_with(p, nullptr, true, out); }
 
 extern "C" int scp_grid_destroy(scp_grid *g) {
     if (!g) return 0;
@@ -330,6 +403,8 @@ int scp_internal::run_init(scp_grid *g, bool pin) {
 
 extern "C" int scp_grid_upload(scp_grid *g, const double *const *cols, int ncols) {
     if (!g || !cols) return set_err(-1, "null argument");
+    if (need_emit(g)) return -1;
+    if (need_state(g)) return -1;
     if (ncols != ncols_of(g->mode)) return set_err(-1, "ncols=%d, mode %d needs %d", ncols, g->mode, ncols_of(g->mode));
     const size_t nb = (size_t)g->kp.n3 * sizeof(double);
     for (int f = 0; f < g->NP; f++) {
@@ -354,8 +429,9 @@ extern "C" int scp_grid_seed_balls(scp_grid *g, const scp_qball *balls, int nbal
         if (!(balls[b].sigma > 0) || (balls[b].winding != 1 && balls[b].winding != -1))
             return set_err(-1, "ball %d: sigma must be > 0 and winding +-1", b);
     HIPCHK(hipStreamSynchronize(g->stream));
+    if (!g->xo || !g->vo) return set_err(-1, "grid holds no emit buffers to seed");
     int rc = seed_balls_device(g->p.N, g->p.L, g->p.dx, g->kp.fs, g->mode == 2, balls, nballs, theta_sigma,
-                               link_sigma, seed, g->xo, g->vo, g->stream);
+                               link_sigma, seed, g->xo, g->vo, g->stream, g->mem);
     if (rc) return set_err(-2, "device seeding failed: %s", hipGetErrorString((hipError_t)rc));
     rc = run_init(g);
     if (rc) return rc;
@@ -366,6 +442,7 @@ extern "C" int scp_grid_seed_balls(scp_grid *g, const scp_qball *balls, int nbal
 
 extern "C" int scp_grid_download(scp_grid *g, double *const *cols, int ncols) {
     if (!g || !cols) return set_err(-1, "null argument");
+    if (need_emit(g)) return -1;
     if (ncols != ncols_of(g->mode)) return set_err(-1, "ncols=%d, mode %d needs %d", ncols, g->mode, ncols_of(g->mode));
     const size_t nb = (size_t)g->kp.n3 * sizeof(double);
     for (int f = 0; f < g->NP; f++) {
@@ -378,9 +455,12 @@ extern "C" int scp_grid_download(scp_grid *g, double *const *cols, int ncols) {
     return 0;
 }
 
-// nsteps launches, the last emitting; da: that launch also accumulates the
-// diagnostics (march kernels of the complex and gauged modes only)
+// nsteps launches, the last emitting (when the grid holds emit buffers); da: that
+// launch also accumulates the diagnostics (march kernels of the complex and
+// gauged modes only)
 static int enqueue_steps(scp_grid *g, int nsteps, const DiagArgs *da = nullptr) {
+    if (int rc = need_state(g)) return rc;
+    if (da && need_emit(g)) return -1;
     for (int s = 0; s < nsteps; s++) {
         int rc = 0;
         if (g->aux) {
@@ -388,7 +468,7 @@ static int enqueue_steps(scp_grid *g, int nsteps, const DiagArgs *da = nullptr) 
                         : run_aux(g, g->P[g->cur], g->W);
             if (rc) return rc;
         }
-        const bool emit = (s == nsteps - 1);
+        const bool emit = (s == nsteps - 1) && g->xo;
         if (emit && da)
             rc = g->f32 ? kerr(launch_march_f32c_diag(march_mode(g), g->lo_kind, g->kp, bufs32_for(g, K_STEP), 0, *da,
                                                       g->stream), "march32 diag")
@@ -544,6 +624,7 @@ static int diag_finish(scp_grid *g, const DiagParams &dp, const double *S, const
 
 extern "C" int scp_grid_diag(scp_grid *g, double *out, int nout) {
     if (!g || !out) return set_err(-1, "null argument");
+    if (need_emit(g)) return -1;
     if (nout < SCP_D_COUNT) return set_err(-1, "nout=%d < %d", nout, SCP_D_COUNT);
     const DiagParams dp = diag_params(g);
     const int nb = g->nblocks_diag;
@@ -607,13 +688,12 @@ extern "C" int scp_grid_step_diag(scp_grid *g, int nsteps, double *out, int nout
     int nbm, nbt;
     fused_rows(g, &nbm, &nbt);
     if (g->fpart_rows < nbm + nbt) {
-        hipFree(g->fpart);
-        g->fpart = nullptr;
+        grelease(g, g->fpart);
         g->fpart_rows = 0;
-        HIPCHK(hipMalloc(&g->fpart, (size_t)(nbm + nbt) * NQ * sizeof(double)));
+        HIPCHK(galloc(g, &g->fpart, (size_t)(nbm + nbt) * NQ * sizeof(double)));
         g->fpart_rows = nbm + nbt;
     }
-    if (g->mode == 2 && !g->rho) HIPCHK(hipMalloc(&g->rho, (size_t)g->kp.n3 * sizeof(double)));
+    if (g->mode == 2 && !g->rho) HIPCHK(galloc(g, &g->rho, (size_t)g->kp.n3 * sizeof(double)));
     return step_diag_run(g, nsteps, out, nout, g->fpart, g->rho);
 }
 
@@ -675,9 +755,11 @@ extern "C" int scp_grid_step_diag_noalloc(scp_grid *g, int nsteps, double *out, 
 
 extern "C" int scp_grid_forces(scp_grid *g, double *const *acc, int nacc) {
     if (!g || !acc) return set_err(-1, "null argument");
+    if (need_emit(g)) return -1;
+    if (need_state(g)) return -1;
     const int want = g->mode == 2 ? 15 : (g->mode == 1 ? 12 : 6);
     if (nacc != want) return set_err(-1, "nacc=%d, mode %d needs %d", nacc, g->mode, want);
-    if (!g->acc) HIPCHK(hipMalloc(&g->acc, (size_t)g->NP * g->kp.fs * sizeof(double)));
+    if (!g->acc) HIPCHK(galloc(g, &g->acc, (size_t)g->NP * g->kp.fs * sizeof(double)));
     HIPCHK(hipMemsetAsync(g->acc, 0, (size_t)g->NP * g->kp.fs * sizeof(double), g->stream));
     int rc = run_aux(g, g->xo, g->vo);
     if (rc) return rc;
@@ -693,6 +775,7 @@ extern "C" int scp_grid_forces(scp_grid *g, double *const *acc, int nacc) {
 
 extern "C" int scp_grid_vec_fit(scp_grid *g, int block_size, float *coeffs, int64_t n_coeffs) {
     if (!g || !coeffs) return set_err(-1, "null argument");
+    if (need_emit(g)) return -1;
     if (g->mode != 0)
         return set_err(-1, "vector frames are real-mode output (complex_phi=1 refuses vec_snap_dt, "
                            "scp_config.h:458)");
@@ -714,6 +797,8 @@ extern "C" int scp_grid_vec_fit(scp_grid *g, int block_size, float *coeffs, int6
 
 extern "C" int scp_grid_rescale_phi(scp_grid *g, double f) {
     if (!g) return set_err(-1, "null grid");
+    if (need_emit(g)) return -1;
+    if (need_state(g)) return -1;
     // Between steps the reference scales phi and its velocity in place and keeps
     // the stored acceleration a_n = F(x_n) of the unscaled state: the next half
     // kick uses it, the next force evaluation sees the scaled positions
@@ -722,7 +807,7 @@ extern "C" int scp_grid_rescale_phi(scp_grid *g, double f) {
     // step's own, so it is the step's a_n; with sigma_freq the theta frequency
     // is then taken from the completed velocities).
     const size_t nacc = (size_t)g->NP * g->kp.fs * sizeof(double);
-    if (!g->acc) HIPCHK(hipMalloc(&g->acc, nacc));
+    if (!g->acc) HIPCHK(galloc(g, &g->acc, nacc));
     int rc = run_aux(g, g->xo, g->vo);
     if (!rc) rc = launch_kind(g, K_FORCES, false);
     if (rc) return rc;
@@ -765,6 +850,8 @@ extern "C" int scp_grid_set_damp_rate(scp_grid *g, double rate) {
 // at most 20000 iterations), in FP64.
 extern "C" int scp_grid_gauss_project(scp_grid *g, int *iters, double *gauss_max) {
     if (!g) return set_err(-1, "null grid");
+    if (need_emit(g)) return -1;
+    if (need_state(g)) return -1;
     if (g->mode != 2) return set_err(-1, "gauss projection needs the gauged mode");
     const KParams &kp = g->kp;
     const uint32_t n = kp.n3;
@@ -773,9 +860,9 @@ extern "C" int scp_grid_gauss_project(scp_grid *g, int *iters, double *gauss_max
     double *b = nullptr, *x = nullptr, *r = nullptr, *q = nullptr, *Aq = nullptr;
     const size_t bytes = (size_t)n * sizeof(double);
     int rc = 0;
-    auto cleanup = [&]() { hipFree(b); hipFree(x); hipFree(r); hipFree(q); hipFree(Aq); };
-    if (hipMalloc(&b, bytes) || hipMalloc(&x, bytes) || hipMalloc(&r, bytes) || hipMalloc(&q, bytes) ||
-        hipMalloc(&Aq, bytes)) {
+    auto cleanup = [&]() { gfree(g, b); gfree(g, x); gfree(g, r); gfree(g, q); gfree(g, Aq); };
+    if (galloc(g, &b, bytes) || galloc(g, &x, bytes) || galloc(g, &r, bytes) || galloc(g, &q, bytes) ||
+        galloc(g, &Aq, bytes)) {
         cleanup();
         return set_err(-2, "gauss projection alloc failed");
     }

@@ -122,6 +122,7 @@ static void fill_desc(BatchGrid<B> *d, const scp_batch *b, scp_grid *g, int nq, 
     d->DI = grid_di(b, g);
     d->cur0 = cur;
     d->nq = nq;
+    d->emit = (g->xo && g->vo) ? 1 : 0;
     d->ntk = march_tiles_k(g->p.N);
     d->ntj = march_tiles_j(g->p.N);
 }
@@ -153,6 +154,8 @@ static int run_chain(scp_batch *b, const int *nsteps, bool graph, bool timed = f
     }
     for (size_t m = 0; m < n; m++) {
         scp_grid *g = b->members[m];
+        if (nsteps[m] > 0 && !g->P[0] && !g->P32[0])
+            return set_err(-1, "member %zu holds no step state (released after its last step)", m);
         HIPCHK(hipStreamSynchronize(g->stream));      // the member's own work comes first
         if (b->f32)
             fill_desc((BatchGrid<Bufs32> *)b->h_grids + m, b, g, nsteps[m], bufs32_for);

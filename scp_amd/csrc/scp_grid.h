@@ -7,8 +7,19 @@
 
 #include <hip/hip_runtime.h>
 
+namespace scp_internal {
+// device memory source of a grid (nullptr: hipMalloc / hipFree); the server
+// serves its grids from a pool through this
+struct GridMem {
+    void *ctx;
+    hipError_t (*alloc)(void *ctx, void **p, size_t bytes);
+    void (*release)(void *ctx, void *p);
+};
+}  // namespace scp_internal
+
 struct scp_grid {
     scp_params p;
+    const scp_internal::GridMem *mem;   // allocator of every device buffer below (null: hipMalloc)
     scp::KParams kp;
     int mode;        // storage mode (0/1/2), number of columns follows it
     int step_mode;   // kernel mode: 2 only when gauged with g != 0
@@ -47,7 +58,7 @@ namespace scp {
 // grid with field stride fs; synchronous on s; returns a hipError_t
 int seed_balls_device(int N, double L, double dx, uint32_t fs, int gauged, const scp_qball *balls, int nballs,
                       double theta_sigma, double link_sigma, uint64_t seed, double *xo, double *vo,
-                      hipStream_t s);
+                      hipStream_t s, const scp_internal::GridMem *mem);
 }  // namespace scp
 
 namespace scp_internal {
@@ -59,6 +70,19 @@ scp::Bufs<double> bufs_for(scp_grid *g, int kind);
 scp::Bufs32 bufs32_for(scp_grid *g, int kind);
 // initial force + first half kick + drift from xo/vo (upload and device seeding)
 int run_init(scp_grid *g, bool pin = true);
+// device buffers of a grid through its allocator
+hipError_t gmalloc(scp_grid *g, void **p, size_t bytes);
+void gfree(scp_grid *g, void *p);
+// scp_grid_create with an allocator; emit = false leaves the FP64 emit buffers
+// xo, vo unallocated (grid_emit_buffers adds them when a launch must emit)
+int grid_create_with(const scp_params *p, const GridMem *mem, bool emit, scp_grid **out);
+// allocate (on) or release (off) xo, vo; steps of a grid without them do not emit
+int grid_emit_buffers(scp_grid *g, bool on);
+// release the step state (P, W and their words) of a grid whose last step has
+// emitted: only xo/vo and the diagnostics buffers remain (no further steps)
+int grid_release_step_state(scp_grid *g);
+// device bytes of the emit buffers xo, vo and of everything else
+int64_t grid_emit_bytes(const scp_params *p);
 }  // namespace scp_internal
 
 // return the HIP error of x as status -2 with a message

@@ -153,7 +153,8 @@ struct BatchGrid {
     B buf[2];          // the grid's buffers when its P parity is 0 / 1
     int32_t DI;        // planes per segment
     int32_t cur0;      // P parity at chain launch 0
-    int32_t nq;        // chain launches this grid takes part in; it emits on the last
+    int32_t nq;        // chain launches this grid takes part in; it emits on the last ...
+    int32_t emit;      // ... when this is 1 (the grid holds its FP64 emit buffers)
     uint32_t ntk, ntj; // tiles per plane along k and j
 };
 struct BatchTile {

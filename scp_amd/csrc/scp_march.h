@@ -627,7 +627,8 @@ inline int launch_march_diag_t(const KParams &kp, const typename S::B &b, int DI
 
 // Many grids: block b runs tiles [b*per_block, (b+1)*per_block) of the table
 // (b in XCD-aware order, so an XCD's blocks take adjacent tiles); a grid takes
-// part in chain launches q < nq and emits its FP64 state on the last.
+// part in chain launches q < nq and emits its FP64 state on the last (if it
+// holds the emit buffers: grids of the request loop release them in between).
 template <class S, int KW, int MODE>
 __global__ void __launch_bounds__(MG<KW>::MNT, KW == 1 ? S::MINB : 1)
 k_march_batch(const BatchGrid<typename S::B> *__restrict__ grids, const BatchTile *__restrict__ tiles,
@@ -642,7 +643,7 @@ k_march_batch(const BatchGrid<typename S::B> *__restrict__ grids, const BatchTil
         const uint32_t tk = bt.tile % g.ntk, r = bt.tile / g.ntk;
         const uint32_t tj = r % g.ntj, seg = r / g.ntj;
         const int par = (g.cur0 + q) & 1;
-        if (q == g.nq - 1)
+        if (q == g.nq - 1 && g.emit)
             march_tile<S, KW, MODE, true>(g.kp, g.buf[par], g.DI, tk, tj, seg, lds);
         else
             march_tile<S, KW, MODE, false>(g.kp, g.buf[par], g.DI, tk, tj, seg, lds);

@@ -97,7 +97,7 @@ __global__ void k_seed(int N, double L, double dx, uint32_t fs, int gauged, cons
 // host side: ball constants, the launch, and the device copy of the ball list
 int seed_balls_device(int N, double L, double dx, uint32_t fs, int gauged, const scp_qball *balls, int nballs,
                       double theta_sigma, double link_sigma, uint64_t seed, double *xo, double *vo,
-                      hipStream_t s) {
+                      hipStream_t s, const scp_internal::GridMem *mem) {
     std::vector<BallDev> bd((size_t)(nballs > 0 ? nballs : 1));
     for (int b = 0; b < nballs; b++) {
         const scp_qball &B = balls[b];
@@ -117,7 +117,8 @@ int seed_balls_device(int N, double L, double dx, uint32_t fs, int gauged, const
         D.ck = whole ? 0 : (int)lround((B.z0 + L) / dx);
     }
     BallDev *d = nullptr;
-    hipError_t e = hipMalloc((void **)&d, bd.size() * sizeof(BallDev));
+    hipError_t e = mem ? mem->alloc(mem->ctx, (void **)&d, bd.size() * sizeof(BallDev))
+                       : hipMalloc((void **)&d, bd.size() * sizeof(BallDev));
     if (e != hipSuccess) return (int)e;
     e = hipMemcpyAsync(d, bd.data(), bd.size() * sizeof(BallDev), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) {
@@ -127,7 +128,8 @@ int seed_balls_device(int N, double L, double dx, uint32_t fs, int gauged, const
         e = hipGetLastError();
     }
     const hipError_t e2 = hipStreamSynchronize(s);   // the ball list is read by this launch only
-    hipFree(d);
+    if (mem) mem->release(mem->ctx, d);
+    else hipFree(d);
     return (int)(e != hipSuccess ? e : e2);
 }
 

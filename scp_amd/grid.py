@@ -43,7 +43,7 @@ ABI_SYMBOLS = ["scp_abi_version", "scp_last_error", "scp_grid_bytes", "scp_grid_
                "scp_server_job", "scp_server_take", "scp_server_stats_get", "scp_server_now",
                "scp_grid_vec_fit", "scp_grid_set_kappa", "scp_grid_set_damp_rate",
                "scp_grid_step_diag_noalloc", "scp_qflow_create", "scp_qflow_destroy", "scp_qflow_upload",
-               "scp_qflow_download", "scp_qflow_norm", "scp_qflow_run"]
+               "scp_qflow_download", "scp_qflow_norm", "scp_qflow_run", "scp_server_set_aging"]
 
 
 class ScpError(RuntimeError):
@@ -195,6 +195,7 @@ def lib():
         L.scp_server_stats_get.argtypes = [VP, C.POINTER(ServerStats)]
         L.scp_server_now.restype = C.c_double
         L.scp_server_now.argtypes = [VP]
+        L.scp_server_set_aging.argtypes = [VP, C.c_int]
         L.scp_qflow_create.argtypes = [C.POINTER(QflowParams), C.POINTER(VP)]
         L.scp_qflow_destroy.argtypes = [VP]
         L.scp_qflow_upload.argtypes = [VP, DPP, C.c_int]
@@ -449,6 +450,9 @@ class Server:
         st = ServerStats()
         _check(lib().scp_server_stats_get(self._h, C.byref(st)), "scp_server_stats_get")
         return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def set_aging(self, rounds):
+        _check(lib().scp_server_set_aging(self._h, int(rounds)), "scp_server_set_aging")
 
     def now(self):
         return lib().scp_server_now(self._h)

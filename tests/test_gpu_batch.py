@@ -127,3 +127,34 @@ def test_batch_membership_rules():
         assert G.lib().scp_batch_size(b._h) == 0
     b64.close()
     real.close()
+
+
+@pytest.mark.parametrize("mode", [G.MODE_COMPLEX, G.MODE_GAUGED])
+def test_fp64_batch_members_match_the_oracle(mode):
+    """FP64 grids of different sizes stepped together by one batch (graph-replayed
+    chains, per-member step counts) against the oracle's restatement of the
+    reference step on each member's uploaded state: bitwise in complex mode,
+    within 1e-12 relative field L2 gauged (this build's link sincos)."""
+    from oracle import core as oracle
+    sizes, counts = (10, 16, 21), (9, 14, 6)
+    states, grids = [], []
+    for i, N in enumerate(sizes):
+        L, st = ball(N, 500 + i)
+        st = st if mode == G.MODE_GAUGED else st[:24].copy()
+        g = G.Grid(G.make_params(N, mode, L, 0.2, g=0.1))
+        g.upload(st)
+        states.append((L, st))
+        grids.append(g)
+    print(f"case fp64 batch vs oracle mode={mode}", flush=True)
+    with G.Batch(grids) as b:
+        b.step([4, 4, 4])
+        b.step([c - 4 for c in counts])
+    for (L, st), g, n in zip(states, grids, counts):
+        N = g.params.N
+        ref, _ = oracle.step(oracle.make_params(N, mode, L, 0.2, g=0.1), st, n)
+        out = g.download()
+        if mode == G.MODE_GAUGED:
+            assert np.linalg.norm(out - ref) / np.linalg.norm(ref) <= 1e-12, N
+        else:
+            assert np.array_equal(out, ref), N
+        g.close()
