@@ -666,8 +666,8 @@ static int step_diag_run(scp_grid *g, int nsteps, double *out, int nout, double 
     int rc = enqueue_steps(g, nsteps, &da);
     if (rc) return rc;
     if (nbt)
-        rc = kerr(launch_gauss_tail(g->kp, da.dp, g->vo, rho, fpart + (size_t)nbm * NQ, nbt, g->stream),
-                  "gauss tail");
+        rc = kerr(launch_gauss_seam(g->kp, da.dp, 0, g->vo, rho, fpart + (size_t)nbm * NQ, nbt, g->stream),
+                  "gauss seam");
     if (rc) return rc;
     rc = kerr(launch_reduce_wide(fpart, nbm + nbt, DS_NSUM, DM_NMAX, g->dout, g->stream), "reduce");
     if (rc) return rc;

@@ -120,22 +120,43 @@ k_diag2(KParams kp, const double *__restrict__ xo, double thr, double cx, double
     block_reduce<1, false>(v, partials + (size_t)blockIdx.x);
 }
 
-// Gauss statistics of the state emitted by a fused-diagnostics launch: E from
-// the emitted velocities, rho from the step (compute_gauss, C:2186-2243). The
-// row's other slots are zero, so it reduces with the step's partials.
+// Gauss statistics of the seam sites of a fused-diagnostics launch: the sites
+// whose backward neighbour (x - e_d) another block of that launch emitted — the
+// first plane of each DI-plane segment, the first row of each 4-row tile, the
+// first column of each 64-column tile — from the emitted E and the rho the
+// launch kept for them (compute_gauss, C:2186-2243); every other site's
+// statistics were accumulated by the launch itself. Seam sites are enumerated
+// compactly: rows j = 0 mod 4 (every i, k), then columns k = 0 mod 64 of the
+// other rows, then planes i = 0 mod DI of the rest. The row's other slots are
+// zero, so it reduces with the step's partials.
+__device__ inline int nth_nonmultiple(uint32_t q, int m) { return (int)((q / (m - 1)) * m + q % (m - 1) + 1); }
+
 __global__ void __launch_bounds__(DIAG_BLOCK)
-k_gauss_tail(KParams kp, DiagParams dp, const double *__restrict__ vo, const double *__restrict__ rho,
+k_gauss_seam(KParams kp, DiagParams dp, int DI, const double *__restrict__ vo, const double *__restrict__ rho,
              double *partials) {
     const int N = kp.N;
-    const uint32_t fs = kp.fs, n3 = kp.n3;
+    const uint32_t fs = kp.fs, uN = (uint32_t)N;
+    const uint32_t nj4 = (uN + 3) / 4, nk64 = (uN + 63) / 64, niD = (uN + DI - 1) / DI;
+    const uint32_t n1 = uN * nj4 * uN, n2 = uN * (uN - nj4) * nk64, n3s = niD * (uN - nj4) * (uN - nk64);
     double S[DS_NSUM], M[DM_NMAX];
 #pragma unroll
     for (int q = 0; q < DS_NSUM; q++) S[q] = 0.0;
 #pragma unroll
     for (int q = 0; q < DM_NMAX; q++) M[q] = 0.0;
     const double *E0 = vo + (size_t)F_TH * fs, *E1 = E0 + fs, *E2 = E1 + fs;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n3; c += gridDim.x * blockDim.x) {
-        const int k = c % N, j = (c / N) % N, i = c / ((uint32_t)N * N);
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n1 + n2 + n3s; t += gridDim.x * blockDim.x) {
+        int i, j, k;
+        if (t < n1) {
+            k = t % uN; j = 4 * (int)((t / uN) % nj4); i = t / (uN * nj4);
+        } else if (t < n1 + n2) {
+            const uint32_t q = t - n1;
+            k = 64 * (int)(q % nk64); j = nth_nonmultiple((q / nk64) % (uN - nj4), 4); i = q / (nk64 * (uN - nj4));
+        } else {
+            const uint32_t q = t - n1 - n2;
+            k = nth_nonmultiple(q % (uN - nk64), 64); j = nth_nonmultiple((q / (uN - nk64)) % (uN - nj4), 4);
+            i = DI * (int)(q / ((uN - nk64) * (uN - nj4)));
+        }
+        const uint32_t c = sidx3(N, i, j, k);
         const double E[3] = {E0[c], E1[c], E2[c]};
         const double Em[3] = {E0[sidx3(N, wmd(i, N), j, k)], E1[sidx3(N, i, wmd(j, N), k)],
                               E2[sidx3(N, i, j, wmd(k, N))]};
@@ -201,9 +222,10 @@ int launch_diag_pass2(const KParams &kp, const double *xo, double thr, double cx
     return (int)hipGetLastError();
 }
 
-int launch_gauss_tail(const KParams &kp, const DiagParams &dp, const double *vo, const double *rho,
+int launch_gauss_seam(const KParams &kp, const DiagParams &dp, int DI, const double *vo, const double *rho,
                       double *partials, int nblocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_gauss_tail, dim3(nblocks), dim3(DIAG_BLOCK), 0, s, kp, dp, vo, rho, partials);
+    if (DI <= 0) DI = march_di(kp.N, 1);
+    hipLaunchKernelGGL(k_gauss_seam, dim3(nblocks), dim3(DIAG_BLOCK), 0, s, kp, dp, DI, vo, rho, partials);
     return (int)hipGetLastError();
 }
 

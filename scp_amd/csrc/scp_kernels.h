@@ -245,8 +245,10 @@ inline int march_blocks(int N, int DI) {
     return (int)(march_tiles_k(N) * march_tiles_j(N) * (uint32_t)((N + DI - 1) / DI));
 }
 // Gauss statistics (DS_GL2, DS_NOMEGA, DS_FSUM, DS_NC, DM_GAUSS; other slots
-// zero) from the emitted E and rho, per block in the pass-1 layout
-int launch_gauss_tail(const KParams &kp, const DiagParams &dp, const double *vo, const double *rho,
+// zero) of the seam sites of a fused launch with segments of DI planes (<= 0:
+// the default), from the emitted E and the rho kept for them, per block in the
+// pass-1 layout
+int launch_gauss_seam(const KParams &kp, const DiagParams &dp, int DI, const double *vo, const double *rho,
                       double *partials, int nblocks, hipStream_t s);
 int launch_diag_pass1(int mode, const KParams &kp, const DiagParams &dp, const double *xo,
                       const double *vo, double *partials, int nblocks, hipStream_t s);

@@ -328,7 +328,35 @@ constexpr int march_lds_elems() { return MNBUF * march_nf<MODE>() * MG<KW>::MPL;
 // Per-thread diagnostics accumulators of the DIAG instance (DS_* sums, DM_* maxima).
 struct DiagState {
     double S[DS_NSUM], M[DM_NMAX];
+    double *ex;   // gauged: LDS row exchange of E_1 between the tile's waves (2 x MTJ x 64 doubles)
 };
+
+// The Gauss residual of a fused gauged launch, one site per thread, completed
+// one plane late: G = div E - g rho needs E_d(x - e_d). E_0(x - e0) is carried
+// from plane i-1 of the column, E_2(x - e2) comes from the next lower lane, and
+// E_1(x - e1) from the row below through LDS, written by its wave in the
+// plane's iteration and read after the next iteration's barrier (two
+// alternating rows of slots, no barrier of its own). Sites whose backward
+// neighbour another block emits (the segment's first plane, the tile's first
+// row and column) keep rho for the seam pass (k_gauss_seam), which forms G
+// from the emitted state in the same expression.
+struct GaussPending {
+    double E[3], E0m, E2m, rho;
+    uint32_t ec;
+    int i;
+};
+
+__device__ inline void gauss_pending(const KParams &kp, const DiagArgs *da, DiagState *ds, const MTile &t,
+                                     const GaussPending &pg) {
+    if (!t.valid) return;
+    const int row = threadIdx.y, lane = threadIdx.x;
+    if (pg.i > t.i0 && row > 0 && lane > 0) {
+        const double Em[3] = {pg.E0m, ds->ex[((pg.i & 1) * MTJ + row - 1) * 64 + lane], pg.E2m};
+        gauss_site(kp, da->dp, pg.E, Em, pg.rho, pg.i, t.j, t.k, ds->S, ds->M);
+    } else {
+        __builtin_nontemporal_store(pg.rho, &da->rho[pg.ec]);
+    }
+}
 
 // One tile of one grid: the march over planes [seg*DI, seg*DI + DI) of the
 // 64*KW x 4 column tile (tk, tj). lds: the block's staging area,
@@ -391,6 +419,11 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
     T cq[S::REGQ ? NF : 1];     // register copy of plane i (register-queue policy)
     GaugeCarry<T> cy{};
     T th0m = T(0);              // DIAG, gauged: th_0 at plane i-1 (backward transport along i)
+    // DIAG, gauged: this site's emitted E and charge density (the Gauss residual is
+    // completed after the compute, with the backward neighbours' E), and E_0 of the
+    // own column at plane i-1
+    double gE[3] = {0.0, 0.0, 0.0}, grho = 0.0;
+    GaussPending pg{};          // DIAG, gauged: the previous plane's site
     {
         const uint32_t em = (uint32_t)im0 * NN + col, e0 = (uint32_t)t.i0 * NN + col, ep = (uint32_t)ip0 * NN + col;
         T c0[NF];
@@ -472,6 +505,9 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
         // plane i+1 into the buffer that held plane i-2
         publish(lds + nbuf * NF * MPL, cp, hr, hc);
         __syncthreads();
+        if constexpr (DIAG && MODE == 2) {
+            if (i > t.i0) gauss_pending(kp, da, ds, t, pg);
+        }
 
         // own column of planes i and i+1: registers (REGQ) or read back from the staged tiles
         auto compute = [&](const auto &c0, const auto &c1) {
@@ -535,7 +571,10 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
                     return (double)nb.face(f, d, sg);
                 };
                 const double rho = diag_site<MODE == 3 ? 0 : MODE>(kp, da->dp, xe, ve, xn, i, t.j, t.k, ds->S, ds->M);
-                if constexpr (MODE == 2) __builtin_nontemporal_store(rho, &da->rho[ec]);
+                if constexpr (MODE == 2) {
+                    gE[0] = ve[F_TH]; gE[1] = ve[F_TH + 1]; gE[2] = ve[F_TH + 2];
+                    grho = rho;
+                }
             }
         };
         if constexpr (S::REGQ) {
@@ -558,8 +597,23 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
         for (int q = 0; q < RQ; q++) hr[q] = hrr[q];
         hc = hcc;
         bb = nbuf;
+        if constexpr (DIAG && MODE == 2) {
+            // this plane's Gauss site waits for the row below's E_1 (LDS, visible after
+            // the next barrier); E_2 of the next lower lane comes by shuffle now
+            const double e2m = __shfl_up(gE[2], 1, 64);
+            if (t.valid) ds->ex[((i & 1) * MTJ + threadIdx.y) * 64 + threadIdx.x] = gE[1];
+            pg.E0m = pg.E[0];
+            pg.E[0] = gE[0]; pg.E[1] = gE[1]; pg.E[2] = gE[2];
+            pg.rho = grho;
+            pg.E2m = e2m;
+            pg.ec = ec;
+            pg.i = i;
+        }
     }
     __syncthreads();
+    if constexpr (DIAG && MODE == 2) {
+        if (t.i0 < t.i1) gauss_pending(kp, da, ds, t, pg);
+    }
 }
 
 // One grid: block b runs tile (tk, tj, seg) of the XCD-aware logical numbering.
@@ -589,7 +643,9 @@ k_march_diag(KParams kp, typename S::B b, int DI, DiagArgs da) {
     uint32_t lb = logical_block(blockIdx.x, gridDim.x);
     const uint32_t tk = lb % ntk; lb /= ntk;
     const uint32_t tj = lb % ntj; lb /= ntj;
+    __shared__ double ex[MODE == 2 ? 2 * MTJ * 64 : 1];
     DiagState ds;
+    ds.ex = ex;
 #pragma unroll
     for (int q = 0; q < DS_NSUM; q++) ds.S[q] = 0.0;
 #pragma unroll
