@@ -350,8 +350,15 @@ def sweep_bench(a):
         taken = b.tune()
         cfg = b.config
         log(f"autotune: {cfg} ({taken} steps)")
-        b.step(a.steps)                      # captures the chain graph
-        ms = b.time_steps(a.steps)           # replay
+        chunk = min(a.steps, 1000)           # chains of at most 1000 launches (one captured graph)
+        b.step(chunk)                        # captures the chain graph
+        ms, left = 0.0, a.steps
+        while left > 0:                      # replays
+            n = min(chunk, left)
+            ms += b.time_steps(n)
+            left -= n
+            if a.steps > chunk:
+                log(f"sweep: {a.steps - left} of {a.steps} steps timed")
     ms_step = ms / a.steps
     # the same grids stepped one at a time (each its own launch per step)
     t1 = time.time()

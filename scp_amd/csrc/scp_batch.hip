@@ -40,9 +40,12 @@ struct scp_batch {
     int DI = 0;                 // 0: batch default (batch_default_di)
     int auto_di = 64;
     int persistent = 0;
+    int shape = TILE_64x4;      // tile shape (TILE_*)
     bool tuned = false;
     std::map<int, hipGraphExec_t> graphs;   // chain length -> replayable chain (valid for the current tables)
 };
+
+static bool shape_ok(const scp_batch *b, int shape);
 
 static void drop_graphs(scp_batch *b) {
     for (auto &kv : b->graphs) hipGraphExecDestroy(kv.second);
@@ -61,11 +64,12 @@ static int grid_di(const scp_batch *b, const scp_grid *g) {
 // re-reads) that still give >= 4096 blocks over all members
 static int batch_default_di(const scp_batch *b) {
     int di = 128;
+    const int kw = tile_kw(b->shape), tj = tile_tj(b->shape);
     for (; di > 16; di /= 2) {
         uint64_t tiles = 0;
         for (const scp_grid *g : b->members) {
             const int N = g->p.N, d = std::min(di, N);
-            tiles += (uint64_t)march_tiles_k(N) * march_tiles_j(N) * (uint64_t)((N + d - 1) / d);
+            tiles += (uint64_t)march_tiles_k(N, kw) * march_tiles_j(N, tj) * (uint64_t)((N + d - 1) / d);
         }
         if (tiles >= 4096) break;
     }
@@ -80,7 +84,8 @@ static int build_tables(scp_batch *b) {
     for (size_t m = 0; m < n; m++) {
         const scp_grid *g = b->members[m];
         const int N = g->p.N, di = grid_di(b, g);
-        const uint32_t nt = march_tiles_k(N) * march_tiles_j(N) * (uint32_t)((N + di - 1) / di);
+        const uint32_t nt = march_tiles_k(N, tile_kw(b->shape)) * march_tiles_j(N, tile_tj(b->shape)) *
+                            (uint32_t)((N + di - 1) / di);
         for (uint32_t t = 0; t < nt; t++) tiles.push_back({(uint32_t)m, t});
     }
     drop_graphs(b);
@@ -123,15 +128,17 @@ static void fill_desc(BatchGrid<B> *d, const scp_batch *b, scp_grid *g, int nq, 
     d->cur0 = cur;
     d->nq = nq;
     d->emit = (g->xo && g->vo) ? 1 : 0;
-    d->ntk = march_tiles_k(g->p.N);
-    d->ntj = march_tiles_j(g->p.N);
+    d->ntk = march_tiles_k(g->p.N, tile_kw(b->shape));
+    d->ntj = march_tiles_j(g->p.N, tile_tj(b->shape));
 }
 
 static int launch_one(scp_batch *b, int q) {
-    const uint32_t nblocks = b->persistent ? std::min<uint32_t>(b->ntiles, (uint32_t)(b->f32 ? 2 : 1) * b->ncu)
-                                           : b->ntiles;
+    const uint32_t nblocks =
+        b->persistent ? std::min<uint32_t>(b->ntiles, (uint32_t)tile_blocks_per_cu(b->shape, b->f32) * b->ncu)
+                      : b->ntiles;
     const uint32_t per = (b->ntiles + nblocks - 1) / nblocks;
-    const int rc = b->f32 ? launch_march_batch_f32c(b->step_mode, b->lo_kind, (const BatchGrid<Bufs32> *)b->d_grids,
+    const int rc = b->f32 ? launch_march_batch_f32c(b->step_mode, b->lo_kind, b->shape,
+                                                    (const BatchGrid<Bufs32> *)b->d_grids,
                                                     b->d_tiles, b->ntiles, nblocks, per, q, b->stream)
                           : launch_march_batch_f64(b->step_mode, (const BatchGrid<Bufs<double>> *)b->d_grids,
                                                    b->d_tiles, b->ntiles, nblocks, per, q, b->stream);
@@ -255,6 +262,7 @@ extern "C" int scp_batch_add(scp_batch *b, scp_grid *g) {
     }
     b->members.push_back(g);
     g->batch = b;
+    if (!shape_ok(b, b->shape)) b->shape = TILE_64x4;
     b->stale = true;
     return 0;
 }
@@ -312,37 +320,69 @@ extern "C" int scp_batch_config(const scp_batch *b, int *di, int *persistent, in
     return 0;
 }
 
+// tile shapes a batch of this class may use
+static bool shape_ok(const scp_batch *b, int shape) {
+    if (shape == TILE_64x4) return true;
+    return shape > 0 && shape < TILE_NSHAPES && b->f32 == 1 && b->lo_kind == LO_Q8P;
+}
+
+extern "C" int scp_batch_set_tile(scp_batch *b, int kw, int tj) {
+    if (!b) return set_err(-1, "null batch");
+    int shape = -1;
+    for (int q = 0; q < TILE_NSHAPES; q++)
+        if (tile_kw(q) == kw && tile_tj(q) == tj) shape = q;
+    if (shape < 0) return set_err(-1, "tile %d x %d: the shapes are 64 x 4, 128 x 4, 64 x 8, 64 x 2", 64 * kw, tj);
+    if (!b->members.empty() && !shape_ok(b, shape))
+        return set_err(-1, "tile %d x %d runs FP32C batches with int8 position words only", 64 * kw, tj);
+    b->shape = shape;
+    b->stale = true;
+    return 0;
+}
+
+extern "C" int scp_batch_tile(const scp_batch *b, int *kw, int *tj) {
+    if (!b) return set_err(-1, "null batch");
+    if (kw) *kw = tile_kw(b->shape);
+    if (tj) *tj = tile_tj(b->shape);
+    return 0;
+}
+
 extern "C" int scp_batch_tune(scp_batch *b, int *steps_taken) {
     if (!b) return set_err(-1, "null batch");
     if (steps_taken) *steps_taken = 0;
     if (b->members.empty()) return set_err(-1, "empty batch");
-    // candidates: the default depth and 16..128 planes, each as one block per
-    // tile and as persistent blocks; each timed on a 3-step chain after one
-    // untimed step (tables, caches)
+    // candidates: the tile shapes of the class x the default depth and 16..128
+    // planes x one block per tile / persistent blocks; each timed on a 3-step
+    // chain after one untimed step (tables, caches)
     static const int DIS[] = {0, 16, 32, 64, 128};
     const int REPS = 3;
     std::vector<int> one(b->members.size(), 1), reps(b->members.size(), REPS);
     float best = 1e30f;
-    int bdi = 0, bper = 0, taken = 0;
-    for (int per = 0; per < 2; per++) {
-        for (int di : DIS) {
-            b->DI = di;
-            b->persistent = per;
-            b->stale = true;
-            int rc = run_chain(b, one.data(), false);
-            if (rc) return rc;
-            rc = run_chain(b, reps.data(), false, true);
-            if (rc) return rc;
-            float ms = 0;
-            HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-            taken += 1 + REPS;
-            if (ms < best) {
-                best = ms;
-                bdi = di;
-                bper = per;
+    int bdi = 0, bper = 0, bshape = TILE_64x4, taken = 0;
+    for (int shape = 0; shape < TILE_NSHAPES; shape++) {
+        if (!shape_ok(b, shape)) continue;
+        for (int per = 0; per < 2; per++) {
+            for (int di : DIS) {
+                b->shape = shape;
+                b->DI = di;
+                b->persistent = per;
+                b->stale = true;
+                int rc = run_chain(b, one.data(), false);
+                if (rc) return rc;
+                rc = run_chain(b, reps.data(), false, true);
+                if (rc) return rc;
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+                taken += 1 + REPS;
+                if (ms < best) {
+                    best = ms;
+                    bdi = di;
+                    bper = per;
+                    bshape = shape;
+                }
             }
         }
     }
+    b->shape = bshape;
     b->DI = bdi;
     b->persistent = bper;
     b->stale = true;

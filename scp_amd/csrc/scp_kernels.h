@@ -183,11 +183,22 @@ int launch_march_f32c_kw(int kw, int mode, bool emit, int lo_kind, const KParams
 // [b*per_block, (b+1)*per_block) of the table (XCD-aware block order).
 int launch_march_batch_f64(int mode, const BatchGrid<Bufs<double>> *grids, const BatchTile *tiles,
                            uint32_t ntiles, uint32_t nblocks, uint32_t per_block, int q, hipStream_t s);
-int launch_march_batch_f32c(int mode, int lo_kind, const BatchGrid<Bufs32> *grids, const BatchTile *tiles,
-                            uint32_t ntiles, uint32_t nblocks, uint32_t per_block, int q, hipStream_t s);
+int launch_march_batch_f32c(int mode, int lo_kind, int shape, const BatchGrid<Bufs32> *grids,
+                            const BatchTile *tiles, uint32_t ntiles, uint32_t nblocks, uint32_t per_block, int q,
+                            hipStream_t s);
 // tiles per plane of the march at N (k, j), for building tile tables
-inline uint32_t march_tiles_k(int N) { return (uint32_t)(N + 63) / 64; }
-inline uint32_t march_tiles_j(int N) { return (uint32_t)(N + 3) / 4; }
+inline uint32_t march_tiles_k(int N, int kw = 1) { return (uint32_t)(N + 64 * kw - 1) / (uint32_t)(64 * kw); }
+inline uint32_t march_tiles_j(int N, int tj = 4) { return (uint32_t)(N + tj - 1) / (uint32_t)tj; }
+// tile shapes of the batched march: 64 x 4 (every class), and for FP32C grids
+// with int8 position words 128 x 4, 64 x 8 and 64 x 2
+enum { TILE_64x4 = 0, TILE_128x4 = 1, TILE_64x8 = 2, TILE_64x2 = 3, TILE_NSHAPES = 4 };
+inline int tile_kw(int shape) { return shape == TILE_128x4 ? 2 : 1; }
+inline int tile_tj(int shape) { return shape == TILE_64x8 ? 8 : (shape == TILE_64x2 ? 2 : 4); }
+// resident blocks per CU of a shape (its staged planes in the 160 KB of LDS)
+inline int tile_blocks_per_cu(int shape, bool f32) {
+    if (!f32) return 1;
+    return shape == TILE_64x4 ? 2 : (shape == TILE_64x2 ? 3 : 1);
+}
 // default planes per block of the march (64*KW x 4 tiles): deep enough to
 // amortise the 3-plane prologue (3/DI extra plane reads), shallow enough for
 // >= ~16 blocks per CU (measured at 512^3: DI = 128, 4096 blocks, within 1 %

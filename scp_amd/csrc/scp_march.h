@@ -37,18 +37,20 @@
 
 namespace scp {
 
-constexpr int MTJ = 4;                 // j rows per block
-constexpr int MR = MTJ + 2;            // staged rows
+constexpr int MTJ = 4;                 // j rows per block (the single-grid launches)
 constexpr int MNBUF = 3;
 
 // Tile geometry: KW waves along k per row (a row of 64*KW sites; wider rows
-// give the HBM longer contiguous runs per field), MTJ rows.
-template <int KW>
+// give the HBM longer contiguous runs per field), TJ rows (4 for single grids;
+// batches may pick 2 or 8).
+template <int KW, int TJ = MTJ>
 struct MG {
     static constexpr int TKM = TK * KW;        // sites along k
+    static constexpr int ROWS = TJ;            // rows of the tile
+    static constexpr int MR = TJ + 2;          // staged rows
     static constexpr int MC = TKM + 2;         // staged columns
     static constexpr int MPL = MR * MC;        // staged cells per field
-    static constexpr int MNT = TKM * MTJ;      // threads per block
+    static constexpr int MNT = TKM * TJ;       // threads per block
     static constexpr int NW = MNT / 64;        // waves per block
 };
 
@@ -60,15 +62,15 @@ struct MTile {
 __device__ inline int wrapc(int x, int N) { return x < 0 ? x + N : (x >= N ? x - N : x); }
 
 // tile (tk, tj) of segment seg of an N^3 grid; see MG for the tile size
-template <int KW>
+template <int KW, int TJ = MTJ>
 __device__ inline MTile mtile(int N, int DI, uint32_t tk, uint32_t tj, uint32_t seg) {
-    using G = MG<KW>;
+    using G = MG<KW, TJ>;
     MTile t;
     t.N = N;
     t.k0 = tk * G::TKM;
-    t.j0 = tj * MTJ;
+    t.j0 = tj * TJ;
     t.W = min(G::TKM, N - t.k0);
-    t.H = min(MTJ, N - t.j0);
+    t.H = min(TJ, N - t.j0);
     t.i0 = seg * DI;
     t.i1 = min(N, t.i0 + DI);
     t.k = t.k0 + threadIdx.x;
@@ -105,11 +107,13 @@ __device__ inline void sto_nt(V *base, uint32_t e, V v) {
 //     row segment per load;
 //   columns (k0-1 and k0+W, rows j0-1 .. j0+H): one (field, side, row) cell
 //     per thread for the first NF*2*MR threads.
-template <int NF, int KW>
+template <int NF, int KW, int TJ = MTJ>
 struct Halo {
-    using G = MG<KW>;
+    using G = MG<KW, TJ>;
+    static constexpr int MR = G::MR;
     static constexpr int NSEG = NF * 2 * KW;              // row segments per plane
     static constexpr int RQ = (NSEG + G::NW - 1) / G::NW; // row segments per wave
+    static_assert(NF * 2 * MR <= G::MNT, "one halo column cell per thread");
     int wave, lane, W;
     uint32_t rtop, rbot;      // in-plane element offsets (padded rows) of this lane's top/bottom halo cell (half 0)
     int lrow_top, lrow_bot;   // LDS offsets (field 0, half 0) of this lane's halo row cells
@@ -151,16 +155,16 @@ struct Halo {
 };
 
 // own column of a staged plane, indexable by field
-template <typename R, int KW>
+template <typename R, int KW, int TJ = MTJ>
 struct LdsCol {
     const R *p;
-    __device__ R operator[](int f) const { return p[f * MG<KW>::MPL]; }
+    __device__ R operator[](int f) const { return p[f * MG<KW, TJ>::MPL]; }
 };
 
 // neighbour provider over the staged planes (see GlobalNB for the contract)
-template <typename R, int KW>
+template <typename R, int KW, int TJ = MTJ>
 struct LdsNB {
-    static constexpr int MPL = MG<KW>::MPL, MC = MG<KW>::MC;
+    static constexpr int MPL = MG<KW, TJ>::MPL, MC = MG<KW, TJ>::MC;
     const R *c;   // plane i buffer, at the own cell
     const R *n;   // plane i+1 buffer, at the own cell
     __device__ R face(int f, int d, int sg) const { return c[f * MPL + (d == 1 ? sg * MC : sg)]; }
@@ -322,8 +326,8 @@ constexpr int march_nf() { return MODE == 2 ? 15 : (MODE == 1 ? 12 : 6); }   // 
 // MODE: 0 real, 1 complex, 2 gauged, 3 real with the pass-1 fields (alpha_cs,
 // beta_h, sigma_freq: an auxiliary array read from global memory; a separate
 // instance so that the plain real step keeps its registers)
-template <class S, int KW, int MODE>
-constexpr int march_lds_elems() { return MNBUF * march_nf<MODE>() * MG<KW>::MPL; }
+template <class S, int KW, int MODE, int TJ = MTJ>
+constexpr int march_lds_elems() { return MNBUF * march_nf<MODE>() * MG<KW, TJ>::MPL; }
 
 // Per-thread diagnostics accumulators of the DIAG instance (DS_* sums, DM_* maxima).
 struct DiagState {
@@ -364,20 +368,20 @@ __device__ inline void gauss_pending(const KParams &kp, const DiagArgs *da, Diag
 // next tile into the same area. DIAG (with EMIT): the emitted state's per-site
 // diagnostics are accumulated into ds, and the gauged charge density is
 // written to da->rho.
-template <class S, int KW, int MODE, bool EMIT, bool DIAG = false>
+template <class S, int KW, int MODE, bool EMIT, bool DIAG = false, int TJ = MTJ>
 __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::B &b, int DI, uint32_t tk,
                                            uint32_t tj, uint32_t seg, typename S::T *lds,
                                            const DiagArgs *da = nullptr, DiagState *ds = nullptr) {
     using T = typename S::T;
     constexpr int NF = march_nf<MODE>();
     constexpr int NM = MODE == 2 ? 12 : NF;   // fields of the i-1 plane (gauged: matter only)
-    constexpr int MPL = MG<KW>::MPL;
-    constexpr int RQ = Halo<NF, KW>::RQ;
+    constexpr int MPL = MG<KW, TJ>::MPL;
+    constexpr int RQ = Halo<NF, KW, TJ>::RQ;
 
     const int N = kp.N;
-    const MTile t = mtile<KW>(N, DI, tk, tj, seg);
+    const MTile t = mtile<KW, TJ>(N, DI, tk, tj, seg);
     if (t.i0 >= N) return;                    // whole block: uniform
-    Halo<NF, KW> hs;
+    Halo<NF, KW, TJ> hs;
     hs.init(t, kp.pitch);
     // step state: padded rows (kp.pitch), plane stride kp.plane, field stride kp.fsh;
     // emitted state: unpadded (N, N*N, kp.fs)
@@ -511,7 +515,7 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
 
         // own column of planes i and i+1: registers (REGQ) or read back from the staged tiles
         auto compute = [&](const auto &c0, const auto &c1) {
-            const LdsNB<T, KW> nb{lds + bb * NF * MPL + t.own, lds + nbuf * NF * MPL + t.own};
+            const LdsNB<T, KW, TJ> nb{lds + bb * NF * MPL + t.own, lds + nbuf * NF * MPL + t.own};
             T acc[NF];
             if constexpr (MODE == 2)
                 forces_gauge(kp, c0, c1, cm, nb, cy, acc);
@@ -585,7 +589,7 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
 #pragma unroll
             for (int f = 0; f < NF; f++) { cq[f] = cp[f]; cp[f] = cpp[f]; }
         } else {
-            const LdsCol<T, KW> c0{lds + bb * NF * MPL + t.own}, c1{lds + nbuf * NF * MPL + t.own};
+            const LdsCol<T, KW, TJ> c0{lds + bb * NF * MPL + t.own}, c1{lds + nbuf * NF * MPL + t.own};
             if (t.valid) compute(c0, c1);
 #pragma unroll
             for (int f = 0; f < NM; f++) cm[f] = c0[f];
@@ -685,11 +689,16 @@ inline int launch_march_diag_t(const KParams &kp, const typename S::B &b, int DI
 // (b in XCD-aware order, so an XCD's blocks take adjacent tiles); a grid takes
 // part in chain launches q < nq and emits its FP64 state on the last (if it
 // holds the emit buffers: grids of the request loop release them in between).
-template <class S, int KW, int MODE>
-__global__ void __launch_bounds__(MG<KW>::MNT, KW == 1 ? S::MINB : 1)
+// Tiles are 64*KW x TJ sites; blocks per CU: the default 64 x 4 tile fits
+// S::MINB (its staged planes), 64 x 2 three, the larger ones one.
+template <class S, int KW, int TJ>
+constexpr int march_minb() { return (KW == 1 && TJ == MTJ) ? S::MINB : ((KW == 1 && TJ < MTJ) ? 3 : 1); }
+
+template <class S, int KW, int MODE, int TJ = MTJ>
+__global__ void __launch_bounds__((MG<KW, TJ>::MNT), (march_minb<S, KW, TJ>()))
 k_march_batch(const BatchGrid<typename S::B> *__restrict__ grids, const BatchTile *__restrict__ tiles,
               uint32_t ntiles, uint32_t per_block, int q) {
-    __shared__ typename S::T lds[march_lds_elems<S, KW, MODE>()];
+    __shared__ typename S::T lds[march_lds_elems<S, KW, MODE, TJ>()];
     const uint32_t b = logical_block(blockIdx.x, gridDim.x);
     const uint32_t t1 = min(ntiles, (b + 1) * per_block);
     for (uint32_t t = b * per_block; t < t1; t++) {
@@ -700,17 +709,17 @@ k_march_batch(const BatchGrid<typename S::B> *__restrict__ grids, const BatchTil
         const uint32_t tj = r % g.ntj, seg = r / g.ntj;
         const int par = (g.cur0 + q) & 1;
         if (q == g.nq - 1 && g.emit)
-            march_tile<S, KW, MODE, true>(g.kp, g.buf[par], g.DI, tk, tj, seg, lds);
+            march_tile<S, KW, MODE, true, false, TJ>(g.kp, g.buf[par], g.DI, tk, tj, seg, lds);
         else
-            march_tile<S, KW, MODE, false>(g.kp, g.buf[par], g.DI, tk, tj, seg, lds);
+            march_tile<S, KW, MODE, false, false, TJ>(g.kp, g.buf[par], g.DI, tk, tj, seg, lds);
     }
 }
 
-template <class S, int MODE>
+template <class S, int MODE, int KW = 1, int TJ = MTJ>
 inline int launch_march_batch_t(const BatchGrid<typename S::B> *grids, const BatchTile *tiles, uint32_t ntiles,
                                 uint32_t nblocks, uint32_t per_block, int q, hipStream_t s) {
-    hipLaunchKernelGGL((k_march_batch<S, 1, MODE>), dim3(nblocks), dim3(MG<1>::TKM, MTJ), 0, s, grids, tiles,
-                       ntiles, per_block, q);
+    hipLaunchKernelGGL((k_march_batch<S, KW, MODE, TJ>), dim3(nblocks), dim3(MG<KW, TJ>::TKM, TJ), 0, s, grids,
+                       tiles, ntiles, per_block, q);
     return (int)hipGetLastError();
 }
 

@@ -54,22 +54,32 @@ int launch_march_f32c_diag(int mode, int lo_kind, const KParams &kp, const Bufs3
     return diag_lo<LO_NONE>(mode, kp, b, DI, da, s);
 }
 
-template <int LOK>
+template <int LOK, int KW, int TJ>
 static int batch_lo(int mode, const BatchGrid<Bufs32> *grids, const BatchTile *tiles, uint32_t ntiles,
                     uint32_t nblocks, uint32_t per_block, int q, hipStream_t s) {
-    return mode == 2   ? launch_march_batch_t<S32<LOK>, 2>(grids, tiles, ntiles, nblocks, per_block, q, s)
-           : mode == 1 ? launch_march_batch_t<S32<LOK>, 1>(grids, tiles, ntiles, nblocks, per_block, q, s)
-                       : launch_march_batch_t<S32<LOK>, 0>(grids, tiles, ntiles, nblocks, per_block, q, s);
+    return mode == 2   ? launch_march_batch_t<S32<LOK>, 2, KW, TJ>(grids, tiles, ntiles, nblocks, per_block, q, s)
+           : mode == 1 ? launch_march_batch_t<S32<LOK>, 1, KW, TJ>(grids, tiles, ntiles, nblocks, per_block, q, s)
+                       : launch_march_batch_t<S32<LOK>, 0, KW, TJ>(grids, tiles, ntiles, nblocks, per_block, q, s);
 }
 
-int launch_march_batch_f32c(int mode, int lo_kind, const BatchGrid<Bufs32> *grids, const BatchTile *tiles,
-                            uint32_t ntiles, uint32_t nblocks, uint32_t per_block, int q, hipStream_t s) {
+// shape (TILE_*): 64 x 4 for every storage kind; the other shapes for the
+// production words (int8 on the positions) only
+int launch_march_batch_f32c(int mode, int lo_kind, int shape, const BatchGrid<Bufs32> *grids,
+                            const BatchTile *tiles, uint32_t ntiles, uint32_t nblocks, uint32_t per_block, int q,
+                            hipStream_t s) {
     if (mode < 0 || mode > 2) return (int)hipErrorInvalidValue;
-    if (lo_kind == LO_F32) return batch_lo<LO_F32>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
-    if (lo_kind == LO_BF16) return batch_lo<LO_BF16>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
-    if (lo_kind == LO_Q8) return batch_lo<LO_Q8>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
-    if (lo_kind == LO_Q8P) return batch_lo<LO_Q8P>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
-    return batch_lo<LO_NONE>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
+    if (shape != TILE_64x4) {
+        if (lo_kind != LO_Q8P) return (int)hipErrorInvalidValue;
+        if (shape == TILE_128x4) return batch_lo<LO_Q8P, 2, 4>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
+        if (shape == TILE_64x8) return batch_lo<LO_Q8P, 1, 8>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
+        if (shape == TILE_64x2) return batch_lo<LO_Q8P, 1, 2>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
+        return (int)hipErrorInvalidValue;
+    }
+    if (lo_kind == LO_F32) return batch_lo<LO_F32, 1, 4>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
+    if (lo_kind == LO_BF16) return batch_lo<LO_BF16, 1, 4>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
+    if (lo_kind == LO_Q8) return batch_lo<LO_Q8, 1, 4>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
+    if (lo_kind == LO_Q8P) return batch_lo<LO_Q8P, 1, 4>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
+    return batch_lo<LO_NONE, 1, 4>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
 }
 
 }  // namespace scp

@@ -43,7 +43,8 @@ ABI_SYMBOLS = ["scp_abi_version", "scp_last_error", "scp_grid_bytes", "scp_grid_
                "scp_server_job", "scp_server_take", "scp_server_stats_get", "scp_server_now",
                "scp_grid_vec_fit", "scp_grid_set_kappa", "scp_grid_set_damp_rate",
                "scp_grid_step_diag_noalloc", "scp_qflow_create", "scp_qflow_destroy", "scp_qflow_upload",
-               "scp_qflow_download", "scp_qflow_norm", "scp_qflow_run", "scp_server_set_aging"]
+               "scp_qflow_download", "scp_qflow_norm", "scp_qflow_run", "scp_server_set_aging",
+               "scp_batch_set_tile", "scp_batch_tile"]
 
 
 class ScpError(RuntimeError):
@@ -185,6 +186,8 @@ def lib():
         L.scp_batch_tune.argtypes = [VP, IP]
         L.scp_batch_set_config.argtypes = [VP, C.c_int, C.c_int]
         L.scp_batch_config.argtypes = [VP, IP, IP, IP]
+        L.scp_batch_set_tile.argtypes = [VP, C.c_int, C.c_int]
+        L.scp_batch_tile.argtypes = [VP, IP, IP]
         L.scp_grid_seed_balls.argtypes = [VP, C.POINTER(QBall), C.c_int, C.c_double, C.c_double, C.c_uint64]
         L.scp_server_create.argtypes = [C.c_int64, C.c_int, C.c_int, C.POINTER(VP)]
         L.scp_server_destroy.argtypes = [VP]
@@ -382,11 +385,18 @@ class Batch:
     def set_config(self, di=0, persistent=False):
         _check(lib().scp_batch_set_config(self._h, int(di), 1 if persistent else 0), "scp_batch_set_config")
 
+    def set_tile(self, kw, tj):
+        """Tile of the batched launch: 64*kw x tj sites (scp_batch_set_tile)."""
+        _check(lib().scp_batch_set_tile(self._h, int(kw), int(tj)), "scp_batch_set_tile")
+
     @property
     def config(self):
         di, per, tuned = C.c_int(), C.c_int(), C.c_int()
         _check(lib().scp_batch_config(self._h, C.byref(di), C.byref(per), C.byref(tuned)), "scp_batch_config")
-        return {"di": di.value, "persistent": bool(per.value), "tuned": bool(tuned.value)}
+        kw, tj = C.c_int(), C.c_int()
+        _check(lib().scp_batch_tile(self._h, C.byref(kw), C.byref(tj)), "scp_batch_tile")
+        return {"di": di.value, "persistent": bool(per.value), "tuned": bool(tuned.value),
+                "tile": (64 * kw.value, tj.value)}
 
     def close(self):
         if self._h:
