@@ -388,7 +388,7 @@ def sweep_bench(a):
     print(json.dumps(out), flush=True)
 
 
-def serve_bench(a):
+def serve_bench(a, emit=True):
     """Serving: Poisson job arrivals (rate a.rate per second for a.duration
     seconds); each job a gauged periodic grid of edge log-uniform in
     [48, 256], steps uniform in [steps_lo, steps_hi], 1-4 seeded Q-balls;
@@ -469,7 +469,9 @@ def serve_bench(a):
                          "max": float(np.nanmax(lat))},
            "offered_gsite_updates_per_s": offered / a.duration / 1e9,
            "busy_fraction": st["busy_seconds"] / elapsed, "elapsed_s": elapsed}
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def main():
@@ -487,6 +489,8 @@ def main():
                     help="step workload: gauged (the headline) or real (6-field real-mode step)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="step workload: skip the short serving window that adds p50/p99 job latency")
     ap.add_argument("--drift-steps", type=int, default=10000, help="drift: steps of the long run")
     ap.add_argument("--chunk", type=int, default=1000, help="drift: steps between diagnostics")
     ap.add_argument("--workload", default="step", choices=["step", "sweep", "serve", "drift"],
@@ -538,6 +542,21 @@ def main():
     if a.prec == "f32c" and not a.no_compare:
         cmp64 = run_one(G, N, L, st, "f64", 0, a.steps, a.warmup, a.mode)
     del st
+    serving = None
+    if a.mode == "gauged" and not a.no_serve:
+        # the metric's latency half: a short window of the serving workload (Poisson
+        # arrivals at 4 jobs/s, the job mix of the full serving run, at most 16 jobs on
+        # the GPU so that jobs complete within the window); the full 600 s run is
+        # `--workload serve`
+        import copy
+        sa = copy.copy(a)
+        sa.rate, sa.duration, sa.horizon, sa.max_active = 4.0, 15.0, 45.0, 16
+        log("serving window: 15 s of arrivals at 4 jobs/s, 45 s horizon, at most 16 active")
+        sv = serve_bench(sa, emit=False)
+        serving = {"p50_latency_s": sv["latency_s"]["p50"], "p99_latency_s": sv["latency_s"]["p99"],
+                   "aggregate_gsite_updates_per_s": sv["value"], "jobs": sv["jobs"],
+                   "window": "Poisson 4 jobs/s for 15 s + 45 s horizon, at most 16 active, HBM budget 250 GB; "
+                             "gauged FP32C jobs, edge log-uniform [48,256], steps U[1000,20000], 1-4 Q-balls"}
     cpu = None if a.no_cpu_baseline else cpu_baseline(a.mode)
     dtype = "f64" if a.prec == "f64" else f"f32c ({LO_NAMES[a.lo_words]} compensation words)"
     out = {
@@ -556,6 +575,7 @@ def main():
                      "stored_frac": head["stored_GBps"] / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
         "diag_step_ms": head["diag_step_ms"],
+        "serving": serving,
     }
     if cmp64:
         out["fp64"] = {"value": cmp64["value"], "ms_per_step": cmp64["ms_per_step"],

@@ -116,11 +116,13 @@ int  scp_grid_diag(scp_grid *g, double *out, int nout);
 /* scp_grid_step followed by scp_grid_diag of the resulting state, with the
  * diagnostics fused into the step's last launch (grids on the LDS-staged
  * march: every energy, charge and moment sum is reduced by the step kernel
- * itself; gauged, the Gauss-law statistics take one light pass over E).
+ * itself; gauged, so are the Gauss-law statistics of every site whose
+ * backward neighbours the same block emits, and a light pass adds those of
+ * the tile and segment seams).
  * Replaces the main loop's verlet_step + diagnostics pair on a diag step
  * (scp_sim.c:3082 and 2993-3065; scp_sim.cu:5492, 3697-4041). Other grids
  * run the step and the stand-alone pass. The first call on a gauged grid
- * allocates N^3 doubles of charge density. */
+ * allocates N^3 doubles for the seam sites' charge density. */
 int  scp_grid_step_diag(scp_grid *g, int nsteps, double *out, int nout);
 /* scp_grid_step_diag without any device allocation. No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under
@@ -292,7 +294,7 @@ double scp_server_now(const scp_server *s);
 int  scp_server_set_aging(scp_server *s, int rounds);
 
 /* Fixed-charge gradient flow: the relaxer sfa/seed/eta_qflow.c (its fixed-Q
- * mode, eta_qflow.c:242-375) on the device. A complex-mode state of 12 fields
+ * mode, eta_qflow.c:242-375, and its fixed-(Q, J) mode, below) on the device. A complex-mode state of 12 fields
  * (u_a, v_a = Re, Im Phi_a; tu_a, tv_a = Re, Im Theta_a; a = 0..2) flows by
  * x <- x + dtau F with F the fixed-charge forces (7-point Laplacian, central
  * curl, periodic), omega = Q / N recomputed from the current state every
@@ -328,6 +330,29 @@ int  scp_qflow_norm(scp_qflow *f, double *n_phi, double *n_theta);
  * multiplier of the last iteration (the rotating seed's velocities use it). */
 int  scp_qflow_run(scp_qflow *f, double Q, double dtau, int iters, int printev, scp_qflow_row *rows, int maxrows,
                    int *nrows, double *omega_last);
+
+/* One printed row of the fixed-(Q, J) mode (eta_qflow.c:353-357): omega and
+ * Omega of that iteration, E_QJ of the updated state, max |F|, s_max, N of that
+ * iteration, J_act = omega ell - Omega K, cond = (N K - ell^2) / (N K). */
+typedef struct {
+    int32_t iter;
+    int32_t reserved;
+    double omega, Omega, E_QJ, maxF, s_max, N, J_act, cond;
+} scp_qflow_row_j;
+
+/* The fixed-(Q, J) flow (eta_qflow.c fixJ = 1, :207-262, :294-310, :377-399):
+ * every iteration forms Jhat w = x d_y w - y d_x w - zhat x w of the 12 fields,
+ * N, ell = int (u.Jv - v.Ju + tu.Jtv - tv.Jtu), K = int |Jw|^2, the multipliers
+ * (omega, Omega) of Q = omega N - Omega ell, J = omega ell - Omega K (Tikhonov
+ * 1e-10, |Omega| <= 0.15), and adds the Coriolis and centrifugal forces masked
+ * at the light cylinder. Rows as scp_qflow_run. last[0..5) (may be NULL):
+ * omega, Omega, cond of the last iteration, ell and K of the final state. */
+int  scp_qflow_run_fixj(scp_qflow *f, double Q, double J, double dtau, int iters, int printev,
+                        scp_qflow_row_j *rows, int maxrows, int *nrows, double *last);
+/* Jhat of the current state's 12 fields (ncols = 12), after scp_qflow_run_fixj
+ * (the rotating seed's velocities carry -Omega wm Jhat, eta_qflow.c:447-460);
+ * an error if the state changed since. */
+int  scp_qflow_download_jhat(scp_qflow *f, double *const *cols, int ncols);
 
 #ifdef __cplusplus
 }

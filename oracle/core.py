@@ -68,6 +68,9 @@ def _lib():
     lib.or_qflow_norm.restype = C.c_double
     lib.or_qflow_norm.argtypes = [Q, D]
     lib.or_qflow.argtypes = [Q, D, C.c_double, C.c_double, C.c_int, C.c_int, D, C.c_int, D]
+    lib.or_qflow_j.argtypes = [Q, D, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, D, C.c_int, D, D]
+    lib.or_qflow_wmask.restype = C.c_double
+    lib.or_qflow_wmask.argtypes = [Q, C.c_int, C.c_int]
     return lib
 
 
@@ -265,3 +268,23 @@ def qflow(p, state, Q, dtau, iters, printev):
     n = lib().or_qflow(C.byref(p), _ptr(st), float(Q), float(dtau), int(iters), int(printev), _ptr(rows),
                        maxrows, C.byref(om))
     return st, [tuple(r) for r in rows[:n]], om.value
+
+
+def qflow_j(p, state, Q, Jt, dtau, iters, printev):
+    """The fixed-(Q, J) flow (eta_qflow.c fixJ = 1): returns (state, rows, last, jhat);
+    rows: (it, omega, Omega, E_QJ, maxF, s_max, N, J_act, cond); last: omega, Omega,
+    cond of the last iteration, ell and K of the final state; jhat: Jhat of the
+    final state's 12 fields."""
+    st = np.array(state, dtype=np.float64).reshape(12, -1).copy()
+    maxrows = iters // max(1, printev) + 2
+    rows = np.zeros((maxrows, 9))
+    last = np.zeros(5)
+    jh = np.zeros_like(st)
+    n = lib().or_qflow_j(C.byref(p), _ptr(st), float(Q), float(Jt), float(dtau), int(iters), int(printev),
+                         _ptr(rows), maxrows, _ptr(last), _ptr(jh))
+    return st, [tuple(r) for r in rows[:n]], tuple(last), jh
+
+
+def qflow_wmask(p):
+    """The light-cylinder mask (eta_qflow.c:213-217) as an N x N table over (i, j)."""
+    return np.array([[lib().or_qflow_wmask(C.byref(p), i, j) for j in range(p.N)] for i in range(p.N)])

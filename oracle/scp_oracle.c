@@ -1026,3 +1026,178 @@ int or_qflow(const OrQflow *p, double *st, double Q, double dtau, int iters, int
     if (omega_last) *omega_last = omega;
     return nrows;
 }
+
+double or_qflow_wmask(const OrQflow *p, int i, int j) {
+    const double dx = 2.0 * p->L / (p->N - 1);
+    const double x = -p->L + i * dx, y = -p->L + j * dx, rho = sqrt(x * x + y * y);
+    return 1.0 / (1.0 + exp((rho - 11.0) / 0.5));           /* eta_qflow.c:215-216 */
+}
+
+/* Jhat of the 12 fields of `st` into jh, with the sums N, ell, K in site order
+ * (eta_qflow.c:232-252 and :381-396; the same expressions, the sums as one thread
+ * of the reference accumulates them) */
+static void qflow_jhat(const OrQflow *p, const double *st, double *jh, double *Nn, double *el, double *kk) {
+    const int N = p->N;
+    const long N3 = (long)N * N * N, NN = (long)N * N;
+    const double L = p->L, dx = 2.0 * L / (N - 1), inv2dx = 1.0 / (2.0 * dx);
+    double sn = 0, se = 0, sk = 0;
+#define WR(a) (((a) + N) % N)
+#define IX3(i, j, k) (((long)(i) * N + (j)) * N + (k))
+    for (long ii = 0; ii < N3; ii++) {
+        const int i = (int)(ii / NN), j = (int)((ii / N) % N), k = (int)(ii % N);
+        const long Xp = IX3(WR(i + 1), j, k), Xm = IX3(WR(i - 1), j, k);
+        const long Yp = IX3(i, WR(j + 1), k), Ym = IX3(i, WR(j - 1), k);
+        const double x = -L + i * dx, y = -L + j * dx;
+        for (int a = 0; a < 3; a++) {
+            double jw[4];
+            for (int q = 0; q < 4; q++) {                      /* u, v, tu, tv */
+                const double *w = st + (long)(3 * q) * N3;
+                const double spin = a == 0 ? w[N3 + ii] : a == 1 ? -w[ii] : 0.0;
+                const double *wa = w + a * N3;
+                jw[q] = x * (wa[Yp] - wa[Ym]) * inv2dx - y * (wa[Xp] - wa[Xm]) * inv2dx + spin;
+                jh[(long)(3 * q + a) * N3 + ii] = jw[q];
+            }
+            const double u = st[a * N3 + ii], v = st[(3 + a) * N3 + ii];
+            const double tu = st[(6 + a) * N3 + ii], tv = st[(9 + a) * N3 + ii];
+            if (Nn) sn += u * u + v * v + tu * tu + tv * tv;
+            se += u * jw[1] - v * jw[0] + tu * jw[3] - tv * jw[2];
+            sk += jw[0] * jw[0] + jw[1] * jw[1] + jw[2] * jw[2] + jw[3] * jw[3];
+        }
+    }
+#undef WR
+#undef IX3
+    if (Nn) *Nn = sn;
+    *el = se;
+    *kk = sk;
+}
+
+int or_qflow_j(const OrQflow *p, double *st, double Q, double Jt, double dtau, int iters, int printev, double *rows,
+               int maxrows, double *out, double *jhat_out) {
+    const int N = p->N;
+    const long N3 = (long)N * N * N, NN = (long)N * N;
+    const double L = p->L, dx = 2.0 * L / (N - 1), dx2 = dx * dx, dV = dx * dx * dx;
+    const double invdx2 = 1.0 / dx2, inv2dx = 1.0 / (2.0 * dx);
+    const double eta = p->eta, m = p->m, mth = p->mth, mu = p->mu, kappa = p->kappa;
+    const double OMEGA_CAP = 0.15;                            /* eta_qflow.c:208 */
+    double *nx = malloc(sizeof(double) * 12 * (size_t)N3);
+    double *jh = malloc(sizeof(double) * 12 * (size_t)N3);
+    double *wmask = malloc(sizeof(double) * (size_t)NN);
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) wmask[(long)i * N + j] = or_qflow_wmask(p, i, j);
+    double *cur = st, *nxt = nx;
+    int nrows = 0;
+    double omega = 0, Omega = 0, ell = 0, Kk = 0, cond = 1.0;
+#define WR(a) (((a) + N) % N)
+#define IX3(i, j, k) (((long)(i) * N + (j)) * N + (k))
+    for (int it = 0; it <= iters; it++) {
+        const double *u = cur, *v = cur + 3 * N3, *tu = cur + 6 * N3, *tv = cur + 9 * N3;
+        double Nn, el, kk;                                   /* eta_qflow.c:232-262 */
+        qflow_jhat(p, cur, jh, &Nn, &el, &kk);
+        Nn *= dV; ell = el * dV; Kk = kk * dV;
+        const double D = Nn * Kk - ell * ell, Dr = D + 1e-10 * Nn * Kk;
+        cond = D / (Nn * Kk);
+        omega = (Kk * Q - ell * Jt) / Dr;
+        Omega = (ell * Q - Nn * Jt) / Dr;
+        if (Omega > OMEGA_CAP) { Omega = OMEGA_CAP; omega = (Q + Omega * ell) / Nn; }
+        if (Omega < -OMEGA_CAP) { Omega = -OMEGA_CAP; omega = (Q + Omega * ell) / Nn; }
+        const double m2w = m * m - omega * omega, mt2w = mth * mth - omega * omega;
+        double maxF = 0;
+        for (long ii = 0; ii < N3; ii++) {                 /* eta_qflow.c:268-318 */
+            const int i = (int)(ii / NN), j = (int)((ii / N) % N), k = (int)(ii % N);
+            const long X[2][3] = {{IX3(WR(i + 1), j, k), IX3(i, WR(j + 1), k), IX3(i, j, WR(k + 1))},
+                                  {IX3(WR(i - 1), j, k), IX3(i, WR(j - 1), k), IX3(i, j, WR(k - 1))}};
+            const double r0 = u[ii] * u[ii] + v[ii] * v[ii];
+            const double r1 = u[N3 + ii] * u[N3 + ii] + v[N3 + ii] * v[N3 + ii];
+            const double r2 = u[2 * N3 + ii] * u[2 * N3 + ii] + v[2 * N3 + ii] * v[2 * N3 + ii];
+            const double s = r0 * r1 * r2, den = 1.0 + kappa * s;
+            const double Vtp = 0.5 * mu / (den * den);
+            const double Pa[3] = {r1 * r2, r0 * r2, r0 * r1};
+            const double x = -L + i * dx, y = -L + j * dx, wm = wmask[(long)i * N + j];
+            const double wO = 2.0 * omega * Omega * wm, O2 = Omega * Omega * wm;
+            for (int a = 0; a < 3; a++) {
+                const double *ua = u + a * N3, *va = v + a * N3, *ta = tu + a * N3, *sa = tv + a * N3;
+#define LAP(f) (((f)[X[0][0]] + (f)[X[1][0]] + (f)[X[0][1]] + (f)[X[1][1]] + (f)[X[0][2]] + (f)[X[1][2]] - 6.0 * (f)[ii]) * invdx2)
+#define DRV(f, ax, comp) ((f)[(comp) * N3 + X[0][ax]] - (f)[(comp) * N3 + X[1][ax]])
+                const double lu = LAP(ua), lv = LAP(va), ltu = LAP(ta), ltv = LAP(sa);
+                const int b = (a + 1) % 3, c = (a + 2) % 3;
+                const double curl_tu = (DRV(tu, b, c) - DRV(tu, c, b)) * inv2dx;
+                const double curl_tv = (DRV(tv, b, c) - DRV(tv, c, b)) * inv2dx;
+                const double curl_u = (DRV(u, b, c) - DRV(u, c, b)) * inv2dx;
+                const double curl_v = (DRV(v, b, c) - DRV(v, c, b)) * inv2dx;
+                double Fu = lu - m2w * ua[ii] - 2.0 * Vtp * ua[ii] * Pa[a] + eta * curl_tu;
+                double Fv = lv - m2w * va[ii] - 2.0 * Vtp * va[ii] * Pa[a] + eta * curl_tv;
+                double Ftu = ltu - mt2w * ta[ii] + eta * curl_u;
+                double Ftv = ltv - mt2w * sa[ii] + eta * curl_v;
+                double j2[4];                                   /* Jhat(Jhat w) of u, v, tu, tv */
+                for (int q = 0; q < 4; q++) {
+                    const double *w = jh + (long)(3 * q) * N3, *wa = w + a * N3;
+                    const double spin = a == 0 ? w[N3 + ii] : a == 1 ? -w[ii] : 0.0;
+                    j2[q] = x * (wa[X[0][1]] - wa[X[1][1]]) * inv2dx - y * (wa[X[0][0]] - wa[X[1][0]]) * inv2dx + spin;
+                }
+                Fu += -wO * jh[(3 + a) * N3 + ii] - O2 * j2[0];
+                Fv += +wO * jh[a * N3 + ii] - O2 * j2[1];
+                Ftu += -wO * jh[(9 + a) * N3 + ii] - O2 * j2[2];
+                Ftv += +wO * jh[(6 + a) * N3 + ii] - O2 * j2[3];
+                nxt[a * N3 + ii] = ua[ii] + dtau * Fu;
+                nxt[(3 + a) * N3 + ii] = va[ii] + dtau * Fv;
+                nxt[(6 + a) * N3 + ii] = ta[ii] + dtau * Ftu;
+                nxt[(9 + a) * N3 + ii] = sa[ii] + dtau * Ftv;
+                double af = fabs(Fu);
+                if (af > maxF) maxF = af;
+                af = fabs(Ftu);
+                if (af > maxF) maxF = af;
+            }
+        }
+        double *t = cur; cur = nxt; nxt = t;
+        if (it % printev == 0 || it == iters) {               /* eta_qflow.c:336-358 */
+            const double *U = cur, *V = cur + 3 * N3, *TU = cur + 6 * N3, *TV = cur + 9 * N3;
+            double Eg = 0, Em = 0, Ep = 0, Ex = 0, smax = 0;
+            for (long ii = 0; ii < N3; ii++) {
+                const int i = (int)(ii / NN), j = (int)((ii / N) % N), k = (int)(ii % N);
+                const long X[2][3] = {{IX3(WR(i + 1), j, k), IX3(i, WR(j + 1), k), IX3(i, j, WR(k + 1))},
+                                      {IX3(WR(i - 1), j, k), IX3(i, WR(j - 1), k), IX3(i, j, WR(k - 1))}};
+                const double r0 = U[ii] * U[ii] + V[ii] * V[ii];
+                const double r1 = U[N3 + ii] * U[N3 + ii] + V[N3 + ii] * V[N3 + ii];
+                const double r2 = U[2 * N3 + ii] * U[2 * N3 + ii] + V[2 * N3 + ii] * V[2 * N3 + ii];
+                const double s = r0 * r1 * r2;
+                if (s > smax) smax = s;
+                Ep += 0.5 * mu * s / (1.0 + kappa * s);
+                for (int a = 0; a < 3; a++) {
+                    const double *F4[4] = {U + a * N3, V + a * N3, TU + a * N3, TV + a * N3};
+                    for (int q = 0; q < 4; q++) {
+                        const double *f = F4[q];
+                        const double gx = (f[X[0][0]] - f[X[1][0]]) * inv2dx, gy = (f[X[0][1]] - f[X[1][1]]) * inv2dx;
+                        const double gz = (f[X[0][2]] - f[X[1][2]]) * inv2dx;
+                        Eg += 0.5 * (gx * gx + gy * gy + gz * gz);
+                    }
+                    Em += 0.5 * m * m * (U[a * N3 + ii] * U[a * N3 + ii] + V[a * N3 + ii] * V[a * N3 + ii])
+                        + 0.5 * mth * mth * (TU[a * N3 + ii] * TU[a * N3 + ii] + TV[a * N3 + ii] * TV[a * N3 + ii]);
+                    const int b = (a + 1) % 3, c = (a + 2) % 3;
+                    const double curl_u = (DRV(U, b, c) - DRV(U, c, b)) * inv2dx;
+                    const double curl_v = (DRV(V, b, c) - DRV(V, c, b)) * inv2dx;
+                    Ex += -eta * (TU[a * N3 + ii] * curl_u + TV[a * N3 + ii] * curl_v);
+                }
+            }
+            const double EQJ = (Eg + Em + Ep + Ex) * dV + 0.5 * (omega * Q - Omega * Jt);
+            const double Jact = omega * ell - Omega * Kk;
+            if (nrows < maxrows) {
+                double *r = rows + 9 * nrows;
+                r[0] = it; r[1] = omega; r[2] = Omega; r[3] = EQJ; r[4] = maxF; r[5] = smax; r[6] = Nn;
+                r[7] = Jact; r[8] = cond;
+            }
+            nrows++;
+        }
+#undef LAP
+#undef DRV
+    }
+#undef WR
+#undef IX3
+    double el, kk;                                            /* eta_qflow.c:377-399 */
+    qflow_jhat(p, cur, jh, NULL, &el, &kk);
+    ell = el * dV; Kk = kk * dV;
+    if (cur != st) memcpy(st, cur, sizeof(double) * 12 * (size_t)N3);
+    if (jhat_out) memcpy(jhat_out, jh, sizeof(double) * 12 * (size_t)N3);
+    free(nx); free(jh); free(wmask);
+    if (out) { out[0] = omega; out[1] = Omega; out[2] = cond; out[3] = ell; out[4] = Kk; }
+    return nrows;
+}

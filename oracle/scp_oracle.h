@@ -83,6 +83,18 @@ double or_qflow_norm(const OrQflow *p, const double *state);
  * multiplier of the last iteration (the one the output velocities use). */
 int    or_qflow(const OrQflow *p, double *state, double Q, double dtau, int iters, int printev, double *rows,
                 int maxrows, double *omega_last);
+/* The fixed-(Q, J) mode (eta_qflow.c fixJ = 1): Jhat w = x d_y w - y d_x w - zhat x w
+ * of the 12 fields (:210-262), the two multipliers (omega, Omega) from the
+ * regularised 2x2 solve with |Omega| capped at 0.15, the Coriolis and centrifugal
+ * forces masked at the light cylinder (:294-310). A row {it, omega, Omega, E_QJ,
+ * maxF, s_max, N, J_act, cond} for it % printev == 0 and it == iters (:352-358);
+ * returns the row count. out[0..5): omega, Omega, cond of the last iteration,
+ * ell and K of the final state (the refresh :377-399); jhat (12 N^3, or NULL):
+ * Jhat of the final state's fields. */
+int    or_qflow_j(const OrQflow *p, double *state, double Q, double Jt, double dtau, int iters, int printev,
+                  double *rows, int maxrows, double *out, double *jhat);
+/* The light-cylinder mask of the rotational terms at column (i, j) (eta_qflow.c:213-217) */
+double or_qflow_wmask(const OrQflow *p, int i, int j);
 
 #ifdef __cplusplus
 }

@@ -4,8 +4,10 @@
  * (usage eta_qflow.c:72-86, header lines :199-206, rows :370, final lines
  * :400-432, the f32 rotating seed :436-468), with the flow itself on the GPU
  * through the C-ABI (scp_qflow_*, include/scp_amd.h) and the seed built by
- * libscp_io (qflow_seed.h). The fixed-(Q, J) mode (fixJ = 1) is not built and
- * is refused; the ball and the ring/twisted seeds (wind, wtor, twist) are.
+ * libscp_io (qflow_seed.h): the ball and the ring/twisted seeds (wind, wtor,
+ * twist), in the fixed-Q mode and the fixed-(Q, J) mode (fixJ = 1, Jtarget;
+ * header :193-197, rows :353-357, final line :398-402, the seed's rotational
+ * velocity terms :447-460).
  */
 #include "qflow_seed.h"
 #include "scp_amd.h"
@@ -44,10 +46,7 @@ int main(int argc, char **argv) {
     const double wtor = (argc > 17) ? atof(argv[17]) : 2.0;
     const int twist = (argc > 18) ? atoi(argv[18]) : 0;
     const int fixJ = (argc > 19) ? atoi(argv[19]) : 0;
-    if (fixJ) {
-        fprintf(stderr, "ERROR: fixJ=1 (the fixed-(Q,J) two-multiplier mode) is not built in this relaxer\n");
-        return 1;
-    }
+    const double Jarg = (argc > 20) ? atof(argv[20]) : 0.0;
     if (N < 4 || printev < 1 || iters < 0) {
         fprintf(stderr, "ERROR: N >= 4, iters >= 0 and printev >= 1 required\n");
         return 1;
@@ -75,31 +74,63 @@ int main(int argc, char **argv) {
     if (scp_qflow_norm(f, &nphi, &nth)) die_abi("qflow norm");
     const double Nnorm = nphi + nth;
     const double Q = (Qarg > 0) ? Qarg : omega0 * Nnorm;
+    const double Jt = (fixJ && Jarg == 0.0 && wind > 0) ? (double)wind * Q : Jarg;
     printf("# eta_qflow N=%d L=%g dx=%.4f eta=%.3f m=%.3f mth=%.3f mu=%g kappa=%g\n", N, L, dx, eta, m, mth, mu,
            kappa);
     printf("# thinit=%s g_max=%.4e  N_init=%.4f  Q=%.4f  omega_init=Q/N=%.6f  dtau=%.5g iters=%d\n", thinit, gmax,
            Nnorm, Q, Q / Nnorm, dtau, iters);
-    printf("# %-8s %-10s %-12s %-12s %-12s %-12s\n", "iter", "omega", "E_Q", "maxF", "s_max", "N");
+    if (fixJ) {
+        printf("# fixJ: J_target=%.4f  (two multipliers w,Om; Jhat = d_phi - zhat x)\n", Jt);
+        printf("# %-8s %-10s %-10s %-12s %-12s %-12s %-12s %-10s %-10s\n", "iter", "omega", "Omega", "E_QJ", "maxF",
+               "s_max", "N", "J_act", "cond");
+        if (argc <= 13 || atof(argv[13]) <= 0) dtau = 0.02 * dx2;   /* after the header, as the reference */
+    } else
+        printf("# %-8s %-10s %-12s %-12s %-12s %-12s\n", "iter", "omega", "E_Q", "maxF", "s_max", "N");
     fflush(stdout);
 
     const int maxrows = iters / printev + 2;
-    scp_qflow_row *rows = calloc((size_t)maxrows, sizeof *rows);
     int nrows = 0;
-    double omega = 0;
-    if (scp_qflow_run(f, Q, dtau, iters, printev, rows, maxrows, &nrows, &omega)) die_abi("qflow run");
-    for (int r = 0; r < nrows && r < maxrows; r++)
-        printf("  %-8d %-10.6f %-12.6e %-12.4e %-12.4e %-12.4f\n", rows[r].iter, rows[r].omega, rows[r].E_Q,
-               rows[r].maxF, rows[r].s_max, rows[r].N);
-    free(rows);
+    double omega = 0, last[5] = {0, 0, 1.0, 0, 0};
+    if (fixJ) {
+        scp_qflow_row_j *rows = calloc((size_t)maxrows, sizeof *rows);
+        if (scp_qflow_run_fixj(f, Q, Jt, dtau, iters, printev, rows, maxrows, &nrows, last)) die_abi("qflow run");
+        for (int r = 0; r < nrows && r < maxrows; r++)
+            printf("  %-8d %-10.6f %-10.6f %-12.6e %-12.4e %-12.4e %-12.4f %-10.4f %-10.3e\n", rows[r].iter,
+                   rows[r].omega, rows[r].Omega, rows[r].E_QJ, rows[r].maxF, rows[r].s_max, rows[r].N,
+                   rows[r].J_act, rows[r].cond);
+        free(rows);
+        omega = last[0];
+    } else {
+        scp_qflow_row *rows = calloc((size_t)maxrows, sizeof *rows);
+        if (scp_qflow_run(f, Q, dtau, iters, printev, rows, maxrows, &nrows, &omega)) die_abi("qflow run");
+        for (int r = 0; r < nrows && r < maxrows; r++)
+            printf("  %-8d %-10.6f %-12.6e %-12.4e %-12.4e %-12.4f\n", rows[r].iter, rows[r].omega, rows[r].E_Q,
+                   rows[r].maxF, rows[r].s_max, rows[r].N);
+        free(rows);
+    }
+    const double Omega = last[1], cond = last[2], ell = last[3], Kk = last[4];
 
     double Nphi, Nth;
     if (scp_qflow_norm(f, &Nphi, &Nth)) die_abi("qflow norm");
     double *fin[12];
     for (int q = 0; q < 12; q++) fin[q] = st + (size_t)q * N3;
     if (scp_qflow_download(f, fin, 12)) die_abi("qflow download");
+    double *jh = NULL;                       /* Jhat of the final fields (fixJ) */
+    if (fixJ) {
+        jh = malloc(sizeof(double) * 12 * (size_t)N3);
+        if (!jh) { fprintf(stderr, "FATAL: malloc failed\n"); return 1; }
+        double *jc[12];
+        for (int q = 0; q < 12; q++) jc[q] = jh + (size_t)q * N3;
+        if (scp_qflow_download_jhat(f, jc, 12)) die_abi("qflow Jhat download");
+    }
     scp_qflow_destroy(f);
     printf("# final: omega=%.6f  Q_phi=%.4f  Q_theta=%.4f  Q_total=%.4f (target %.4f)\n", omega, omega * Nphi,
            omega * Nth, omega * (Nphi + Nth), Q);
+    if (fixJ) {
+        const double Nfin = Nphi + Nth;
+        printf("# fixJ final: Omega=%.6f  J_actual=%.4f (target %.4f)  Q_actual=%.4f  cond=%.3e  w_eff(Q/N)=%.6f\n",
+               Omega, omega * ell - Omega * Kk, Jt, omega * Nfin - Omega * ell, cond, Q / Nfin);
+    }
     if (omega >= m) printf("# WARNING: omega >= m (outside existence window)\n");
     if (omega * omega >= mth * mth) printf("# WARNING: omega >= m_theta (Theta unbound)\n");
     const double *u[3], *v[3], *tu[3], *tv[3];
@@ -155,20 +186,32 @@ int main(int argc, char **argv) {
                rho_pk, wsum / (2 * M_PI));
     }
 
-    /* the rotating seed, gen_qball_boost column order: Phidot = i omega phi */
+    /* the rotating seed, gen_qball_boost column order: Phidot = i omega phi, and
+     * with fixJ - Omega Jhat phi (masked at the light cylinder, eta_qflow.c:213-217) */
     float *fc = malloc(sizeof(float) * 24 * (size_t)N3);
     if (!fc) { fprintf(stderr, "FATAL: malloc failed\n"); return 1; }
-    for (long ii = 0; ii < N3; ii++)
+    for (long ii = 0; ii < N3; ii++) {
+        double wm = 0;
+        if (fixJ) {
+            const int i = (int)(ii / ((long)N * N)), j = (int)((ii / N) % N);
+            const double x = -L + i * dx, y = -L + j * dx, rho = sqrt(x * x + y * y);
+            wm = 1.0 / (1.0 + exp((rho - 11.0) / 0.5));
+        }
         for (int a = 0; a < 3; a++) {
+            const double Ju = fixJ ? wm * jh[(size_t)a * N3 + ii] : 0, Jvv = fixJ ? wm * jh[(size_t)(3 + a) * N3 + ii] : 0;
+            const double Jtu = fixJ ? wm * jh[(size_t)(6 + a) * N3 + ii] : 0;
+            const double Jtv = fixJ ? wm * jh[(size_t)(9 + a) * N3 + ii] : 0;
             fc[(size_t)(0 + a) * N3 + ii] = (float)u[a][ii];
             fc[(size_t)(12 + a) * N3 + ii] = (float)v[a][ii];
-            fc[(size_t)(6 + a) * N3 + ii] = (float)(-omega * v[a][ii]);
-            fc[(size_t)(18 + a) * N3 + ii] = (float)(omega * u[a][ii]);
+            fc[(size_t)(6 + a) * N3 + ii] = (float)(-omega * v[a][ii] - Omega * Ju);
+            fc[(size_t)(18 + a) * N3 + ii] = (float)(omega * u[a][ii] - Omega * Jvv);
             fc[(size_t)(3 + a) * N3 + ii] = (float)tu[a][ii];
             fc[(size_t)(15 + a) * N3 + ii] = (float)tv[a][ii];
-            fc[(size_t)(9 + a) * N3 + ii] = (float)(-omega * tv[a][ii]);
-            fc[(size_t)(21 + a) * N3 + ii] = (float)(omega * tu[a][ii]);
+            fc[(size_t)(9 + a) * N3 + ii] = (float)(-omega * tv[a][ii] - Omega * Jtu);
+            fc[(size_t)(21 + a) * N3 + ii] = (float)(omega * tu[a][ii] - Omega * Jtv);
         }
+    }
+    free(jh);
     static const char *nm[24] = {"phi_x", "phi_y", "phi_z", "theta_x", "theta_y", "theta_z",
                                  "phi_vx", "phi_vy", "phi_vz", "theta_vx", "theta_vy", "theta_vz",
                                  "phiim_x", "phiim_y", "phiim_z", "thetaim_x", "thetaim_y", "thetaim_z",

@@ -43,7 +43,8 @@ ABI_SYMBOLS = ["scp_abi_version", "scp_last_error", "scp_grid_bytes", "scp_grid_
                "scp_server_job", "scp_server_take", "scp_server_stats_get", "scp_server_now",
                "scp_grid_vec_fit", "scp_grid_set_kappa", "scp_grid_set_damp_rate",
                "scp_grid_step_diag_noalloc", "scp_qflow_create", "scp_qflow_destroy", "scp_qflow_upload",
-               "scp_qflow_download", "scp_qflow_norm", "scp_qflow_run", "scp_server_set_aging",
+               "scp_qflow_download", "scp_qflow_norm", "scp_qflow_run", "scp_qflow_run_fixj",
+               "scp_qflow_download_jhat", "scp_server_set_aging",
                "scp_batch_set_tile", "scp_batch_tile"]
 
 
@@ -113,6 +114,12 @@ class QflowParams(C.Structure):
 class QflowRow(C.Structure):
     _fields_ = [("iter", C.c_int32), ("reserved", C.c_int32), ("omega", C.c_double), ("E_Q", C.c_double),
                 ("maxF", C.c_double), ("s_max", C.c_double), ("N", C.c_double)]
+
+
+class QflowRowJ(C.Structure):
+    _fields_ = [("iter", C.c_int32), ("reserved", C.c_int32), ("omega", C.c_double), ("Omega", C.c_double),
+                ("E_QJ", C.c_double), ("maxF", C.c_double), ("s_max", C.c_double), ("N", C.c_double),
+                ("J_act", C.c_double), ("cond", C.c_double)]
 
 
 def make_params(N, mode, L, dt_factor, precision=PREC_F64, **kw):
@@ -206,6 +213,9 @@ def lib():
         L.scp_qflow_norm.argtypes = [VP, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.scp_qflow_run.argtypes = [VP, C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(QflowRow), C.c_int,
                                     IP, C.POINTER(C.c_double)]
+        L.scp_qflow_run_fixj.argtypes = [VP, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                         C.POINTER(QflowRowJ), C.c_int, IP, C.POINTER(C.c_double)]
+        L.scp_qflow_download_jhat.argtypes = [VP, DPP, C.c_int]
         _typed = True
     return L
 
@@ -487,7 +497,7 @@ class Server:
 
 class QFlow:
     """The fixed-charge gradient flow on the device (scp_qflow_* of include/scp_amd.h):
-    the relaxer sfa/seed/eta_qflow.c in its fixed-Q mode. State: (12, N^3) rows
+    the relaxer sfa/seed/eta_qflow.c in its fixed-Q and fixed-(Q, J) modes. State: (12, N^3) rows
     u0..2, v0..2, tu0..2, tv0..2."""
 
     def __init__(self, N, L, eta=0.5, m=1.5, mth=0.0, mu=-41.345, kappa=50.0):
@@ -535,3 +545,19 @@ class QFlow:
         _check(lib().scp_qflow_run(self._h, float(Q), float(dtau), int(iters), int(printev), rows, maxrows,
                                    C.byref(n), C.byref(om)), "scp_qflow_run")
         return [(r.iter, r.omega, r.E_Q, r.maxF, r.s_max, r.N) for r in rows[:n.value]], om.value
+
+    def run_fixj(self, Q, J, dtau, iters, printev):
+        """The fixed-(Q, J) mode: returns (rows [(it, omega, Omega, E_QJ, maxF, s_max, N,
+        J_act, cond)], (omega, Omega, cond) of the last iteration + (ell, K) of the final state)."""
+        maxrows = iters // max(1, printev) + 2
+        rows = (QflowRowJ * maxrows)()
+        n, last = C.c_int(), (C.c_double * 5)()
+        _check(lib().scp_qflow_run_fixj(self._h, float(Q), float(J), float(dtau), int(iters), int(printev), rows,
+                                        maxrows, C.byref(n), last), "scp_qflow_run_fixj")
+        return ([(r.iter, r.omega, r.Omega, r.E_QJ, r.maxF, r.s_max, r.N, r.J_act, r.cond) for r in rows[:n.value]],
+                tuple(last))
+
+    def download_jhat(self):
+        jh = np.empty((12, self.N ** 3))
+        _check(lib().scp_qflow_download_jhat(self._h, _col_ptrs(jh), 12), "scp_qflow_download_jhat")
+        return jh

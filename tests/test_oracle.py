@@ -629,3 +629,51 @@ def test_oracle_qflow_vs_reference_program(case):
     assert [list(r) for r in rows] == [qc.fmt_row(r) for r in orows]
     assert np.array_equal(ref_cols.astype(np.float32), qc.output_columns(fin, om_last))
     assert np.abs(fin - st).max() > 1e-6          # the flow moved the seed
+
+
+@pytest.mark.skipif(not os.path.exists(QFLOW_REF), reason="reference eta_qflow not built here")
+@pytest.mark.parametrize("case", ["ring_wind1_J", "ball_J", "ring_wind1_Jcap"])
+def test_oracle_qflow_fixed_q_j_vs_reference_program(case):
+    """The fixed-(Q, J) mode (fixJ = 1): the oracle's restatement against
+    sfa/seed/eta_qflow.c built from source, one thread: every printed row (omega,
+    Omega, E_QJ, maxF, s_max, N, J_act, cond) is the same text, the final lines
+    match, and the f32 output seed (velocities with the -Omega wm Jhat terms) is
+    bitwise the oracle's. Cases: a winding-1 ring at its default target J = Q, a
+    ball spun up to J = Q/20 (Omega ~ -0.1 from the start, v and Theta_v grow from
+    zero), and a ring pushed to J = 3Q, where |Omega| sits at its cap."""
+    from scp_amd import seeds
+    from tests import qflow_cases as qc
+    N, L, om0, iters, pev = 20, 5.0, 1.35, 60, 20
+    kw = dict(ring_wind1_J=dict(wind=1, wtor=2.0), ring_wind1_Jcap=dict(wind=1, wtor=2.0),
+              ball_J={})[case]
+    jarg = 0.0
+    with tempfile.TemporaryDirectory() as wd:
+        prof = os.path.join(wd, "prof.txt")
+        qc.write_profile(prof)
+        st, gmax = seeds.qflow_seed(N, L, prof, om0, qc.FLOW["eta"], qc.FLOW["mth"], **kw)
+        p = oracle.qflow_params(N, L, **qc.FLOW)
+        n0 = oracle.qflow_norm(p, st)
+        Q = om0 * n0
+        jarg = {"ring_wind1_Jcap": 3.0 * Q, "ball_J": 0.05 * Q}.get(case, 0.0)
+        out = os.path.join(wd, "seed.sfa")
+        text, rows = qc.run(qc.argv(QFLOW_REF, N, L, prof, om0, out, iters, pev, fixJ=1, Jtarget=jarg, **kw))
+        with sfa.SFAFile(out) as f:
+            ref_cols = f.read_frame(0)
+    Jt = (kw.get("wind", 0) * Q) if jarg == 0.0 else jarg
+    assert f"# fixJ: J_target={Jt:.4f}" in text
+    dx = 2 * L / (N - 1)
+    fin, orows, last, jh = oracle.qflow_j(p, st, Q, Jt, 0.02 * dx * dx, iters, pev)
+    assert [r[0] for r in orows] == [0, 20, 40, 60]
+    assert [list(r) for r in rows] == [qc.fmt_row_j(r) for r in orows]
+    om, Om, cond, ell, K = last
+    if case == "ring_wind1_Jcap":
+        assert abs(Om) == 0.15
+    if case == "ball_J":
+        assert 0.05 < -orows[0][2] < 0.15 and np.abs(fin[3:6]).max() > 1e-6   # Omega acts; v grows from 0
+    nphi, nth = [float(np.sum(fin[s] ** 2) * dx ** 3) for s in (slice(0, 6), slice(6, 12))]
+    fixj_final = [l for l in text.splitlines() if l.startswith("# fixJ final:")]
+    assert len(fixj_final) == 1 and f"Omega={Om:.6f}" in fixj_final[0] and f"cond={cond:.3e}" in fixj_final[0]
+    assert f"J_actual={om * ell - Om * K:.4f}" in fixj_final[0]
+    cols = qc.output_columns_j(fin, om, Om, jh, oracle.qflow_wmask(p))
+    assert np.array_equal(ref_cols.astype(np.float32), cols)
+    assert np.abs(fin - st).max() > 1e-6
