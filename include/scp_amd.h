@@ -216,17 +216,18 @@ int  scp_batch_step_all(scp_batch *b, int nsteps);
 /* scp_batch_step_all timed with HIP events around the graph replay. */
 int  scp_batch_time_steps(scp_batch *b, int nsteps, float *ms_total);
 /* Autotuner: tile shape (64 x 4 sites; for FP32C batches with int8 position
- * words also 128 x 4, 64 x 8 and 64 x 2) x planes per block (default, 16, 32,
+ * words also 128 x 4, 64 x 8, 64 x 2 and 32 x 8) x planes per block (default, 16, 32,
  * 64, 128) x launch form (one block per tile, or persistent resident blocks
  * over contiguous tile ranges), each timed on a 3-step chain of the batch's
  * own steps; every member advances *steps_taken steps (results do not depend
  * on the configuration). */
 int  scp_batch_tune(scp_batch *b, int *steps_taken);
-/* Tile of the batched launch: 64*kw (k) x tj (j) sites, (kw, tj) in (1, 4),
- * (2, 4), (1, 8), (1, 2); all but (1, 4) for FP32C batches with int8 position
- * words only. */
-int  scp_batch_set_tile(scp_batch *b, int kw, int tj);
-int  scp_batch_tile(const scp_batch *b, int *kw, int *tj);
+/* Tile of the batched launch: tk (k) x tj (j) sites, (tk, tj) in (64, 4),
+ * (128, 4), (64, 8), (64, 2), (32, 8); all but (64, 4) for FP32C batches with
+ * int8 position words only. The 32 x 8 tile runs rows of 32 sites, two per
+ * wave: grids whose edge is not a multiple of 64 leave fewer lanes idle. */
+int  scp_batch_set_tile(scp_batch *b, int tk, int tj);
+int  scp_batch_tile(const scp_batch *b, int *tk, int *tj);
 /* di = 0: default depth (the deepest of 128, 64, 32, 16 planes that gives
  * >= 4096 blocks over all members); persistent: 0 / 1. */
 int  scp_batch_set_config(scp_batch *b, int di, int persistent);

@@ -326,22 +326,22 @@ static bool shape_ok(const scp_batch *b, int shape) {
     return shape > 0 && shape < TILE_NSHAPES && b->f32 == 1 && b->lo_kind == LO_Q8P;
 }
 
-extern "C" int scp_batch_set_tile(scp_batch *b, int kw, int tj) {
+extern "C" int scp_batch_set_tile(scp_batch *b, int tk, int tj) {
     if (!b) return set_err(-1, "null batch");
     int shape = -1;
     for (int q = 0; q < TILE_NSHAPES; q++)
-        if (tile_kw(q) == kw && tile_tj(q) == tj) shape = q;
-    if (shape < 0) return set_err(-1, "tile %d x %d: the shapes are 64 x 4, 128 x 4, 64 x 8, 64 x 2", 64 * kw, tj);
+        if (tile_tk(q) == tk && tile_tj(q) == tj) shape = q;
+    if (shape < 0) return set_err(-1, "tile %d x %d: the shapes are 64 x 4, 128 x 4, 64 x 8, 64 x 2, 32 x 8", tk, tj);
     if (!b->members.empty() && !shape_ok(b, shape))
-        return set_err(-1, "tile %d x %d runs FP32C batches with int8 position words only", 64 * kw, tj);
+        return set_err(-1, "tile %d x %d runs FP32C batches with int8 position words only", tk, tj);
     b->shape = shape;
     b->stale = true;
     return 0;
 }
 
-extern "C" int scp_batch_tile(const scp_batch *b, int *kw, int *tj) {
+extern "C" int scp_batch_tile(const scp_batch *b, int *tk, int *tj) {
     if (!b) return set_err(-1, "null batch");
-    if (kw) *kw = tile_kw(b->shape);
+    if (tk) *tk = tile_tk(b->shape);
     if (tj) *tj = tile_tj(b->shape);
     return 0;
 }

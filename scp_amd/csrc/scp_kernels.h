@@ -186,18 +186,24 @@ int launch_march_batch_f64(int mode, const BatchGrid<Bufs<double>> *grids, const
 int launch_march_batch_f32c(int mode, int lo_kind, int shape, const BatchGrid<Bufs32> *grids,
                             const BatchTile *tiles, uint32_t ntiles, uint32_t nblocks, uint32_t per_block, int q,
                             hipStream_t s);
+// sites along k of a march tile with kw waves per row (kw = 0: 32-site rows)
+inline int march_tkm(int kw) { return kw == 0 ? 32 : 64 * kw; }
 // tiles per plane of the march at N (k, j), for building tile tables
-inline uint32_t march_tiles_k(int N, int kw = 1) { return (uint32_t)(N + 64 * kw - 1) / (uint32_t)(64 * kw); }
+inline uint32_t march_tiles_k(int N, int kw = 1) {
+    return (uint32_t)(N + march_tkm(kw) - 1) / (uint32_t)march_tkm(kw);
+}
 inline uint32_t march_tiles_j(int N, int tj = 4) { return (uint32_t)(N + tj - 1) / (uint32_t)tj; }
 // tile shapes of the batched march: 64 x 4 (every class), and for FP32C grids
-// with int8 position words 128 x 4, 64 x 8 and 64 x 2
-enum { TILE_64x4 = 0, TILE_128x4 = 1, TILE_64x8 = 2, TILE_64x2 = 3, TILE_NSHAPES = 4 };
-inline int tile_kw(int shape) { return shape == TILE_128x4 ? 2 : 1; }
-inline int tile_tj(int shape) { return shape == TILE_64x8 ? 8 : (shape == TILE_64x2 ? 2 : 4); }
+// with int8 position words 128 x 4, 64 x 8, 64 x 2 and 32 x 8 (rows of 32
+// sites, two per wave: edges that are not multiples of 64 waste fewer lanes)
+enum { TILE_64x4 = 0, TILE_128x4 = 1, TILE_64x8 = 2, TILE_64x2 = 3, TILE_32x8 = 4, TILE_NSHAPES = 5 };
+inline int tile_kw(int shape) { return shape == TILE_128x4 ? 2 : (shape == TILE_32x8 ? 0 : 1); }
+inline int tile_tk(int shape) { return march_tkm(tile_kw(shape)); }
+inline int tile_tj(int shape) { return (shape == TILE_64x8 || shape == TILE_32x8) ? 8 : (shape == TILE_64x2 ? 2 : 4); }
 // resident blocks per CU of a shape (its staged planes in the 160 KB of LDS)
 inline int tile_blocks_per_cu(int shape, bool f32) {
     if (!f32) return 1;
-    return shape == TILE_64x4 ? 2 : (shape == TILE_64x2 ? 3 : 1);
+    return (shape == TILE_64x4 || shape == TILE_32x8) ? 2 : (shape == TILE_64x2 ? 3 : 1);
 }
 // default planes per block of the march (64*KW x 4 tiles): deep enough to
 // amortise the 3-plane prologue (3/DI extra plane reads), shallow enough for

@@ -41,11 +41,12 @@ constexpr int MTJ = 4;                 // j rows per block (the single-grid laun
 constexpr int MNBUF = 3;
 
 // Tile geometry: KW waves along k per row (a row of 64*KW sites; wider rows
-// give the HBM longer contiguous runs per field), TJ rows (4 for single grids;
-// batches may pick 2 or 8).
+// give the HBM longer contiguous runs per field), or KW = 0: rows of 32 sites,
+// two rows per wave (grids whose edge is not a multiple of 64 waste fewer
+// lanes); TJ rows (4 for single grids; batches may pick 2 or 8).
 template <int KW, int TJ = MTJ>
 struct MG {
-    static constexpr int TKM = TK * KW;        // sites along k
+    static constexpr int TKM = KW == 0 ? TK / 2 : TK * KW;   // sites along k
     static constexpr int ROWS = TJ;            // rows of the tile
     static constexpr int MR = TJ + 2;          // staged rows
     static constexpr int MC = TKM + 2;         // staged columns
@@ -104,40 +105,48 @@ __device__ inline void sto_nt(V *base, uint32_t e, V v) {
 // The halo ring of one plane, spread over the block.
 //   rows (j0-1 and j0+H): 64-site segments (field, side, half), segment
 //     s = wave + NW*q, lanes along k — a wave-uniform field and a coalesced
-//     row segment per load;
-//   columns (k0-1 and k0+W, rows j0-1 .. j0+H): one (field, side, row) cell
-//     per thread for the first NF*2*MR threads.
+//     row segment per load; with 32-site rows (KW = 0) one load of a wave
+//     covers both rows of field s, lanes 0-31 the top row, 32-63 the bottom;
+//   columns (k0-1 and k0+W, rows j0-1 .. j0+H): (field, side, row) cells
+//     tid, tid + MNT, ... (CQ per thread: one except for the 32 x 8 tile).
 template <int NF, int KW, int TJ = MTJ>
 struct Halo {
     using G = MG<KW, TJ>;
     static constexpr int MR = G::MR;
-    static constexpr int NSEG = NF * 2 * KW;              // row segments per plane
-    static constexpr int RQ = (NSEG + G::NW - 1) / G::NW; // row segments per wave
-    static_assert(NF * 2 * MR <= G::MNT, "one halo column cell per thread");
+    static constexpr bool HALF = KW == 0;
+    static constexpr int NSEG = HALF ? NF : NF * 2 * KW;   // wave loads of halo rows per plane
+    static constexpr int RQ = (NSEG + G::NW - 1) / G::NW;  // of them per wave
+    static constexpr int NCC = NF * 2 * MR;                // halo column cells per plane
+    static constexpr int CQ = (NCC + G::MNT - 1) / G::MNT; // of them per thread
     int wave, lane, W;
     uint32_t rtop, rbot;      // in-plane element offsets (padded rows) of this lane's top/bottom halo cell (half 0)
     int lrow_top, lrow_bot;   // LDS offsets (field 0, half 0) of this lane's halo row cells
-    int cf;                   // column cell: field, or -1
-    uint32_t cg;              // column cell: in-plane element offset
-    int cl;                   // column cell: LDS offset
+    int cf[CQ];               // column cells: field, or -1
+    uint32_t cg[CQ];          // column cells: in-plane element offset
+    int cl[CQ];               // column cells: LDS offset
     __device__ void init(const MTile &t, uint32_t pitch) {
         const int N = t.N, tid = threadIdx.y * G::TKM + threadIdx.x;
         wave = __builtin_amdgcn_readfirstlane(tid / 64);
         lane = tid & 63;
         W = t.W;
-        rtop = (uint32_t)wrapc(t.j0 - 1, N) * pitch + t.k0 + lane;
-        rbot = (uint32_t)wrapc(t.j0 + t.H, N) * pitch + t.k0 + lane;
-        lrow_top = lane + 1;
-        lrow_bot = (t.H + 1) * G::MC + lane + 1;
-        cf = -1; cg = 0; cl = 0;
-        if (tid < NF * 2 * MR) {
-            const int f = tid / (2 * MR), rem = tid % (2 * MR), side = rem / MR, r = rem % MR;
-            if (r <= t.H + 1) {
-                cf = f;
-                const int gj = wrapc(t.j0 - 1 + r, N);
-                const int ck = side ? wrapc(t.k0 + t.W, N) : wrapc(t.k0 - 1, N);
-                cg = (uint32_t)gj * pitch + ck;
-                cl = f * G::MPL + r * G::MC + (side ? t.W + 1 : 0);
+        const int sl = HALF ? (lane & 31) : lane;           // lane within its row segment
+        rtop = (uint32_t)wrapc(t.j0 - 1, N) * pitch + t.k0 + sl;
+        rbot = (uint32_t)wrapc(t.j0 + t.H, N) * pitch + t.k0 + sl;
+        lrow_top = sl + 1;
+        lrow_bot = (t.H + 1) * G::MC + sl + 1;
+#pragma unroll
+        for (int c = 0; c < CQ; c++) {
+            const int cell = tid + c * G::MNT;
+            cf[c] = -1; cg[c] = 0; cl[c] = 0;
+            if (cell < NCC) {
+                const int f = cell / (2 * MR), rem = cell % (2 * MR), side = rem / MR, r = rem % MR;
+                if (r <= t.H + 1) {
+                    cf[c] = f;
+                    const int gj = wrapc(t.j0 - 1 + r, N);
+                    const int ck = side ? wrapc(t.k0 + t.W, N) : wrapc(t.k0 - 1, N);
+                    cg[c] = (uint32_t)gj * pitch + ck;
+                    cl[c] = f * G::MPL + r * G::MC + (side ? t.W + 1 : 0);
+                }
             }
         }
     }
@@ -145,12 +154,21 @@ struct Halo {
     __device__ bool seg(int q, int &f, uint32_t &g, int &l) const {
         const int sg = wave + G::NW * q;
         if (sg >= NSEG) return false;
-        f = sg / (2 * KW);
-        const int rem = sg % (2 * KW), side = rem / KW, half = rem % KW;
-        if (half * 64 + lane >= W) return false;
-        g = (side ? rbot : rtop) + half * 64;
-        l = (side ? lrow_bot : lrow_top) + half * 64;
-        return true;
+        if constexpr (HALF) {
+            const int side = lane >> 5;
+            if ((lane & 31) >= W) return false;
+            f = sg;
+            g = side ? rbot : rtop;
+            l = side ? lrow_bot : lrow_top;
+            return true;
+        } else {
+            f = sg / (2 * KW);
+            const int rem = sg % (2 * KW), side = rem / KW, half = rem % KW;
+            if (half * 64 + lane >= W) return false;
+            g = (side ? rbot : rtop) + half * 64;
+            l = (side ? lrow_bot : lrow_top) + half * 64;
+            return true;
+        }
     }
 };
 
@@ -376,7 +394,8 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
     constexpr int NF = march_nf<MODE>();
     constexpr int NM = MODE == 2 ? 12 : NF;   // fields of the i-1 plane (gauged: matter only)
     constexpr int MPL = MG<KW, TJ>::MPL;
-    constexpr int RQ = Halo<NF, KW, TJ>::RQ;
+    constexpr int RQ = Halo<NF, KW, TJ>::RQ, CQ = Halo<NF, KW, TJ>::CQ;
+    static_assert(!DIAG || (KW == 1 && TJ == MTJ), "the fused diagnostics run on 64 x 4 tiles");
 
     const int N = kp.N;
     const MTile t = mtile<KW, TJ>(N, DI, tk, tj, seg);
@@ -400,10 +419,11 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
             hr[q] = hs.seg(q, f, g, l) ? ld(f, pe + g) : T(0);
         }
     };
-    auto halo_col = [&](uint32_t pe) -> T {
-        return hs.cf >= 0 ? pin[(size_t)hs.cf * fs + pe + hs.cg] : T(0);
+    auto halo_col = [&](uint32_t pe, T *hc) {
+#pragma unroll
+        for (int c = 0; c < CQ; c++) hc[c] = hs.cf[c] >= 0 ? pin[(size_t)hs.cf[c] * fs + pe + hs.cg[c]] : T(0);
     };
-    auto publish = [&](T *buf, const T *own, const T *hr, T hc) {
+    auto publish = [&](T *buf, const T *own, const T *hr, const T *hc) {
         if (t.valid) {
 #pragma unroll
             for (int f = 0; f < NF; f++) buf[f * MPL + t.own] = own[f];
@@ -414,12 +434,14 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
             uint32_t g;
             if (hs.seg(q, f, g, l)) buf[f * MPL + l] = hr[q];
         }
-        if (hs.cf >= 0) buf[hs.cl] = hc;
+#pragma unroll
+        for (int c = 0; c < CQ; c++)
+            if (hs.cf[c] >= 0) buf[hs.cl[c]] = hc[c];
     };
 
     // prologue: planes i0-1 (own column), i0 (own + halo), i0+1 (own + halo)
     const int im0 = wm(t.i0, N), ip0 = wp(t.i0, N);
-    T cm[NM], cp[NF], hr[RQ], hc;
+    T cm[NM], cp[NF], hr[RQ], hc[CQ];
     T cq[S::REGQ ? NF : 1];     // register copy of plane i (register-queue policy)
     GaugeCarry<T> cy{};
     T th0m = T(0);              // DIAG, gauged: th_0 at plane i-1 (backward transport along i)
@@ -449,11 +471,11 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
 #pragma unroll
             for (int f = 0; f < NF; f++) { c0[f] = T(0); cp[f] = T(0); }
         }
-        T h0r[RQ];
+        T h0r[RQ], h0c[CQ];
         halo_rows((uint32_t)t.i0 * NN, h0r);
-        const T h0c = halo_col((uint32_t)t.i0 * NN);
+        halo_col((uint32_t)t.i0 * NN, h0c);
         halo_rows((uint32_t)ip0 * NN, hr);
-        hc = halo_col((uint32_t)ip0 * NN);
+        halo_col((uint32_t)ip0 * NN, hc);
         publish(lds, c0, h0r, h0c);
         if constexpr (S::REGQ) {
 #pragma unroll
@@ -504,8 +526,9 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
 #pragma unroll
             for (int f = 0; f < NF; f++) cpp[f] = T(0);
         }
+        T hcc[CQ];
         halo_rows(p2e, hrr);
-        const T hcc = halo_col(p2e);
+        halo_col(p2e, hcc);
         // plane i+1 into the buffer that held plane i-2
         publish(lds + nbuf * NF * MPL, cp, hr, hc);
         __syncthreads();
@@ -599,7 +622,8 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
         }
 #pragma unroll
         for (int q = 0; q < RQ; q++) hr[q] = hrr[q];
-        hc = hcc;
+#pragma unroll
+        for (int c = 0; c < CQ; c++) hc[c] = hcc[c];
         bb = nbuf;
         if constexpr (DIAG && MODE == 2) {
             // this plane's Gauss site waits for the row below's E_1 (LDS, visible after
@@ -689,10 +713,13 @@ inline int launch_march_diag_t(const KParams &kp, const typename S::B &b, int DI
 // (b in XCD-aware order, so an XCD's blocks take adjacent tiles); a grid takes
 // part in chain launches q < nq and emits its FP64 state on the last (if it
 // holds the emit buffers: grids of the request loop release them in between).
-// Tiles are 64*KW x TJ sites; blocks per CU: the default 64 x 4 tile fits
-// S::MINB (its staged planes), 64 x 2 three, the larger ones one.
+// Tiles are 64*KW x TJ sites (KW = 0: 32 x TJ); blocks per CU: the default
+// 64 x 4 tile and the 32 x 8 tile (61 KB of FP32 planes) fit S::MINB, 64 x 2
+// three, the larger ones one.
 template <class S, int KW, int TJ>
-constexpr int march_minb() { return (KW == 1 && TJ == MTJ) ? S::MINB : ((KW == 1 && TJ < MTJ) ? 3 : 1); }
+constexpr int march_minb() {
+    return ((KW == 1 && TJ == MTJ) || (KW == 0 && TJ == 8)) ? S::MINB : ((KW == 1 && TJ < MTJ) ? 3 : 1);
+}
 
 template <class S, int KW, int MODE, int TJ = MTJ>
 __global__ void __launch_bounds__((MG<KW, TJ>::MNT), (march_minb<S, KW, TJ>()))

@@ -73,6 +73,7 @@ int launch_march_batch_f32c(int mode, int lo_kind, int shape, const BatchGrid<Bu
         if (shape == TILE_128x4) return batch_lo<LO_Q8P, 2, 4>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
         if (shape == TILE_64x8) return batch_lo<LO_Q8P, 1, 8>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
         if (shape == TILE_64x2) return batch_lo<LO_Q8P, 1, 2>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
+        if (shape == TILE_32x8) return batch_lo<LO_Q8P, 0, 8>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
         return (int)hipErrorInvalidValue;
     }
     if (lo_kind == LO_F32) return batch_lo<LO_F32, 1, 4>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);

@@ -395,18 +395,18 @@ class Batch:
     def set_config(self, di=0, persistent=False):
         _check(lib().scp_batch_set_config(self._h, int(di), 1 if persistent else 0), "scp_batch_set_config")
 
-    def set_tile(self, kw, tj):
-        """Tile of the batched launch: 64*kw x tj sites (scp_batch_set_tile)."""
-        _check(lib().scp_batch_set_tile(self._h, int(kw), int(tj)), "scp_batch_set_tile")
+    def set_tile(self, tk, tj):
+        """Tile of the batched launch: tk (k) x tj (j) sites (scp_batch_set_tile)."""
+        _check(lib().scp_batch_set_tile(self._h, int(tk), int(tj)), "scp_batch_set_tile")
 
     @property
     def config(self):
         di, per, tuned = C.c_int(), C.c_int(), C.c_int()
         _check(lib().scp_batch_config(self._h, C.byref(di), C.byref(per), C.byref(tuned)), "scp_batch_config")
-        kw, tj = C.c_int(), C.c_int()
-        _check(lib().scp_batch_tile(self._h, C.byref(kw), C.byref(tj)), "scp_batch_tile")
+        tk, tj = C.c_int(), C.c_int()
+        _check(lib().scp_batch_tile(self._h, C.byref(tk), C.byref(tj)), "scp_batch_tile")
         return {"di": di.value, "persistent": bool(per.value), "tuned": bool(tuned.value),
-                "tile": (64 * kw.value, tj.value)}
+                "tile": (tk.value, tj.value)}
 
     def close(self):
         if self._h:

@@ -160,19 +160,20 @@ def test_fp64_batch_members_match_the_oracle(mode):
         g.close()
 
 
-@pytest.mark.parametrize("kw,tj,persistent", [(2, 4, False), (1, 8, False), (1, 2, False), (1, 2, True),
-                                              (1, 8, True)])
-def test_batch_tile_shapes_equal_each_grid_alone(kw, tj, persistent):
+@pytest.mark.parametrize("tk,tj,persistent", [(128, 4, False), (64, 8, False), (64, 2, False), (64, 2, True),
+                                              (64, 8, True), (32, 8, False), (32, 8, True)])
+def test_batch_tile_shapes_equal_each_grid_alone(tk, tj, persistent):
     """The other tile shapes of FP32C batches with int8 position words (128 x 4,
-    64 x 8, 64 x 2; ragged edges 33, 70) step every member bitwise as it steps
-    alone, also as persistent blocks and across chain replays."""
-    print(f"case tile {64 * kw}x{tj} persistent={persistent}", flush=True)
+    64 x 8, 64 x 2, and 32 x 8 with two 32-site rows per wave; ragged edges 33,
+    70) step every member bitwise as it steps alone, also as persistent blocks
+    and across chain replays."""
+    print(f"case tile {tk}x{tj} persistent={persistent}", flush=True)
     ref = [make(N, 600 + i, G.PREC_F32C, 4) for i, N in enumerate(SIZES)]
     got = [make(N, 600 + i, G.PREC_F32C, 4) for i, N in enumerate(SIZES)]
     with G.Batch(got) as b:
-        b.set_tile(kw, tj)
+        b.set_tile(tk, tj)
         b.set_config(di=32, persistent=persistent)
-        assert b.config["tile"] == (64 * kw, tj)
+        assert b.config["tile"] == (tk, tj)
         for rep in range(2):
             b.step([5, 3, 6, 5, 4])
             for g, n in zip(ref, [5, 3, 6, 5, 4]):
@@ -184,7 +185,7 @@ def test_batch_tile_shapes_equal_each_grid_alone(kw, tj, persistent):
 
 
 def test_batch_autotune_over_tile_shapes_keeps_results():
-    """FP32C int8-position-word batches tune over the four tile shapes as well."""
+    """FP32C int8-position-word batches tune over the five tile shapes as well."""
     ref = [make(N, 700 + i, G.PREC_F32C, 4) for i, N in enumerate(SIZES)]
     got = [make(N, 700 + i, G.PREC_F32C, 4) for i, N in enumerate(SIZES)]
     with G.Batch(got) as b:
@@ -192,7 +193,7 @@ def test_batch_autotune_over_tile_shapes_keeps_results():
         cfg = b.config
         b.step(3)
     print(f"tuned: {cfg}, steps taken {taken}")
-    assert cfg["tuned"] and cfg["tile"] in ((64, 4), (128, 4), (64, 8), (64, 2)) and taken == 4 * 2 * 5 * 4
+    assert cfg["tuned"] and cfg["tile"] in ((64, 4), (128, 4), (64, 8), (64, 2), (32, 8)) and taken == 5 * 2 * 5 * 4
     for g, h in zip(ref, got):
         g.step(taken + 3)
         assert np.array_equal(g.download(), h.download()), g.params.N
@@ -204,8 +205,10 @@ def test_batch_tile_shapes_are_refused_for_other_classes():
     g = make(16, 1, G.PREC_F64, 0)
     with G.Batch([g]) as b:
         with pytest.raises(G.ScpError, match="int8 position words"):
-            b.set_tile(2, 4)
+            b.set_tile(128, 4)
+        with pytest.raises(G.ScpError, match="int8 position words"):
+            b.set_tile(32, 8)
         with pytest.raises(G.ScpError, match="the shapes are"):
-            b.set_tile(1, 3)
-        b.set_tile(1, 4)
+            b.set_tile(64, 3)
+        b.set_tile(64, 4)
     g.close()
