@@ -225,11 +225,14 @@ int  scp_batch_tune(scp_batch *b, int *steps_taken);
 /* Tile of the batched launch: tk (k) x tj (j) sites, (tk, tj) in (64, 4),
  * (128, 4), (64, 8), (64, 2), (32, 8); all but (64, 4) for FP32C batches with
  * int8 position words only. The 32 x 8 tile runs rows of 32 sites, two per
- * wave: grids whose edge is not a multiple of 64 leave fewer lanes idle. */
+ * wave: grids whose edge is not a multiple of 64 leave fewer lanes idle. Until
+ * this call or the autotuner picks one, such a batch runs 32 x 8 whenever that
+ * issues at most 90 % of the lanes of 64 x 4 over its members (re-decided as
+ * they change; results do not depend on the shape), every other batch 64 x 4. */
 int  scp_batch_set_tile(scp_batch *b, int tk, int tj);
 int  scp_batch_tile(const scp_batch *b, int *tk, int *tj);
 /* di = 0: default depth (the deepest of 128, 64, 32, 16 planes that gives
- * >= 4096 blocks over all members); persistent: 0 / 1. */
+ * >= 16384 blocks over all members); persistent: 0 / 1. */
 int  scp_batch_set_config(scp_batch *b, int di, int persistent);
 int  scp_batch_config(const scp_batch *b, int *di, int *persistent, int *tuned);
 

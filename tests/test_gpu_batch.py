@@ -212,3 +212,31 @@ def test_batch_tile_shapes_are_refused_for_other_classes():
             b.set_tile(64, 3)
         b.set_tile(64, 4)
     g.close()
+
+
+def test_batch_default_shape_follows_members():
+    """An FP32C int8-position-word batch nobody configured runs 32 x 8 tiles when
+    they issue fewer lanes than 64 x 4 (edges 16..70 here), 64 x 4 when the edges
+    are multiples of 64, and re-decides as members leave; bf16-word batches stay
+    at 64 x 4. Results stay bitwise each grid's alone across the switch."""
+    got = [make(N, 800 + i, G.PREC_F32C, 4) for i, N in enumerate(SIZES)]
+    even = make(64, 900, G.PREC_F32C, 4)
+    ref = [make(N, 800 + i, G.PREC_F32C, 4) for i, N in enumerate(SIZES)]
+    ref_even = make(64, 900, G.PREC_F32C, 4)
+    with G.Batch(got + [even]) as b:
+        assert b.config["tile"] == (32, 8)
+        b.step(3)
+        for g in got:
+            b.remove(g)
+        assert b.config["tile"] == (64, 4)
+        b.step(2)
+    for g, h in zip(ref, got):
+        g.step(3)
+        assert np.array_equal(g.download(), h.download()), g.params.N
+    ref_even.step(5)
+    assert np.array_equal(ref_even.download(), even.download())
+    bf = make(33, 901, G.PREC_F32C, 1)
+    with G.Batch([bf]) as b:
+        assert b.config["tile"] == (64, 4)
+    for g in ref + got + [even, ref_even, bf]:
+        g.close()
