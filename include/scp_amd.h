@@ -223,12 +223,15 @@ int  scp_batch_time_steps(scp_batch *b, int nsteps, float *ms_total);
  * on the configuration). */
 int  scp_batch_tune(scp_batch *b, int *steps_taken);
 /* Tile of the batched launch: tk (k) x tj (j) sites, (tk, tj) in (64, 4),
- * (128, 4), (64, 8), (64, 2), (32, 8); all but (64, 4) for FP32C batches with
- * int8 position words only. The 32 x 8 tile runs rows of 32 sites, two per
- * wave: grids whose edge is not a multiple of 64 leave fewer lanes idle. Until
- * this call or the autotuner picks one, such a batch runs 32 x 8 whenever that
- * issues at most 90 % of the lanes of 64 x 4 over its members (re-decided as
- * they change; results do not depend on the shape), every other batch 64 x 4. */
+ * (128, 4), (64, 8), (64, 2), (32, 8), or (0, 0): each member its own of 64 x 4
+ * and 32 x 8 in one launch; all but (64, 4) for FP32C batches with int8
+ * position words only. The 32 x 8 tile runs rows of 32 sites, two per wave:
+ * grids whose edge is not a multiple of 64 leave fewer lanes idle. Until this
+ * call or the autotuner picks one, such a batch gives each member the shape it
+ * runs fastest (32 x 8 where it issues at most 90 % of the lanes of 64 x 4, or
+ * no more lanes at edges <= 128), re-decided as members change (results do not
+ * depend on the shape); every other batch runs 64 x 4. scp_batch_tile reports
+ * (0, 0) for the per-member launch. */
 int  scp_batch_set_tile(scp_batch *b, int tk, int tj);
 int  scp_batch_tile(const scp_batch *b, int *tk, int *tj);
 /* di = 0: default depth (the deepest of 128, 64, 32, 16 planes that gives

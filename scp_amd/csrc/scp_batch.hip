@@ -57,6 +57,11 @@ static size_t desc_size(const scp_batch *b) {
     return b->f32 ? sizeof(BatchGrid<Bufs32>) : sizeof(BatchGrid<Bufs<double>>);
 }
 
+// the tile shape of member g: the batch's, or under TILE_MIXED the grid's own
+static int member_shape(const scp_batch *b, const scp_grid *g) {
+    return b->shape == TILE_MIXED ? grid_tile_shape(g->p.N) : b->shape;
+}
+
 static int grid_di(const scp_batch *b, const scp_grid *g) {
     return std::max(1, std::min(b->DI > 0 ? b->DI : b->auto_di, g->p.N));
 }
@@ -67,38 +72,25 @@ static int grid_di(const scp_batch *b, const scp_grid *g) {
 // of the best depth, 128 planes 4-9 % slower; a lone 512^3 grid: 32 planes)
 static int batch_default_di(const scp_batch *b) {
     int di = 128;
-    const int kw = tile_kw(b->shape), tj = tile_tj(b->shape);
     for (; di > 16; di /= 2) {
         uint64_t tiles = 0;
         for (const scp_grid *g : b->members) {
-            const int N = g->p.N, d = std::min(di, N);
-            tiles += (uint64_t)march_tiles_k(N, kw) * march_tiles_j(N, tj) * (uint64_t)((N + d - 1) / d);
+            const int N = g->p.N, d = std::min(di, N), sh = member_shape(b, g);
+            tiles += (uint64_t)march_tiles_k(N, tile_kw(sh)) * march_tiles_j(N, tile_tj(sh)) *
+                     (uint64_t)((N + d - 1) / d);
         }
         if (tiles >= 16384) break;
     }
     return di;
 }
 
-// lanes a shape issues per step over the members (its tiles padded to whole
-// tiles along k and j)
-static uint64_t shape_lanes(const scp_batch *b, int shape) {
-    uint64_t lanes = 0;
-    for (const scp_grid *g : b->members) {
-        const uint64_t N = (uint64_t)g->p.N;
-        lanes += (uint64_t)march_tiles_k((int)N, tile_kw(shape)) * tile_tk(shape) *
-                 march_tiles_j((int)N, tile_tj(shape)) * tile_tj(shape) * N;
-    }
-    return lanes;
-}
-
-// the shape of a batch nobody configured: 32 x 8 where it issues at most 90 %
-// of the lanes of 64 x 4, else 64 x 4. Measured: in batches of equal grids 32 x 8
-// runs as fast or faster per site at every edge from 32 to 192 (+62 % at 32,
-// +11 % at 80, +5 % at 96) and 2 % slower at 512; on a sweep of 256 grids of
-// 32-96 (80 % of the lanes) +16 %, on 40 grids of 48-256 (94 %) 2 % slower.
+// the shape of a batch nobody configured: every member's own (grid_tile_shape):
+// 64 x 4 or 32 x 8 when they all agree, else the mixed launch
 static int auto_shape(const scp_batch *b) {
     if (!shape_ok(b, TILE_32x8)) return TILE_64x4;
-    return shape_lanes(b, TILE_32x8) * 10 <= shape_lanes(b, TILE_64x4) * 9 ? TILE_32x8 : TILE_64x4;
+    size_t n32 = 0;
+    for (const scp_grid *g : b->members) n32 += grid_tile_shape(g->p.N) == TILE_32x8;
+    return n32 == 0 ? TILE_64x4 : (n32 == b->members.size() ? TILE_32x8 : TILE_MIXED);
 }
 
 // rebuild descriptor and tile tables for the current members (device copies)
@@ -109,8 +101,8 @@ static int build_tables(scp_batch *b) {
     std::vector<BatchTile> tiles;
     for (size_t m = 0; m < n; m++) {
         const scp_grid *g = b->members[m];
-        const int N = g->p.N, di = grid_di(b, g);
-        const uint32_t nt = march_tiles_k(N, tile_kw(b->shape)) * march_tiles_j(N, tile_tj(b->shape)) *
+        const int N = g->p.N, di = grid_di(b, g), sh = member_shape(b, g);
+        const uint32_t nt = march_tiles_k(N, tile_kw(sh)) * march_tiles_j(N, tile_tj(sh)) *
                             (uint32_t)((N + di - 1) / di);
         for (uint32_t t = 0; t < nt; t++) tiles.push_back({(uint32_t)m, t});
     }
@@ -154,8 +146,10 @@ static void fill_desc(BatchGrid<B> *d, const scp_batch *b, scp_grid *g, int nq, 
     d->cur0 = cur;
     d->nq = nq;
     d->emit = (g->xo && g->vo) ? 1 : 0;
-    d->ntk = march_tiles_k(g->p.N, tile_kw(b->shape));
-    d->ntj = march_tiles_j(g->p.N, tile_tj(b->shape));
+    const int sh = member_shape(b, g);
+    d->ntk = march_tiles_k(g->p.N, tile_kw(sh));
+    d->ntj = march_tiles_j(g->p.N, tile_tj(sh));
+    d->tk = tile_tk(sh);
 }
 
 static int launch_one(scp_batch *b, int q) {
@@ -340,7 +334,7 @@ extern "C" int scp_batch_set_config(scp_batch *b, int di, int persistent) {
 
 extern "C" int scp_batch_config(const scp_batch *b, int *di, int *persistent, int *tuned) {
     if (!b) return set_err(-1, "null batch");
-    if (di) *di = b->DI > 0 ? b->DI : b->auto_di;
+    if (di) *di = b->DI > 0 ? b->DI : (b->stale ? batch_default_di(b) : b->auto_di);
     if (persistent) *persistent = b->persistent;
     if (tuned) *tuned = b->tuned ? 1 : 0;
     return 0;
@@ -356,8 +350,11 @@ extern "C" int scp_batch_set_tile(scp_batch *b, int tk, int tj) {
     if (!b) return set_err(-1, "null batch");
     int shape = -1;
     for (int q = 0; q < TILE_NSHAPES; q++)
-        if (tile_tk(q) == tk && tile_tj(q) == tj) shape = q;
-    if (shape < 0) return set_err(-1, "tile %d x %d: the shapes are 64 x 4, 128 x 4, 64 x 8, 64 x 2, 32 x 8", tk, tj);
+        if (q != TILE_MIXED && tile_tk(q) == tk && tile_tj(q) == tj) shape = q;
+    if (tk == 0 && tj == 0) shape = TILE_MIXED;
+    if (shape < 0)
+        return set_err(-1, "tile %d x %d: the shapes are 64 x 4, 128 x 4, 64 x 8, 64 x 2, 32 x 8 and 0 x 0 (per grid)",
+                       tk, tj);
     if (!b->members.empty() && !shape_ok(b, shape))
         return set_err(-1, "tile %d x %d runs FP32C batches with int8 position words only", tk, tj);
     b->shape = shape;
@@ -369,8 +366,8 @@ extern "C" int scp_batch_set_tile(scp_batch *b, int tk, int tj) {
 extern "C" int scp_batch_tile(const scp_batch *b, int *tk, int *tj) {
     if (!b) return set_err(-1, "null batch");
     const int shape = b->shape_auto ? auto_shape(b) : b->shape;
-    if (tk) *tk = tile_tk(shape);
-    if (tj) *tj = tile_tj(shape);
+    if (tk) *tk = shape == TILE_MIXED ? 0 : tile_tk(shape);
+    if (tj) *tj = shape == TILE_MIXED ? 0 : tile_tj(shape);
     return 0;
 }
 
@@ -387,8 +384,11 @@ extern "C" int scp_batch_tune(scp_batch *b, int *steps_taken) {
     float best = 1e30f;
     int bdi = 0, bper = 0, bshape = TILE_64x4, taken = 0;
     b->shape_auto = false;      // the candidates below set the shape
+    bool uniform = true;     // every member's own shape the same: the mixed launch adds nothing
+    for (const scp_grid *g : b->members)
+        uniform = uniform && grid_tile_shape(g->p.N) == grid_tile_shape(b->members[0]->p.N);
     for (int shape = 0; shape < TILE_NSHAPES; shape++) {
-        if (!shape_ok(b, shape)) continue;
+        if (!shape_ok(b, shape) || (shape == TILE_MIXED && uniform)) continue;
         for (int per = 0; per < 2; per++) {
             for (int di : DIS) {
                 b->shape = shape;

@@ -156,6 +156,8 @@ struct BatchGrid {
     int32_t nq;        // chain launches this grid takes part in; it emits on the last ...
     int32_t emit;      // ... when this is 1 (the grid holds its FP64 emit buffers)
     uint32_t ntk, ntj; // tiles per plane along k and j
+    int32_t tk;        // tile width of this grid's tiles (sites along k; the mixed-shape launch reads it)
+    int32_t pad_;
 };
 struct BatchTile {
     uint32_t grid, tile;
@@ -196,14 +198,30 @@ inline uint32_t march_tiles_j(int N, int tj = 4) { return (uint32_t)(N + tj - 1)
 // tile shapes of the batched march: 64 x 4 (every class), and for FP32C grids
 // with int8 position words 128 x 4, 64 x 8, 64 x 2 and 32 x 8 (rows of 32
 // sites, two per wave: edges that are not multiples of 64 waste fewer lanes)
-enum { TILE_64x4 = 0, TILE_128x4 = 1, TILE_64x8 = 2, TILE_64x2 = 3, TILE_32x8 = 4, TILE_NSHAPES = 5 };
+// TILE_MIXED: each grid its own of 64 x 4 and 32 x 8, in one launch (the grid
+// shape is then given by grid_tile_shape)
+enum { TILE_64x4 = 0, TILE_128x4 = 1, TILE_64x8 = 2, TILE_64x2 = 3, TILE_32x8 = 4, TILE_MIXED = 5, TILE_NSHAPES = 6 };
 inline int tile_kw(int shape) { return shape == TILE_128x4 ? 2 : (shape == TILE_32x8 ? 0 : 1); }
 inline int tile_tk(int shape) { return march_tkm(tile_kw(shape)); }
 inline int tile_tj(int shape) { return (shape == TILE_64x8 || shape == TILE_32x8) ? 8 : (shape == TILE_64x2 ? 2 : 4); }
 // resident blocks per CU of a shape (its staged planes in the 160 KB of LDS)
 inline int tile_blocks_per_cu(int shape, bool f32) {
     if (!f32) return 1;
-    return (shape == TILE_64x4 || shape == TILE_32x8) ? 2 : (shape == TILE_64x2 ? 3 : 1);
+    return (shape == TILE_64x4 || shape == TILE_32x8 || shape == TILE_MIXED) ? 2 : (shape == TILE_64x2 ? 3 : 1);
+}
+// lanes a tile shape issues per step on an N^3 grid (tiles padded to whole tiles)
+inline uint64_t tile_lanes(int shape, int N) {
+    return (uint64_t)march_tiles_k(N, tile_kw(shape)) * (uint64_t)tile_tk(shape) * march_tiles_j(N, tile_tj(shape)) *
+           (uint64_t)tile_tj(shape) * (uint64_t)N;
+}
+// the 256-thread shape an N^3 grid runs fastest, from batches of equal grids:
+// 32 x 8 where it issues at most 90 % of the lanes of 64 x 4 (+5 to +62 % per
+// site at edges 32, 80, 96, 144, 160, 200), or no more lanes at edges up to 128
+// (+1 to +2 % at 48, 64, 128); 64 x 4 otherwise (32 x 8 0-2 % slower at 176 to
+// 256 and at 512)
+inline int grid_tile_shape(int N) {
+    const uint64_t l32 = tile_lanes(TILE_32x8, N), l64 = tile_lanes(TILE_64x4, N);
+    return (l32 * 10 <= l64 * 9 || (l32 <= l64 && N <= 128)) ? TILE_32x8 : TILE_64x4;
 }
 // default planes per block of the march (64*KW x 4 tiles): deep enough to
 // amortise the 3-plane prologue (3/DI extra plane reads), shallow enough for

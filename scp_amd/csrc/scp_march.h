@@ -57,16 +57,29 @@ struct MG {
 
 struct MTile {
     int N, k0, j0, W, H, i0, i1, j, k, own;
+    int tid;          // thread of the tile (row-major: tid = row * TKM + column)
     bool valid;
 };
 
 __device__ inline int wrapc(int x, int N) { return x < 0 ? x + N : (x >= N ? x - N : x); }
 
-// tile (tk, tj) of segment seg of an N^3 grid; see MG for the tile size
-template <int KW, int TJ = MTJ>
+// tile (tk, tj) of segment seg of an N^3 grid; see MG for the tile size. BX:
+// the launch's block width; a block of BX x (MNT / BX) threads runs the tile
+// with its linear thread id (so one launch can run tiles of either 256-thread
+// shape, 64 x 4 or 32 x 8)
+template <int KW, int TJ = MTJ, int BX = MG<KW, TJ>::TKM>
 __device__ inline MTile mtile(int N, int DI, uint32_t tk, uint32_t tj, uint32_t seg) {
     using G = MG<KW, TJ>;
     MTile t;
+    int tx, ty;
+    if constexpr (BX == G::TKM) {
+        tx = threadIdx.x;
+        ty = threadIdx.y;
+    } else {
+        const int lin = threadIdx.y * BX + threadIdx.x;
+        tx = lin % G::TKM;
+        ty = lin / G::TKM;
+    }
     t.N = N;
     t.k0 = tk * G::TKM;
     t.j0 = tj * TJ;
@@ -74,10 +87,11 @@ __device__ inline MTile mtile(int N, int DI, uint32_t tk, uint32_t tj, uint32_t 
     t.H = min(TJ, N - t.j0);
     t.i0 = seg * DI;
     t.i1 = min(N, t.i0 + DI);
-    t.k = t.k0 + threadIdx.x;
-    t.j = t.j0 + threadIdx.y;
-    t.valid = (int)threadIdx.x < t.W && (int)threadIdx.y < t.H;
-    t.own = (threadIdx.y + 1) * G::MC + threadIdx.x + 1;
+    t.k = t.k0 + tx;
+    t.j = t.j0 + ty;
+    t.tid = ty * G::TKM + tx;
+    t.valid = tx < t.W && ty < t.H;
+    t.own = (ty + 1) * G::MC + tx + 1;
     return t;
 }
 
@@ -125,7 +139,7 @@ struct Halo {
     uint32_t cg[CQ];          // column cells: in-plane element offset
     int cl[CQ];               // column cells: LDS offset
     __device__ void init(const MTile &t, uint32_t pitch) {
-        const int N = t.N, tid = threadIdx.y * G::TKM + threadIdx.x;
+        const int N = t.N, tid = t.tid;
         wave = __builtin_amdgcn_readfirstlane(tid / 64);
         lane = tid & 63;
         W = t.W;
@@ -386,7 +400,7 @@ __device__ inline void gauss_pending(const KParams &kp, const DiagArgs *da, Diag
 // next tile into the same area. DIAG (with EMIT): the emitted state's per-site
 // diagnostics are accumulated into ds, and the gauged charge density is
 // written to da->rho.
-template <class S, int KW, int MODE, bool EMIT, bool DIAG = false, int TJ = MTJ>
+template <class S, int KW, int MODE, bool EMIT, bool DIAG = false, int TJ = MTJ, int BX = MG<KW, TJ>::TKM>
 __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::B &b, int DI, uint32_t tk,
                                            uint32_t tj, uint32_t seg, typename S::T *lds,
                                            const DiagArgs *da = nullptr, DiagState *ds = nullptr) {
@@ -395,10 +409,10 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
     constexpr int NM = MODE == 2 ? 12 : NF;   // fields of the i-1 plane (gauged: matter only)
     constexpr int MPL = MG<KW, TJ>::MPL;
     constexpr int RQ = Halo<NF, KW, TJ>::RQ, CQ = Halo<NF, KW, TJ>::CQ;
-    static_assert(!DIAG || (KW == 1 && TJ == MTJ), "the fused diagnostics run on 64 x 4 tiles");
+    static_assert(!DIAG || (KW == 1 && TJ == MTJ && BX == 64), "the fused diagnostics run on 64 x 4 blocks");
 
     const int N = kp.N;
-    const MTile t = mtile<KW, TJ>(N, DI, tk, tj, seg);
+    const MTile t = mtile<KW, TJ, BX>(N, DI, tk, tj, seg);
     if (t.i0 >= N) return;                    // whole block: uniform
     Halo<NF, KW, TJ> hs;
     hs.init(t, kp.pitch);
@@ -740,6 +754,43 @@ k_march_batch(const BatchGrid<typename S::B> *__restrict__ grids, const BatchTil
         else
             march_tile<S, KW, MODE, false, false, TJ>(g.kp, g.buf[par], g.DI, tk, tj, seg, lds);
     }
+}
+
+// Many grids, each with its own 256-thread tile shape (BatchGrid::tk: 64 for
+// 64 x 4, 32 for 32 x 8), in blocks of 64 x 4 threads that run either shape's
+// march by their linear thread id; staging sized for the larger (64 x 4).
+template <class S, int MODE>
+__global__ void __launch_bounds__(256, S::MINB)
+k_march_batch_mix(const BatchGrid<typename S::B> *__restrict__ grids, const BatchTile *__restrict__ tiles,
+                  uint32_t ntiles, uint32_t per_block, int q) {
+    static_assert(march_lds_elems<S, 0, MODE, 8>() <= march_lds_elems<S, 1, MODE, 4>(), "staging of 32 x 8 fits");
+    __shared__ typename S::T lds[march_lds_elems<S, 1, MODE, 4>()];
+    const uint32_t b = logical_block(blockIdx.x, gridDim.x);
+    const uint32_t t1 = min(ntiles, (b + 1) * per_block);
+    for (uint32_t t = b * per_block; t < t1; t++) {
+        const BatchTile bt = tiles[t];
+        const BatchGrid<typename S::B> &g = grids[bt.grid];
+        if (q >= g.nq) continue;
+        const uint32_t tk = bt.tile % g.ntk, r = bt.tile / g.ntk;
+        const uint32_t tj = r % g.ntj, seg = r / g.ntj;
+        const int par = (g.cur0 + q) & 1;
+        const bool emit = q == g.nq - 1 && g.emit;
+        if (g.tk == 32) {
+            if (emit) march_tile<S, 0, MODE, true, false, 8, 64>(g.kp, g.buf[par], g.DI, tk, tj, seg, lds);
+            else march_tile<S, 0, MODE, false, false, 8, 64>(g.kp, g.buf[par], g.DI, tk, tj, seg, lds);
+        } else {
+            if (emit) march_tile<S, 1, MODE, true, false, 4, 64>(g.kp, g.buf[par], g.DI, tk, tj, seg, lds);
+            else march_tile<S, 1, MODE, false, false, 4, 64>(g.kp, g.buf[par], g.DI, tk, tj, seg, lds);
+        }
+    }
+}
+
+template <class S, int MODE>
+inline int launch_march_batch_mix_t(const BatchGrid<typename S::B> *grids, const BatchTile *tiles, uint32_t ntiles,
+                                    uint32_t nblocks, uint32_t per_block, int q, hipStream_t s) {
+    hipLaunchKernelGGL((k_march_batch_mix<S, MODE>), dim3(nblocks), dim3(64, 4), 0, s, grids, tiles, ntiles,
+                       per_block, q);
+    return (int)hipGetLastError();
 }
 
 template <class S, int MODE, int KW = 1, int TJ = MTJ>

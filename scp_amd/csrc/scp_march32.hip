@@ -62,6 +62,14 @@ static int batch_lo(int mode, const BatchGrid<Bufs32> *grids, const BatchTile *t
                        : launch_march_batch_t<S32<LOK>, 0, KW, TJ>(grids, tiles, ntiles, nblocks, per_block, q, s);
 }
 
+template <int LOK>
+static int batch_mix(int mode, const BatchGrid<Bufs32> *grids, const BatchTile *tiles, uint32_t ntiles,
+                     uint32_t nblocks, uint32_t per_block, int q, hipStream_t s) {
+    return mode == 2   ? launch_march_batch_mix_t<S32<LOK>, 2>(grids, tiles, ntiles, nblocks, per_block, q, s)
+           : mode == 1 ? launch_march_batch_mix_t<S32<LOK>, 1>(grids, tiles, ntiles, nblocks, per_block, q, s)
+                       : launch_march_batch_mix_t<S32<LOK>, 0>(grids, tiles, ntiles, nblocks, per_block, q, s);
+}
+
 // shape (TILE_*): 64 x 4 for every storage kind; the other shapes for the
 // production words (int8 on the positions) only
 int launch_march_batch_f32c(int mode, int lo_kind, int shape, const BatchGrid<Bufs32> *grids,
@@ -74,6 +82,7 @@ int launch_march_batch_f32c(int mode, int lo_kind, int shape, const BatchGrid<Bu
         if (shape == TILE_64x8) return batch_lo<LO_Q8P, 1, 8>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
         if (shape == TILE_64x2) return batch_lo<LO_Q8P, 1, 2>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
         if (shape == TILE_32x8) return batch_lo<LO_Q8P, 0, 8>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
+        if (shape == TILE_MIXED) return batch_mix<LO_Q8P>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);
         return (int)hipErrorInvalidValue;
     }
     if (lo_kind == LO_F32) return batch_lo<LO_F32, 1, 4>(mode, grids, tiles, ntiles, nblocks, per_block, q, s);

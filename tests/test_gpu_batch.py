@@ -161,12 +161,14 @@ def test_fp64_batch_members_match_the_oracle(mode):
 
 
 @pytest.mark.parametrize("tk,tj,persistent", [(128, 4, False), (64, 8, False), (64, 2, False), (64, 2, True),
-                                              (64, 8, True), (32, 8, False), (32, 8, True)])
+                                              (64, 8, True), (32, 8, False), (32, 8, True), (0, 0, False),
+                                              (0, 0, True)])
 def test_batch_tile_shapes_equal_each_grid_alone(tk, tj, persistent):
     """The other tile shapes of FP32C batches with int8 position words (128 x 4,
-    64 x 8, 64 x 2, and 32 x 8 with two 32-site rows per wave; ragged edges 33,
-    70) step every member bitwise as it steps alone, also as persistent blocks
-    and across chain replays."""
+    64 x 8, 64 x 2, and 32 x 8 with two 32-site rows per wave, and 0 x 0: each
+    grid its own of 64 x 4 and 32 x 8 in one launch; ragged edges 33, 70) step
+    every member bitwise as it steps alone, also as persistent blocks and across
+    chain replays."""
     print(f"case tile {tk}x{tj} persistent={persistent}", flush=True)
     ref = [make(N, 600 + i, G.PREC_F32C, 4) for i, N in enumerate(SIZES)]
     got = [make(N, 600 + i, G.PREC_F32C, 4) for i, N in enumerate(SIZES)]
@@ -185,7 +187,8 @@ def test_batch_tile_shapes_equal_each_grid_alone(tk, tj, persistent):
 
 
 def test_batch_autotune_over_tile_shapes_keeps_results():
-    """FP32C int8-position-word batches tune over the five tile shapes as well."""
+    """FP32C int8-position-word batches tune over the tile shapes as well (the five
+    uniform ones and, members differing in their own shape here, the mixed launch)."""
     ref = [make(N, 700 + i, G.PREC_F32C, 4) for i, N in enumerate(SIZES)]
     got = [make(N, 700 + i, G.PREC_F32C, 4) for i, N in enumerate(SIZES)]
     with G.Batch(got) as b:
@@ -193,7 +196,8 @@ def test_batch_autotune_over_tile_shapes_keeps_results():
         cfg = b.config
         b.step(3)
     print(f"tuned: {cfg}, steps taken {taken}")
-    assert cfg["tuned"] and cfg["tile"] in ((64, 4), (128, 4), (64, 8), (64, 2), (32, 8)) and taken == 5 * 2 * 5 * 4
+    assert cfg["tuned"] and cfg["tile"] in ((64, 4), (128, 4), (64, 8), (64, 2), (32, 8), (0, 0))
+    assert taken == 6 * 2 * 5 * 4
     for g, h in zip(ref, got):
         g.step(taken + 3)
         assert np.array_equal(g.download(), h.download()), g.params.N
@@ -215,28 +219,28 @@ def test_batch_tile_shapes_are_refused_for_other_classes():
 
 
 def test_batch_default_shape_follows_members():
-    """An FP32C int8-position-word batch nobody configured runs 32 x 8 tiles when
-    they issue fewer lanes than 64 x 4 (edges 16..70 here), 64 x 4 when the edges
-    are multiples of 64, and re-decides as members leave; bf16-word batches stay
-    at 64 x 4. Results stay bitwise each grid's alone across the switch."""
+    """An FP32C int8-position-word batch nobody configured gives every member the
+    256-thread shape it runs fastest (32 x 8 for edges 16, 24, 40, 70, 64 x 4 for
+    33): one mixed launch while both are present, the uniform launch once the
+    members agree, re-decided as they leave; bf16-word batches stay at 64 x 4.
+    Results stay bitwise each grid's alone across the switches."""
     got = [make(N, 800 + i, G.PREC_F32C, 4) for i, N in enumerate(SIZES)]
-    even = make(64, 900, G.PREC_F32C, 4)
     ref = [make(N, 800 + i, G.PREC_F32C, 4) for i, N in enumerate(SIZES)]
-    ref_even = make(64, 900, G.PREC_F32C, 4)
-    with G.Batch(got + [even]) as b:
-        assert b.config["tile"] == (32, 8)
+    odd = SIZES.index(33)
+    with G.Batch(got) as b:
+        assert b.config["tile"] == (0, 0)          # per grid
         b.step(3)
-        for g in got:
-            b.remove(g)
+        b.remove(got[odd])
+        assert b.config["tile"] == (32, 8)
+        b.step(2)
+    with G.Batch([got[odd]]) as b:
         assert b.config["tile"] == (64, 4)
         b.step(2)
-    for g, h in zip(ref, got):
-        g.step(3)
+    for i, (g, h) in enumerate(zip(ref, got)):
+        g.step(3 + 2)
         assert np.array_equal(g.download(), h.download()), g.params.N
-    ref_even.step(5)
-    assert np.array_equal(ref_even.download(), even.download())
-    bf = make(33, 901, G.PREC_F32C, 1)
+    bf = make(40, 901, G.PREC_F32C, 1)
     with G.Batch([bf]) as b:
         assert b.config["tile"] == (64, 4)
-    for g in ref + got + [even, ref_even, bf]:
+    for g in ref + got + [bf]:
         g.close()
