@@ -107,6 +107,32 @@ def test_real_batch_equals_each_grid_alone(prec, lo):
         g.close()
 
 
+@pytest.mark.parametrize("tile", [None, (32, 8), (0, 0), (64, 4)])
+def test_complex_f32c_batch_shapes_equal_each_grid_alone(tile):
+    """Complex-mode FP32C grids (int8 position words) batched under every 256-thread
+    shape (None: the members' own, here mixed) step bitwise as they step alone."""
+    def mk(i, N):
+        L, st = ball(N, 450 + i)
+        g = G.Grid(G.make_params(N, G.MODE_COMPLEX, L, 0.2, precision=G.PREC_F32C, lo_words=4))
+        g.upload(st[:24].copy())
+        return g
+    print(f"case complex f32c tile={tile}", flush=True)
+    alone = [mk(i, N) for i, N in enumerate(SIZES)]
+    batched = [mk(i, N) for i, N in enumerate(SIZES)]
+    with G.Batch(batched) as b:
+        if tile is not None:
+            b.set_tile(*tile)
+        assert b.config["tile"] == (tile if tile is not None else (0, 0))
+        for rep in range(2):
+            for g in alone:
+                g.step(4)
+            b.step(4)
+            for g, h in zip(alone, batched):
+                assert np.array_equal(g.download(), h.download()), (rep, g.params.N)
+    for g in alone + batched:
+        g.close()
+
+
 def test_batch_membership_rules():
     a = make(16, 1, G.PREC_F32C, 1)
     b64 = make(16, 2, G.PREC_F64, 0)
