@@ -419,7 +419,7 @@ def serve_bench(a, emit=True):
         specs.append(dict(N=N, L=L, balls=balls, steps=int(rng.integers(a.steps_lo, a.steps_hi + 1))))
     offered = sum(s_["N"] ** 3 * s_["steps"] for s_ in specs)
     log(f"{len(specs)} jobs over {a.duration:.0f}s, offered {offered / 1e9:.1f} G site-updates")
-    lat, done_sites = [], 0
+    lat, wait, run, done_sites = [], [], [], 0
     with G.Server(int(a.budget_gb * 1e9), quantum=a.quantum, max_active=a.max_active) as srv:
         pending, i, next_log = {}, 0, 10.0
         while True:
@@ -450,10 +450,15 @@ def serve_bench(a, emit=True):
                     k = pending.pop(jid)
                     srv.take(jid, state=False)
                     lat.append(info["t_done"] - arrivals[k])
+                    wait.append(info["t_admit"] - arrivals[k])
+                    run.append(info["t_done"] - info["t_admit"])
                     done_sites += specs[k]["N"] ** 3 * specs[k]["steps"]
         st = srv.stats()
     elapsed = st["now"]
     completed = len(lat)
+    edges = [0, 1, 2, 5, 10, 20, 50, 100, 200, 500, 1000, 2000]
+    hist = np.histogram(np.clip(lat, 0, edges[-1]), bins=edges)[0].tolist() if lat else [0] * (len(edges) - 1)
+    pct = lambda v, q: float(np.nanpercentile(np.array(v), q)) if v else float("nan")
     lat = np.array(lat) if lat else np.array([float("nan")])
     out = {"metric": "serving: p50/p99 job completion latency and aggregate Gsite-updates/s",
            "value": st["site_updates"] / elapsed / 1e9, "unit": "Gsite-updates/s", "n_gpus": 1,
@@ -467,6 +472,10 @@ def serve_bench(a, emit=True):
            "jobs": {"submitted": i, "completed": completed, "unfinished": i - completed},
            "latency_s": {"p50": float(np.nanpercentile(lat, 50)), "p99": float(np.nanpercentile(lat, 99)),
                          "max": float(np.nanmax(lat))},
+           # completed jobs: arrival -> admission (queueing) and admission -> completion (on the GPU)
+           "wait_s": {"p50": pct(wait, 50), "p99": pct(wait, 99)},
+           "run_s": {"p50": pct(run, 50), "p99": pct(run, 99)},
+           "latency_histogram": {"edges_s": edges, "counts": hist},
            "offered_gsite_updates_per_s": offered / a.duration / 1e9,
            "busy_fraction": st["busy_seconds"] / elapsed, "elapsed_s": elapsed}
     if emit:
@@ -555,6 +564,7 @@ def main():
         sv = serve_bench(sa, emit=False)
         serving = {"p50_latency_s": sv["latency_s"]["p50"], "p99_latency_s": sv["latency_s"]["p99"],
                    "aggregate_gsite_updates_per_s": sv["value"], "jobs": sv["jobs"],
+                   "wait_p50_s": sv["wait_s"]["p50"], "run_p50_s": sv["run_s"]["p50"],
                    "window": "Poisson 4 jobs/s for 15 s + 45 s horizon, at most 16 active, HBM budget 250 GB; "
                              "gauged FP32C jobs, edge log-uniform [48,256], steps U[1000,20000], 1-4 Q-balls"}
     cpu = None if a.no_cpu_baseline else cpu_baseline(a.mode)
