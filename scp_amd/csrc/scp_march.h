@@ -214,6 +214,7 @@ struct LdsNB {
 
 struct S64 {
     static constexpr int MINB = 1;             // the staged planes fill the LDS: one block per CU
+    static constexpr uint32_t MARCH_BLOCKS = 4096;   // single-grid launch: blocks the default depth aims at
     // own-site words loaded with their plane: one plane ahead (30 more VGPRs) measured
     // 1-6 % slower at 512^3 with one 4-wave block per CU
     static constexpr bool WAHEAD = false;
@@ -285,6 +286,9 @@ template <int LOK> struct LoIO {
 template <int LOK>
 struct S32 {
     static constexpr int MINB = 2;             // two blocks per CU: registers capped at 256
+    // single-grid launch: blocks the default depth aims at (512^3: 32 planes per
+    // block, 5 % faster than 128; shorter marches keep adjacent tiles closer in step)
+    static constexpr uint32_t MARCH_BLOCKS = 16384;
     static constexpr bool WAHEAD = true;       // own-site words loaded one plane ahead (+3 % at 512^3)
     static constexpr bool REGQ = false;        // own column of planes i, i+1 read back from LDS
     static constexpr bool SPLIT_DAMP = true;
@@ -804,7 +808,7 @@ inline int launch_march_batch_t(const BatchGrid<typename S::B> *grids, const Bat
 template <class S, int KW, int MODE>
 inline int launch_march_t(bool emit, const KParams &kp, const typename S::B &b, int DI, hipStream_t s) {
     using G = MG<KW>;
-    if (DI <= 0) DI = march_di(kp.N, KW);
+    if (DI <= 0) DI = march_di(kp.N, KW, S::MARCH_BLOCKS);
     const uint32_t ntk = (kp.N + G::TKM - 1) / G::TKM, ntj = (kp.N + MTJ - 1) / MTJ, nseg = (kp.N + DI - 1) / DI;
     const dim3 grid(ntk * ntj * nseg), block(G::TKM, MTJ);
     if (emit) hipLaunchKernelGGL((k_march<S, KW, MODE, true>), grid, block, 0, s, kp, b, DI);

@@ -1,0 +1,172 @@
+// hbm_probe.hip — copy ceilings of the HBM for the access patterns of the
+// gauged FP32C step (a measurement aid; not part of the library).
+//
+// Prints, in GB/s and as a fraction of the 8 TB/s peak:
+//   * a plain copy (16 bytes per lane, nontemporal), read-only and write-only
+//     streams: what the memory delivers to a well-formed stream;
+//   * 34 + 34 streams of 512^3 FP32 words read and written site-linearly, one
+//     word per lane and stream (the step's 272 B/site: 15 position and 15
+//     velocity fields and 4 packed compensation-word fields each way), with the
+//     fields at a power-of-two stride (2^27 words) and staggered by 4 KiB;
+//   * the same streams in the step's march pattern: 64 x 4 site tiles marching
+//     along i over DI planes (the step's XCD-aware tile numbering), without and
+//     with the position halo ring the step stages (two rows per tile plane of
+//     every position field, and 180 column cells).
+// Usage: hbm_probe [reps]   (about 40 GB of device memory, 10-20 s)
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+
+#define HCHK(x)                                                                                  \
+    do {                                                                                         \
+        hipError_t e_ = (x);                                                                     \
+        if (e_ != hipSuccess) {                                                                  \
+            fprintf(stderr, "hbm_probe: %s at line %d\n", hipGetErrorString(e_), __LINE__);      \
+            exit(1);                                                                             \
+        }                                                                                        \
+    } while (0)
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__global__ void k_copy4(const f4 *__restrict__ s, f4 *__restrict__ d, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) __builtin_nontemporal_store(__builtin_nontemporal_load(&s[i]), &d[i]);
+}
+__global__ void k_read4(const f4 *__restrict__ s, float *__restrict__ sink, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const f4 v = __builtin_nontemporal_load(&s[i]);
+        if (v.x + v.y + v.z + v.w == 1234.5f) sink[0] = v.x;   // never true: keeps the load
+    }
+}
+__global__ void k_write4(f4 *__restrict__ d, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) __builtin_nontemporal_store(f4{1.f, 2.f, 3.f, 4.f}, &d[i]);
+}
+
+constexpr int NFP = 15, NQ = 4, NS = 2 * NFP + NQ;   // streams each way
+struct Streams {
+    const float *in[NS];
+    float *out[NS];
+};
+
+__global__ void k_linear(Streams p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v[NS];
+#pragma unroll
+    for (int f = 0; f < NS; f++) v[f] = __builtin_nontemporal_load(p.in[f] + i);
+#pragma unroll
+    for (int f = 0; f < NS; f++) __builtin_nontemporal_store(v[f] * 1.0001f, p.out[f] + i);
+}
+
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb) { return (b % 8) * (nb / 8) + b / 8; }
+
+// march pattern over an N^3 grid (N a multiple of 64); streams 0..14 are the
+// positions (read with the halo), the rest own-site only
+template <bool HALO>
+__global__ void __launch_bounds__(256) k_march_pattern(Streams p, int N, int DI) {
+    const uint32_t ntk = N / 64, ntj = N / 4;
+    uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t tk = lb % ntk;
+    lb /= ntk;
+    const uint32_t tj = lb % ntj, seg = lb / ntj;
+    const int k = tk * 64 + threadIdx.x, j = tj * 4 + threadIdx.y, tid = threadIdx.y * 64 + threadIdx.x;
+    const int jt = (tj * 4 + N - 1) % N, jb = (tj * 4 + 4) % N;
+    // column cell of this thread (tid < 180): field tid / 12, side, row j0-1 .. j0+4
+    const int cf = tid / 12, cr = tid % 6, cs = (tid / 6) % 2;
+    const int cj = (tj * 4 + N - 1 + cr) % N, ck = cs ? (tk * 64 + 64) % N : (tk * 64 + N - 1) % N;
+    const size_t NN = (size_t)N * N;
+    float acc = 0.f;
+    for (int i = seg * DI; i < seg * DI + DI && i < N; i++) {
+        const size_t e = (size_t)i * NN + (size_t)j * N + k;
+        float v[NS];
+#pragma unroll
+        for (int f = 0; f < NS; f++) v[f] = f < NFP ? p.in[f][e] : __builtin_nontemporal_load(p.in[f] + e);
+        float h = 0.f;
+        if constexpr (HALO) {
+#pragma unroll
+            for (int r = 0; r < 8; r++) {   // rows: wave w takes fields w, w+4, w+8, w+12, top and bottom
+                const int f = threadIdx.y + 4 * (r / 2);
+                if (f < NFP) h += p.in[f][(size_t)i * NN + (size_t)((r & 1) ? jb : jt) * N + k];
+            }
+            if (tid < 180) h += p.in[cf][(size_t)i * NN + (size_t)cj * N + ck];
+        }
+#pragma unroll
+        for (int f = 0; f < NS; f++) __builtin_nontemporal_store(v[f] + 1e-3f * h, p.out[f] + e);
+        acc += h;
+    }
+    if (acc == 1234.5f) p.out[0][0] = acc;
+}
+
+static hipEvent_t ev0, ev1;
+template <class F>
+static double time_ms(F launch, int reps) {
+    launch();
+    HCHK(hipGetLastError());
+    HCHK(hipDeviceSynchronize());
+    HCHK(hipEventRecord(ev0));
+    for (int r = 0; r < reps; r++) launch();
+    HCHK(hipEventRecord(ev1));
+    HCHK(hipEventSynchronize(ev1));
+    float ms = 0.f;
+    HCHK(hipEventElapsedTime(&ms, ev0, ev1));
+    return ms / reps;
+}
+static void line(const char *what, double bytes, double ms) {
+    const double gbs = bytes / ms / 1e6;
+    printf("%-62s %8.3f ms %8.1f GB/s %6.1f %% of 8 TB/s\n", what, ms, gbs, gbs / 80.0);
+    fflush(stdout);
+}
+
+int main(int argc, char **argv) {
+    const int reps = argc > 1 ? atoi(argv[1]) : 10;
+    HCHK(hipEventCreate(&ev0));
+    HCHK(hipEventCreate(&ev1));
+    {
+        const size_t n4 = (size_t)1 << 28;   // 4 GiB of 16-byte vectors each way
+        f4 *s, *d;
+        float *sink;
+        HCHK(hipMalloc(&s, n4 * 16));
+        HCHK(hipMalloc(&d, n4 * 16));
+        HCHK(hipMalloc(&sink, 16));
+        HCHK(hipMemset(s, 0, n4 * 16));
+        const uint32_t nb = (uint32_t)(n4 / 256);
+        line("copy, 16 B per lane, nontemporal (4 GiB each way)", 2.0 * n4 * 16,
+             time_ms([&] { k_copy4<<<nb, 256>>>(s, d, n4); }, reps));
+        line("read only, 16 B per lane (4 GiB)", 1.0 * n4 * 16, time_ms([&] { k_read4<<<nb, 256>>>(s, sink, n4); }, reps));
+        line("write only, 16 B per lane (4 GiB)", 1.0 * n4 * 16, time_ms([&] { k_write4<<<nb, 256>>>(d, n4); }, reps));
+        HCHK(hipFree(s));
+        HCHK(hipFree(d));
+        HCHK(hipFree(sink));
+    }
+    const int N = 512;
+    const size_t n3 = (size_t)N * N * N;
+    const double bytes = 2.0 * NS * 4 * n3;   // 272 B per site
+    for (int stagger = 0; stagger < 2; stagger++) {
+        const size_t fstride = n3 + (stagger ? 1024 : 0);   // words between fields
+        float *slab;
+        HCHK(hipMalloc(&slab, 2 * NS * fstride * 4));
+        HCHK(hipMemset(slab, 0, 2 * NS * fstride * 4));
+        Streams p;
+        for (int f = 0; f < NS; f++) {
+            p.in[f] = slab + f * fstride;
+            p.out[f] = slab + (NS + f) * fstride;
+        }
+        char what[96];
+        snprintf(what, sizeof what, "34 + 34 streams, site-linear, fields %s", stagger ? "staggered by 4 KiB" : "2^27 words apart");
+        line(what, bytes, time_ms([&] { k_linear<<<(uint32_t)(n3 / 256), 256>>>(p, n3); }, reps));
+        if (!stagger) {
+            for (int DI : {128, 32}) {
+                const uint32_t nb = (uint32_t)((N / 64) * (N / 4) * ((N + DI - 1) / DI));
+                snprintf(what, sizeof what, "34 + 34 streams, march pattern 64 x 4 x %d, no halo", DI);
+                line(what, bytes, time_ms([&] { k_march_pattern<false><<<nb, dim3(64, 4)>>>(p, N, DI); }, reps));
+                snprintf(what, sizeof what, "34 + 34 streams, march pattern 64 x 4 x %d, position halo", DI);
+                line(what, bytes, time_ms([&] { k_march_pattern<true><<<nb, dim3(64, 4)>>>(p, N, DI); }, reps));
+            }
+        }
+        HCHK(hipFree(slab));
+    }
+    return 0;
+}
