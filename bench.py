@@ -31,6 +31,7 @@ binary's real-mode 64^3 run as the CPU baseline. It is not the headline line.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--prec f64|f32c]
                        [--lo-words 0|1|2|3|4] [--mode gauged|real] [--no-compare] [--no-cpu-baseline]
+       python bench.py --depths 128,64,32,16   (march depths on the headline grid, one JSON line)
 """
 import argparse
 import json
@@ -204,6 +205,31 @@ def copy_calibration(gib, reps=3):
     dt = (time.time() - t0) / reps
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "reps": reps,
                       "GBps": 2 * n * 4 / dt / 1e9}), flush=True)
+
+
+def depth_sweep(N, depths, lo_words, reps=3, steps=30):
+    """The headline state as a one-grid batch (the batched march runs the same
+    tile code as the single-grid launch) timed at each march depth in turn,
+    reps rounds of alternating depths, FP32C then FP64: one JSON line of mean
+    ms/step per depth."""
+    from scp_amd import grid as G
+    L = 0.25 * (N - 1)
+    st = build_state(N, L)
+    out = {"depths": depths, "N": N, "rounds": reps, "steps_per_timing": steps}
+    for prec in ("f32c", "f64"):
+        params = G.make_params(N, G.MODE_GAUGED, L, 0.2, g=0.05, lo_words=lo_words,
+                               precision=G.PREC_F64 if prec == "f64" else G.PREC_F32C)
+        with G.Grid(params) as g, G.Batch([g]) as b:
+            g.upload(st)
+            res = {d: [] for d in depths}
+            for r in range(reps):
+                for d in depths:
+                    b.set_config(di=d)
+                    b.step(2)
+                    res[d].append(b.time_steps(steps) / steps)
+                    log(f"{prec} round {r} depth {d}: {res[d][-1]:.4f} ms/step")
+            out[prec] = {str(d): sum(v) / len(v) for d, v in res.items()}
+    print(json.dumps(out), flush=True)
 
 
 REAL_COLS = [0, 1, 2, 3, 4, 5, 12, 13, 14, 15, 16, 17]   # complex columns: Re Phi, Re Theta and their velocities
@@ -518,11 +544,17 @@ def main():
     ap.add_argument("--sizes", default="sweep", choices=["sweep", "serve"],
                     help="sweep: edges from {32,48,64,80,96}; serve: edges log-uniform in [48,256]")
     ap.add_argument("--seed", type=int, default=3, help="sweep: seed of sizes and balls")
+    ap.add_argument("--depths", default="",
+                    help="instead of the step: time the headline grid as a one-grid batch at these march "
+                         "depths (planes per block, comma-separated), alternating")
     ap.add_argument("--copy-calib", type=float, default=0.0,
                     help="instead of the step: time a plain device copy of this many GiB (HBM counter calibration)")
     a = ap.parse_args()
     if a.copy_calib > 0:
         copy_calibration(a.copy_calib)
+        return
+    if a.depths:
+        depth_sweep(a.n, [int(d) for d in a.depths.split(",")], a.lo_words)
         return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: the multi-GPU form (slabs along "

@@ -9,9 +9,10 @@
 //     velocity fields and 4 packed compensation-word fields each way), with the
 //     fields at a power-of-two stride (2^27 words) and staggered by 4 KiB;
 //   * the same streams in the step's march pattern: 64 x 4 site tiles marching
-//     along i over DI planes (the step's XCD-aware tile numbering), without and
-//     with the position halo ring the step stages (two rows per tile plane of
-//     every position field, and 180 column cells).
+//     along i over DI planes (the step's XCD-aware tile numbering), at the
+//     step's two blocks per CU and with its one-plane-ahead loads (the step's
+//     halo re-reads come on top of these streams), in the step's field-major
+//     layout and with the rows of each array's fields side by side.
 // Usage: hbm_probe [reps]   (about 40 GB of device memory, 10-20 s)
 #include <hip/hip_runtime.h>
 
@@ -63,42 +64,48 @@ __global__ void k_linear(Streams p, size_t n) {
 
 __device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb) { return (b % 8) * (nb / 8) + b / 8; }
 
-// march pattern over an N^3 grid (N a multiple of 64); streams 0..14 are the
-// positions (read with the halo), the rest own-site only
-template <bool HALO>
-__global__ void __launch_bounds__(256) k_march_pattern(Streams p, int N, int DI) {
+// march pattern over an N^3 grid (N a multiple of 64): each block streams the
+// own sites of one 64 x 4 tile along i; like the step, the loads of plane i+1
+// are issued before the stores of plane i. No halo: the step's halo re-reads
+// come on top of these streams.
+// Strides within a stream (each stream has its own base pointer): row and
+// plane strides of the position and velocity streams (rs, ps) and of the
+// word streams (rsq, psq). Field-major: N and N^2 for all. Row-interleaved
+// (the rows of one array's fields side by side): 15 N and 15 N^2, 4 N and 4 N^2.
+struct Layout {
+    size_t rs, ps, rsq, psq;
+};
+__global__ void __launch_bounds__(256) k_march_pattern(Streams p, int N, int DI, Layout L) {
     const uint32_t ntk = N / 64, ntj = N / 4;
     uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t tk = lb % ntk;
     lb /= ntk;
     const uint32_t tj = lb % ntj, seg = lb / ntj;
-    const int k = tk * 64 + threadIdx.x, j = tj * 4 + threadIdx.y, tid = threadIdx.y * 64 + threadIdx.x;
-    const int jt = (tj * 4 + N - 1) % N, jb = (tj * 4 + 4) % N;
-    // column cell of this thread (tid < 180): field tid / 12, side, row j0-1 .. j0+4
-    const int cf = tid / 12, cr = tid % 6, cs = (tid / 6) % 2;
-    const int cj = (tj * 4 + N - 1 + cr) % N, ck = cs ? (tk * 64 + 64) % N : (tk * 64 + N - 1) % N;
-    const size_t NN = (size_t)N * N;
-    float acc = 0.f;
-    for (int i = seg * DI; i < seg * DI + DI && i < N; i++) {
-        const size_t e = (size_t)i * NN + (size_t)j * N + k;
-        float v[NS];
+    const int k = tk * 64 + threadIdx.x, j = tj * 4 + threadIdx.y;
+    const int i0 = seg * DI, i1 = min(N, i0 + DI);
+    extern __shared__ float lds_reserved[];   // only caps the blocks per CU
+    if (N < 0) lds_reserved[threadIdx.x] = 0.f;
+    auto load = [&](int i, float *v) {
+        const size_t e = (size_t)i * L.ps + (size_t)j * L.rs + k, eq = (size_t)i * L.psq + (size_t)j * L.rsq + k;
 #pragma unroll
-        for (int f = 0; f < NS; f++) v[f] = f < NFP ? p.in[f][e] : __builtin_nontemporal_load(p.in[f] + e);
-        float h = 0.f;
-        if constexpr (HALO) {
+        for (int f = 0; f < NS; f++)
+            v[f] = f < NFP ? p.in[f][e] : __builtin_nontemporal_load(p.in[f] + (f < 2 * NFP ? e : eq));
+    };
+    float v[NS];
+    if (i0 < i1) load(i0, v);
+    for (int i = i0; i < i1; i++) {
+        float vn[NS] = {};
+        if (i + 1 < i1) load(i + 1, vn);
+        const size_t e = (size_t)i * L.ps + (size_t)j * L.rs + k, eq = (size_t)i * L.psq + (size_t)j * L.rsq + k;
 #pragma unroll
-            for (int r = 0; r < 8; r++) {   // rows: wave w takes fields w, w+4, w+8, w+12, top and bottom
-                const int f = threadIdx.y + 4 * (r / 2);
-                if (f < NFP) h += p.in[f][(size_t)i * NN + (size_t)((r & 1) ? jb : jt) * N + k];
-            }
-            if (tid < 180) h += p.in[cf][(size_t)i * NN + (size_t)cj * N + ck];
-        }
+        for (int f = 0; f < NS; f++) __builtin_nontemporal_store(v[f] * 1.0001f, p.out[f] + (f < 2 * NFP ? e : eq));
 #pragma unroll
-        for (int f = 0; f < NS; f++) __builtin_nontemporal_store(v[f] + 1e-3f * h, p.out[f] + e);
-        acc += h;
+        for (int f = 0; f < NS; f++) v[f] = vn[f];
     }
-    if (acc == 1234.5f) p.out[0][0] = acc;
 }
+
+// the FP32C step's staged planes per block: 3 planes x 15 fields x 6 x 66 words
+constexpr size_t LDS_STEP = 3 * 15 * 6 * 66 * 4;
 
 static hipEvent_t ev0, ev1;
 template <class F>
@@ -160,12 +167,37 @@ int main(int argc, char **argv) {
         if (!stagger) {
             for (int DI : {128, 32}) {
                 const uint32_t nb = (uint32_t)((N / 64) * (N / 4) * ((N + DI - 1) / DI));
-                snprintf(what, sizeof what, "34 + 34 streams, march pattern 64 x 4 x %d, no halo", DI);
-                line(what, bytes, time_ms([&] { k_march_pattern<false><<<nb, dim3(64, 4)>>>(p, N, DI); }, reps));
-                snprintf(what, sizeof what, "34 + 34 streams, march pattern 64 x 4 x %d, position halo", DI);
-                line(what, bytes, time_ms([&] { k_march_pattern<true><<<nb, dim3(64, 4)>>>(p, N, DI); }, reps));
+                // LDS reserved as the step's staged planes (71 KB per block): two blocks per CU, as in the step
+                const Layout fm{(size_t)N, (size_t)N * N, (size_t)N, (size_t)N * N};
+                snprintf(what, sizeof what, "34 + 34 streams, march pattern 64 x 4 x %d planes", DI);
+                line(what, bytes, time_ms([&] { k_march_pattern<<<nb, dim3(64, 4), LDS_STEP>>>(p, N, DI, fm); }, reps));
             }
         }
+        HCHK(hipFree(slab));
+    }
+    {
+        // row-interleaved step state: in each array (positions, velocities, words;
+        // in and out apart) the rows of its fields for one (i, j) side by side
+        float *slab;
+        HCHK(hipMalloc(&slab, 2 * NS * n3 * 4));
+        HCHK(hipMemset(slab, 0, 2 * NS * n3 * 4));
+        Streams p;
+        const int first[3] = {0, NFP, 2 * NFP}, count[3] = {NFP, NFP, NQ};
+        size_t base = 0;
+        for (int side = 0; side < 2; side++)
+            for (int a = 0; a < 3; a++) {
+                for (int f = 0; f < count[a]; f++) {
+                    float *q = slab + base + (size_t)f * N;
+                    if (side == 0) p.in[first[a] + f] = q;
+                    else p.out[first[a] + f] = q;
+                }
+                base += (size_t)count[a] * n3;
+            }
+        const Layout ri{(size_t)NFP * N, (size_t)NFP * N * N, (size_t)NQ * N, (size_t)NQ * N * N};
+        const int DI = 32;
+        const uint32_t nb = (uint32_t)((N / 64) * (N / 4) * ((N + DI - 1) / DI));
+        line("34 + 34 streams, march pattern 64 x 4 x 32 planes, row-interleaved", bytes,
+             time_ms([&] { k_march_pattern<<<nb, dim3(64, 4), LDS_STEP>>>(p, N, DI, ri); }, reps));
         HCHK(hipFree(slab));
     }
     return 0;
