@@ -224,8 +224,8 @@ inline int grid_tile_shape(int N) {
     return (l32 * 10 <= l64 * 9 || (l32 <= l64 && N <= 128)) ? TILE_32x8 : TILE_64x4;
 }
 // default planes per block of the march (64*KW x 4 tiles): N halved while the
-// launch has fewer than `target` blocks, down to 16 planes (the 3-plane
-// prologue costs 3/DI extra plane reads). The storage policies of the march
+// launch has fewer than `target` blocks, but never below 16 planes (or N) (the
+// 3-plane prologue costs 3/DI extra plane reads). The storage policies of the march
 // set the target (S64::MARCH_BLOCKS, S32::MARCH_BLOCKS), from the 512^3 gauged
 // step timed on one grid with alternating depths: FP32C 8.43 / 8.14 / 8.01 /
 // 8.07 ms at DI = 128 / 64 / 32 / 16 (4096 ... 32768 blocks), FP64 15.68 /
@@ -234,7 +234,7 @@ inline int march_di(int N, int KW, uint32_t target = 4096) {
     const uint32_t tiles = (uint32_t)((N + 64 * KW - 1) / (64 * KW)) * march_tiles_j(N);
     int di = N;
     while (di > 16 && (uint64_t)tiles * (uint64_t)((N + di - 1) / di) < target / (uint32_t)KW) di /= 2;
-    return di;
+    return di < 16 ? (N < 16 ? N : 16) : di;
 }
 // real-mode pass 1 (positions pin, velocities vel the force sees; same layout)
 int launch_real_aux(const KParams &kp, const double *pin, const double *vel, double *aux, hipStream_t s);
