@@ -103,18 +103,33 @@ template <typename V>
 __device__ inline const V *fbase(const V *p, int f, uint32_t fs) { return p + (size_t)f * fs; }
 template <typename V>
 __device__ inline V *fbase(V *p, int f, uint32_t fs) { return p + (size_t)f * fs; }
+// Every access goes through a global-address-space pointer: the batched
+// launches take their buffer pointers from a descriptor in device memory, and
+// with generic pointers the compiler emits flat loads and stores, which also
+// count against the LDS wait counter, so each wait for an LDS read would wait
+// for the plane-ahead loads as well (the single-grid launch gets global
+// accesses from its kernel arguments either way).
 template <typename V>
-__device__ inline V ldo(const V *base, uint32_t e) {
-    return *(const V *)((const char *)base + e * (uint32_t)sizeof(V));
+using gptr = __attribute__((address_space(1))) V *;
+template <typename V>
+__device__ inline gptr<const V> gaddr(const V *base, uint32_t e) {
+    return (gptr<const V>)((gptr<const char>)(base) + e * (uint32_t)sizeof(V));
 }
 template <typename V>
-__device__ inline V ldo_nt(const V *base, uint32_t e) {
-    return __builtin_nontemporal_load((const V *)((const char *)base + e * (uint32_t)sizeof(V)));
+__device__ inline gptr<V> gaddr(V *base, uint32_t e) {
+    return (gptr<V>)((gptr<char>)(base) + e * (uint32_t)sizeof(V));
 }
 template <typename V>
-__device__ inline void sto_nt(V *base, uint32_t e, V v) {
-    __builtin_nontemporal_store(v, (V *)((char *)base + e * (uint32_t)sizeof(V)));
-}
+__device__ inline V ldo(const V *base, uint32_t e) { return *gaddr(base, e); }
+template <typename V>
+__device__ inline V ldo_nt(const V *base, uint32_t e) { return __builtin_nontemporal_load(gaddr(base, e)); }
+template <typename V>
+__device__ inline void sto_nt(V *base, uint32_t e, V v) { __builtin_nontemporal_store(v, gaddr(base, e)); }
+// element idx of a global array (64-bit index)
+template <typename V>
+__device__ inline gptr<V> gel(V *p, size_t idx) { return (gptr<V>)(p) + idx; }
+template <typename V>
+__device__ inline gptr<const V> gel(const V *p, size_t idx) { return (gptr<const V>)(p) + idx; }
 
 // The halo ring of one plane, spread over the block.
 //   rows (j0-1 and j0+H): 64-site segments (field, side, half), segment
@@ -254,8 +269,8 @@ struct S64 {
         xe = x;
         ve = v;
         if constexpr (EMIT) {
-            __builtin_nontemporal_store(x, &b.xo[(size_t)f * fs + ec]);
-            __builtin_nontemporal_store(v, &b.vo[(size_t)f * fs + ec]);
+            __builtin_nontemporal_store(x, gel(b.xo, (size_t)f * fs + ec));
+            __builtin_nontemporal_store(v, gel(b.vo, (size_t)f * fs + ec));
         }
     }
 };
@@ -347,8 +362,8 @@ struct S32 {
         xe = (double)xh0 + (double)xl0;
         ve = (double)vh + (double)vl;
         if constexpr (EMIT) {
-            __builtin_nontemporal_store(xe, &b.xo[(size_t)f * fs + ec]);
-            __builtin_nontemporal_store(ve, &b.vo[(size_t)f * fs + ec]);
+            __builtin_nontemporal_store(xe, gel(b.xo, (size_t)f * fs + ec));
+            __builtin_nontemporal_store(ve, gel(b.vo, (size_t)f * fs + ec));
         }
     }
 };
@@ -394,7 +409,7 @@ __device__ inline void gauss_pending(const KParams &kp, const DiagArgs *da, Diag
         const double Em[3] = {pg.E0m, ds->ex[((pg.i & 1) * MTJ + row - 1) * 64 + lane], pg.E2m};
         gauss_site(kp, da->dp, pg.E, Em, pg.rho, pg.i, t.j, t.k, ds->S, ds->M);
     } else {
-        __builtin_nontemporal_store(pg.rho, &da->rho[pg.ec]);
+        __builtin_nontemporal_store(pg.rho, gel(da->rho, pg.ec));
     }
 }
 
@@ -439,7 +454,7 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
     };
     auto halo_col = [&](uint32_t pe, T *hc) {
 #pragma unroll
-        for (int c = 0; c < CQ; c++) hc[c] = hs.cf[c] >= 0 ? pin[(size_t)hs.cf[c] * fs + pe + hs.cg[c]] : T(0);
+        for (int c = 0; c < CQ; c++) hc[c] = hs.cf[c] >= 0 ? *gel(pin, (size_t)hs.cf[c] * fs + pe + hs.cg[c]) : T(0);
     };
     auto publish = [&](T *buf, const T *own, const T *hr, const T *hc) {
         if (t.valid) {
@@ -570,7 +585,7 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
                 // site index, which equals the step-state index here (real-mode rows are not
                 // padded: pitch = N)
                 const double *__restrict__ aux = b.aux;
-                const auto AUX = [&](int f, uint32_t idx) -> T { return T(aux[(size_t)f * kp.fs + idx]); };
+                const auto AUX = [&](int f, uint32_t idx) -> T { return T(*gel(aux, (size_t)f * kp.fs + idx)); };
                 // neighbour indices formed where a pass-1 term reads them (not held otherwise)
                 const auto nbi = [&](int d, int sg) -> uint32_t {
                     if (d == 3) return e;
