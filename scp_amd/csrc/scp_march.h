@@ -149,6 +149,7 @@ struct Halo {
     static constexpr int CQ = (NCC + G::MNT - 1) / G::MNT; // of them per thread
     int wave, lane, W;
     uint32_t rtop, rbot;      // in-plane element offsets (padded rows) of this lane's top/bottom halo cell (half 0)
+    uint32_t rtop0, rbot0;    // the same at the tile's first column
     int lrow_top, lrow_bot;   // LDS offsets (field 0, half 0) of this lane's halo row cells
     int cf[CQ];               // column cells: field, or -1
     uint32_t cg[CQ];          // column cells: in-plane element offset
@@ -159,8 +160,10 @@ struct Halo {
         lane = tid & 63;
         W = t.W;
         const int sl = HALF ? (lane & 31) : lane;           // lane within its row segment
-        rtop = (uint32_t)wrapc(t.j0 - 1, N) * pitch + t.k0 + sl;
-        rbot = (uint32_t)wrapc(t.j0 + t.H, N) * pitch + t.k0 + sl;
+        rtop0 = (uint32_t)wrapc(t.j0 - 1, N) * pitch + t.k0;
+        rbot0 = (uint32_t)wrapc(t.j0 + t.H, N) * pitch + t.k0;
+        rtop = rtop0 + sl;
+        rbot = rbot0 + sl;
         lrow_top = sl + 1;
         lrow_bot = (t.H + 1) * G::MC + sl + 1;
 #pragma unroll
@@ -177,6 +180,20 @@ struct Halo {
                     cl[c] = f * G::MPL + r * G::MC + (side ? t.W + 1 : 0);
                 }
             }
+        }
+    }
+    // the load of segment q for every lane: the segment's field and this lane's
+    // offset, in bounds also where the lane or the whole segment is idle (field 0,
+    // the column clamped to the tile)
+    __device__ void seg_any(int q, int &f, uint32_t &g) const {
+        const int sg = wave + G::NW * q;
+        if constexpr (HALF) {
+            f = sg < NSEG ? sg : 0;
+            g = ((lane >> 5) ? rbot0 : rtop0) + (uint32_t)min(lane & 31, W - 1);
+        } else {
+            const int rem = sg % (2 * KW), side = rem / KW, half = rem % KW;
+            f = sg < NSEG ? sg / (2 * KW) : 0;
+            g = (side ? rbot0 : rtop0) + (uint32_t)min(half * 64 + lane, W - 1);
         }
     }
     // segment q of this wave: field, global offset, LDS offset; false if idle
@@ -419,7 +436,8 @@ __device__ inline void gauss_pending(const KParams &kp, const DiagArgs *da, Diag
 // next tile into the same area. DIAG (with EMIT): the emitted state's per-site
 // diagnostics are accumulated into ds, and the gauged charge density is
 // written to da->rho.
-template <class S, int KW, int MODE, bool EMIT, bool DIAG = false, int TJ = MTJ, int BX = MG<KW, TJ>::TKM>
+template <class S, int KW, int MODE, bool EMIT, bool DIAG = false, int TJ = MTJ, int BX = MG<KW, TJ>::TKM,
+          bool UNC = false>
 __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::B &b, int DI, uint32_t tk,
                                            uint32_t tj, uint32_t seg, typename S::T *lds,
                                            const DiagArgs *da = nullptr, DiagState *ds = nullptr) {
@@ -455,6 +473,22 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
     auto halo_col = [&](uint32_t pe, T *hc) {
 #pragma unroll
         for (int c = 0; c < CQ; c++) hc[c] = hs.cf[c] >= 0 ? *gel(pin, (size_t)hs.cf[c] * fs + pe + hs.cg[c]) : T(0);
+    };
+    // the same, loading on every lane and segment (idle ones from in-bounds places
+    // whose values publish does not stage)
+    auto halo_rows_any = [&](uint32_t pe, T *hr) {
+#pragma unroll
+        for (int q = 0; q < RQ; q++) {
+            int f;
+            uint32_t g;
+            hs.seg_any(q, f, g);
+            hr[q] = ld(f, pe + g);
+        }
+    };
+    auto halo_col_any = [&](uint32_t pe, T *hc) {
+#pragma unroll
+        for (int c = 0; c < CQ; c++)
+            hc[c] = *gel(pin, (size_t)(hs.cf[c] < 0 ? 0 : hs.cf[c]) * fs + pe + hs.cg[c]);
     };
     auto publish = [&](T *buf, const T *own, const T *hr, const T *hc) {
         if (t.valid) {
@@ -523,8 +557,24 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
 #pragma unroll
         for (int f = 0; f < NF; f++) o[f] = S::own_load(b, f, fs, e, li);
     };
-    typename S::Own swn[S::WAHEAD ? NF : 1];
-    if constexpr (S::WAHEAD) {
+    // INPLACE (own column read back from LDS, own-site words one plane ahead):
+    // the prefetches land in the registers their consumer reads — plane i+2 into
+    // cp / hr / hc right after cp / hr / hc of plane i+1 are staged, and the
+    // own-site words of plane i+1 into sw after plane i's update — so no
+    // loop-carried copy of a load result (which makes the compiler wait at the
+    // loop end for every memory operation in flight, stores included) is left.
+    // UNC: those loads are issued on every lane and plane (idle ones from
+    // in-bounds places), so no branch around them leaves the compiler unsure
+    // how many memory operations are in flight: it then waits for the staged
+    // plane's loads alone, not for everything. FP32C 512^3 step: 7.35 ms against
+    // 7.55 (INPLACE alone) and 7.75-7.99 (copies); the batched launches keep the
+    // branches (their two inlined marches wait conservatively either way).
+    constexpr bool INPLACE = !S::REGQ && S::WAHEAD;
+    typename S::Own swn[S::WAHEAD && !INPLACE ? NF : 1];
+    typename S::Own swi[INPLACE ? NF : 1];
+    if constexpr (INPLACE) {
+        if (t.valid && t.i0 < t.i1) load_own((uint32_t)t.i0 * NN + col, swi);
+    } else if constexpr (S::WAHEAD) {
         if (t.valid && t.i0 < t.i1) load_own((uint32_t)t.i0 * NN + col, swn);
     }
 
@@ -541,29 +591,51 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
         const uint32_t ec = (uint32_t)i * (uint32_t)N * (uint32_t)N + colc;
         // prefetch plane i+2 (own column + halo) and the own-site velocity words of
         // plane i+1 (WAHEAD: a whole plane of compute to arrive) or of plane i
-        T cpp[NF], hrr[RQ];
-        typename S::Own sw[NF];
-        if constexpr (S::WAHEAD) {
+        T cpp[INPLACE ? 1 : NF], hrr[INPLACE ? 1 : RQ], hcc[INPLACE ? 1 : CQ];
+        typename S::Own swc[INPLACE ? 1 : NF];
+        typename S::Own *sw = INPLACE ? swi : swc;
+        if constexpr (INPLACE) {
+            // plane i+1 into the buffer that held plane i-2, then plane i+2 into its registers
+            publish(lds + nbuf * NF * MPL, cp, hr, hc);
+            if constexpr (UNC) {
+                // on every lane (col is 0, an in-bounds site, where the lane is idle)
 #pragma unroll
-            for (int f = 0; f < NF; f++) sw[f] = swn[f];
-        }
-        if (t.valid) {
-#pragma unroll
-            for (int f = 0; f < NF; f++) cpp[f] = ld(f, p2e + col);
-            if constexpr (S::WAHEAD) {
-                if (i + 1 < t.i1) load_own(e + NN, swn);
+                for (int f = 0; f < NF; f++) cp[f] = ld(f, p2e + col);
+                halo_rows_any(p2e, hr);
+                halo_col_any(p2e, hc);
             } else {
-                load_own(e, sw);
+                if (t.valid) {
+#pragma unroll
+                    for (int f = 0; f < NF; f++) cp[f] = ld(f, p2e + col);
+                } else {
+#pragma unroll
+                    for (int f = 0; f < NF; f++) cp[f] = T(0);
+                }
+                halo_rows(p2e, hr);
+                halo_col(p2e, hc);
             }
         } else {
+            if constexpr (S::WAHEAD) {
 #pragma unroll
-            for (int f = 0; f < NF; f++) cpp[f] = T(0);
+                for (int f = 0; f < NF; f++) swc[f] = swn[f];
+            }
+            if (t.valid) {
+#pragma unroll
+                for (int f = 0; f < NF; f++) cpp[f] = ld(f, p2e + col);
+                if constexpr (S::WAHEAD) {
+                    if (i + 1 < t.i1) load_own(e + NN, swn);
+                } else {
+                    load_own(e, swc);
+                }
+            } else {
+#pragma unroll
+                for (int f = 0; f < NF; f++) cpp[f] = T(0);
+            }
+            halo_rows(p2e, hrr);
+            halo_col(p2e, hcc);
+            // plane i+1 into the buffer that held plane i-2
+            publish(lds + nbuf * NF * MPL, cp, hr, hc);
         }
-        T hcc[CQ];
-        halo_rows(p2e, hrr);
-        halo_col(p2e, hcc);
-        // plane i+1 into the buffer that held plane i-2
-        publish(lds + nbuf * NF * MPL, cp, hr, hc);
         __syncthreads();
         if constexpr (DIAG && MODE == 2) {
             if (i > t.i0) gauss_pending(kp, da, ds, t, pg);
@@ -650,13 +722,22 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
 #pragma unroll
             for (int f = 0; f < NM; f++) cm[f] = c0[f];
             if constexpr (DIAG && MODE == 2) th0m = c0[F_TH];
+            if constexpr (INPLACE && UNC) {
+                // on every lane and plane (the segment's last loads the wrapped next plane, unused)
+                load_own((uint32_t)wp(i, N) * NN + col, swi);
+            } else if constexpr (INPLACE) {
+                if (t.valid && i + 1 < t.i1) load_own(e + NN, swi);
+            } else {
 #pragma unroll
-            for (int f = 0; f < NF; f++) cp[f] = cpp[f];
+                for (int f = 0; f < NF; f++) cp[f] = cpp[f];
+            }
         }
+        if constexpr (!INPLACE) {
 #pragma unroll
-        for (int q = 0; q < RQ; q++) hr[q] = hrr[q];
+            for (int q = 0; q < RQ; q++) hr[q] = hrr[q];
 #pragma unroll
-        for (int c = 0; c < CQ; c++) hc[c] = hcc[c];
+            for (int c = 0; c < CQ; c++) hc[c] = hcc[c];
+        }
         bb = nbuf;
         if constexpr (DIAG && MODE == 2) {
             // this plane's Gauss site waits for the row below's E_1 (LDS, visible after
@@ -686,7 +767,8 @@ k_march(KParams kp, typename S::B b, int DI) {
     uint32_t lb = logical_block(blockIdx.x, gridDim.x);
     const uint32_t tk = lb % ntk; lb /= ntk;
     const uint32_t tj = lb % ntj; lb /= ntj;
-    march_tile<S, KW, MODE, EMIT>(kp, b, DI, tk, tj, lb, lds);
+    // single grid: unconditional prefetch loads (no per-lane branches around them)
+    march_tile<S, KW, MODE, EMIT, false, MTJ, MG<KW>::TKM, true>(kp, b, DI, tk, tj, lb, lds);
 }
 
 // The emitting launch with the diagnostics fused (single grid, KW = 1): each
@@ -711,7 +793,7 @@ k_march_diag(KParams kp, typename S::B b, int DI, DiagArgs da) {
     for (int q = 0; q < DS_NSUM; q++) ds.S[q] = 0.0;
 #pragma unroll
     for (int q = 0; q < DM_NMAX; q++) ds.M[q] = 0.0;
-    march_tile<S, 1, MODE, true, true>(kp, b, DI, tk, tj, lb, lds, &da, &ds);
+    march_tile<S, 1, MODE, true, true, MTJ, MG<1>::TKM, true>(kp, b, DI, tk, tj, lb, lds, &da, &ds);
     // march_tile ended with a barrier, or the whole block returned before staging: lds is free
     double *red = reinterpret_cast<double *>(lds);
     const int tid = threadIdx.y * MG<1>::TKM + threadIdx.x, lane = tid & 63, wv = tid >> 6;
