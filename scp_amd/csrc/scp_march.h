@@ -136,16 +136,31 @@ __device__ inline gptr<const V> gel(const V *p, size_t idx) { return (gptr<const
 //     s = wave + NW*q, lanes along k — a wave-uniform field and a coalesced
 //     row segment per load; with 32-site rows (KW = 0) one load of a wave
 //     covers both rows of field s, lanes 0-31 the top row, 32-63 the bottom;
-//   columns (k0-1 and k0+W, rows j0-1 .. j0+H): (field, side, row) cells
-//     tid, tid + MNT, ... (CQ per thread: one except for the 32 x 8 tile).
-template <int NF, int KW, int TJ = MTJ>
+//   columns (k0-1 and k0+W): (field, side, row) cells tid, tid + MNT, ...
+//     (CQ per thread). Which cells, by COLS:
+//     HALO_ALL: every field on both sides, rows j0-1 .. j0+H (corners
+//       included): the fused diagnostics read any neighbour;
+//     HALO_FACE: every field on both sides, rows j0 .. j0+H-1: the real and
+//       complex forces read the four face neighbours only;
+//     HALO_GAUGE: the gauged forces' reads — at k-1 every field, at k+1 every
+//       field but th_2, and two corners, th_2(j0+H, k0-1) (the diagonal
+//       th_2(j+1, k-1) of the last row) and th_1(j0-1, k0+W) (th_1(j-1, k+1) of
+//       the first row): 118 cells of a 64 x 4 tile-plane instead of 180. Each
+//       cell is a 128-byte line that the neighbouring tile streams; the fewer
+//       lines, the fewer that miss L2 and are read again from memory.
+//   Column cells that are not loaded keep stale values in the staged plane and
+//   are never read.
+enum { HALO_ALL = 0, HALO_FACE = 1, HALO_GAUGE = 2 };
+template <int NF, int KW, int TJ = MTJ, int COLS = HALO_ALL>
 struct Halo {
     using G = MG<KW, TJ>;
     static constexpr int MR = G::MR;
     static constexpr bool HALF = KW == 0;
     static constexpr int NSEG = HALF ? NF : NF * 2 * KW;   // wave loads of halo rows per plane
     static constexpr int RQ = (NSEG + G::NW - 1) / G::NW;  // of them per wave
-    static constexpr int NCC = NF * 2 * MR;                // halo column cells per plane
+    static_assert(COLS != HALO_GAUGE || NF == F_TH + 3, "the gauged cell set names the link fields");
+    // halo column cells per plane
+    static constexpr int NCC = COLS == HALO_ALL ? NF * 2 * MR : (COLS == HALO_FACE ? NF * 2 * TJ : (2 * NF - 1) * TJ + 2);
     static constexpr int CQ = (NCC + G::MNT - 1) / G::MNT; // of them per thread
     int wave, lane, W;
     uint32_t rtop, rbot;      // in-plane element offsets (padded rows) of this lane's top/bottom halo cell (half 0)
@@ -171,8 +186,24 @@ struct Halo {
             const int cell = tid + c * G::MNT;
             cf[c] = -1; cg[c] = 0; cl[c] = 0;
             if (cell < NCC) {
-                const int f = cell / (2 * MR), rem = cell % (2 * MR), side = rem / MR, r = rem % MR;
-                if (r <= t.H + 1) {
+                // field, side (0: k0-1, 1: k0+W) and staged row r (0: j0-1 .. H+1: j0+H)
+                int f, side, r, rmax = t.H;
+                if constexpr (COLS == HALO_ALL) {
+                    const int rem = cell % (2 * MR);
+                    f = cell / (2 * MR); side = rem / MR; r = rem % MR; rmax = t.H + 1;
+                } else {
+                    constexpr int NL = NF * TJ, NRT = (COLS == HALO_GAUGE ? NF - 1 : NF) * TJ;
+                    if (cell < NL) {
+                        side = 0; f = cell / TJ; r = 1 + cell % TJ;
+                    } else if (cell < NL + NRT) {
+                        side = 1; f = (cell - NL) / TJ; r = 1 + (cell - NL) % TJ;   // gauged: fields before th_2
+                    } else if (cell == NL + NRT) {
+                        side = 0; f = F_TH + 2; r = t.H + 1; rmax = t.H + 1;
+                    } else {
+                        side = 1; f = F_TH + 1; r = 0;
+                    }
+                }
+                if (r <= rmax) {
                     cf[c] = f;
                     const int gj = wrapc(t.j0 - 1 + r, N);
                     const int ck = side ? wrapc(t.k0 + t.W, N) : wrapc(t.k0 - 1, N);
@@ -247,6 +278,9 @@ struct LdsNB {
 struct S64 {
     static constexpr int MINB = 1;             // the staged planes fill the LDS: one block per CU
     static constexpr uint32_t MARCH_BLOCKS = 4096;   // single-grid launch: blocks the default depth aims at
+    // tile rows per strip of the single-grid tile order (march_tile_of): row
+    // order; strips of four ran 3 % slower at 512^3 (one block per CU)
+    static constexpr uint32_t TILE_STRIP = 1;
     // own-site words loaded with their plane: one plane ahead (30 more VGPRs) measured
     // 1-6 % slower at 512^3 with one 4-wave block per CU
     static constexpr bool WAHEAD = false;
@@ -321,6 +355,9 @@ struct S32 {
     // single-grid launch: blocks the default depth aims at (512^3: 32 planes per
     // block, 5 % faster than 128; shorter marches keep adjacent tiles closer in step)
     static constexpr uint32_t MARCH_BLOCKS = 16384;
+    // strips of four tile rows in the single-grid tile order (march_tile_of):
+    // 512^3 step -0.6 %, 0.6 GB fewer reads per launch
+    static constexpr uint32_t TILE_STRIP = 4;
     static constexpr bool WAHEAD = true;       // own-site words loaded one plane ahead (+3 % at 512^3)
     static constexpr bool REGQ = false;        // own column of planes i, i+1 read back from LDS
     static constexpr bool SPLIT_DAMP = true;
@@ -445,13 +482,15 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
     constexpr int NF = march_nf<MODE>();
     constexpr int NM = MODE == 2 ? 12 : NF;   // fields of the i-1 plane (gauged: matter only)
     constexpr int MPL = MG<KW, TJ>::MPL;
-    constexpr int RQ = Halo<NF, KW, TJ>::RQ, CQ = Halo<NF, KW, TJ>::CQ;
+    // the halo column cells the launch's reads need (the fused diagnostics: all)
+    using HS = Halo<NF, KW, TJ, DIAG ? HALO_ALL : (MODE == 2 ? HALO_GAUGE : HALO_FACE)>;
+    constexpr int RQ = HS::RQ, CQ = HS::CQ;
     static_assert(!DIAG || (KW == 1 && TJ == MTJ && BX == 64), "the fused diagnostics run on 64 x 4 blocks");
 
     const int N = kp.N;
     const MTile t = mtile<KW, TJ, BX>(N, DI, tk, tj, seg);
     if (t.i0 >= N) return;                    // whole block: uniform
-    Halo<NF, KW, TJ> hs;
+    HS hs;
     hs.init(t, kp.pitch);
     // step state: padded rows (kp.pitch), plane stride kp.plane, field stride kp.fsh;
     // emitted state: unpadded (N, N*N, kp.fs)
@@ -586,7 +625,9 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
         // -1 to -4 % FP64 step time at 512^3).
         asm volatile("" : "+s"(fs));
         const int nbuf = (bb + 1 == MNBUF) ? 0 : bb + 1;
-        const uint32_t p2e = (uint32_t)wp(wp(i, N), N) * NN;
+        // plane i+2; on the segment's last plane (whose prefetch is never staged) plane
+        // i+1 again, loaded the plane before, so the unused loads read lines L2 still holds
+        const uint32_t p2e = (uint32_t)(i + 1 < t.i1 ? wp(wp(i, N), N) : wp(i, N)) * NN;
         const uint32_t e = (uint32_t)i * NN + col;
         const uint32_t ec = (uint32_t)i * (uint32_t)N * (uint32_t)N + colc;
         // prefetch plane i+2 (own column + halo) and the own-site velocity words of
@@ -723,8 +764,8 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
             for (int f = 0; f < NM; f++) cm[f] = c0[f];
             if constexpr (DIAG && MODE == 2) th0m = c0[F_TH];
             if constexpr (INPLACE && UNC) {
-                // on every lane and plane (the segment's last loads the wrapped next plane, unused)
-                load_own((uint32_t)wp(i, N) * NN + col, swi);
+                // on every lane and plane (the segment's last reloads its own plane's words, unused)
+                load_own((uint32_t)(i + 1 < t.i1 ? wp(i, N) : i) * NN + col, swi);
             } else if constexpr (INPLACE) {
                 if (t.valid && i + 1 < t.i1) load_own(e + NN, swi);
             } else {
@@ -758,15 +799,36 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
     }
 }
 
+// Tile (tk, tj) and segment (returned) of logical block lb. Within a segment the
+// tiles go in strips of SH j-adjacent tile rows, j fastest in a strip: a tile's
+// j neighbours in its strip come one block before or after it and its k
+// neighbours SH, so with the XCD-aware numbering (an XCD's blocks take
+// consecutive logical blocks) more of the halo rows a tile loads are lines its
+// neighbours stream into the same L2 at about the same time (with row order,
+// SH = 1, the j neighbours are a whole tile row of blocks apart). Row order when
+// the tile rows do not split into strips of SH.
+template <uint32_t SH>
+__device__ inline uint32_t march_tile_of(uint32_t lb, uint32_t ntk, uint32_t ntj, uint32_t &tk, uint32_t &tj) {
+    const uint32_t t = lb % (ntk * ntj);
+    if (ntj % SH == 0) {
+        const uint32_t r = t % (SH * ntk);
+        tk = r / SH;
+        tj = t / (SH * ntk) * SH + r % SH;
+    } else {
+        tk = t % ntk;
+        tj = t / ntk;
+    }
+    return lb / (ntk * ntj);
+}
+
 // One grid: block b runs tile (tk, tj, seg) of the XCD-aware logical numbering.
 template <class S, int KW, int MODE, bool EMIT>
 __global__ void __launch_bounds__(MG<KW>::MNT, KW == 1 ? S::MINB : 1)
 k_march(KParams kp, typename S::B b, int DI) {
     __shared__ typename S::T lds[march_lds_elems<S, KW, MODE>()];
     const uint32_t ntk = (kp.N + MG<KW>::TKM - 1) / MG<KW>::TKM, ntj = (kp.N + MTJ - 1) / MTJ;
-    uint32_t lb = logical_block(blockIdx.x, gridDim.x);
-    const uint32_t tk = lb % ntk; lb /= ntk;
-    const uint32_t tj = lb % ntj; lb /= ntj;
+    uint32_t tk, tj;
+    const uint32_t lb = march_tile_of<S::TILE_STRIP>(logical_block(blockIdx.x, gridDim.x), ntk, ntj, tk, tj);
     // single grid: unconditional prefetch loads (no per-lane branches around them)
     march_tile<S, KW, MODE, EMIT, false, MTJ, MG<KW>::TKM, true>(kp, b, DI, tk, tj, lb, lds);
 }
@@ -783,9 +845,8 @@ k_march_diag(KParams kp, typename S::B b, int DI, DiagArgs da) {
     __shared__ typename S::T lds[march_lds_elems<S, 1, MODE>()];
     constexpr int NQ = DS_NSUM + DM_NMAX, NW = MG<1>::NW;
     const uint32_t ntk = (kp.N + MG<1>::TKM - 1) / MG<1>::TKM, ntj = (kp.N + MTJ - 1) / MTJ;
-    uint32_t lb = logical_block(blockIdx.x, gridDim.x);
-    const uint32_t tk = lb % ntk; lb /= ntk;
-    const uint32_t tj = lb % ntj; lb /= ntj;
+    uint32_t tk, tj;
+    const uint32_t lb = march_tile_of<S::TILE_STRIP>(logical_block(blockIdx.x, gridDim.x), ntk, ntj, tk, tj);
     __shared__ double ex[MODE == 2 ? 2 * MTJ * 64 : 1];
     DiagState ds;
     ds.ex = ex;

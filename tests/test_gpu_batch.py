@@ -55,6 +55,23 @@ def test_batch_equals_each_grid_alone(prec, lo):
         g.close()
 
 
+@pytest.mark.parametrize("N", [62, 126])
+def test_single_grid_tile_strips_equal_the_batch(N):
+    """The single-grid FP32C launch takes its tiles in strips of four tile rows
+    (j fastest in a strip) when the tile rows split into such strips; on such
+    grids, ragged ones included (62: one 62-wide k tile and a 2-row last j
+    tile; 126: two k tiles, the second 62 wide), its result is bitwise that of
+    the same grid in a batch, whose tiles come from a table."""
+    print(f"case N={N}", flush=True)
+    alone, member = make(N, 300, G.PREC_F32C, 4), make(N, 300, G.PREC_F32C, 4)
+    with G.Batch([member]) as b:
+        alone.step(9)
+        b.step(9)
+        assert np.array_equal(alone.download(), member.download())
+    alone.close()
+    member.close()
+
+
 @pytest.mark.parametrize("di,persistent", [(16, False), (16, True), (128, True), (32, False)])
 def test_batch_launch_configs_agree(di, persistent):
     ref = [make(N, 200 + i, G.PREC_F32C, 1) for i, N in enumerate(SIZES)]
