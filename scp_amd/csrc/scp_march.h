@@ -20,8 +20,10 @@
 //     iteration ahead, so HBM latency is covered by a whole plane of compute.
 //
 // Global traffic per site and step: the state read and written once plus the
-// halo ring (2*64*KW + 2*6 cells per 256*KW sites, read from L2 / MALL: the
-// adjacent tiles stream the same planes at the same time).
+// halo ring (per field, two rows of 64*KW cells and at most 2*6 column cells
+// per 256*KW sites; the step launches load only the column cells their forces
+// read), served by L2 where the adjacent tiles stream the same lines at about
+// the same time.
 //
 // Ragged tiles (N not a multiple of 64 or 4) keep the scheme: the right halo
 // column sits at column W+1 and the bottom halo row at row H+1, where W x H is

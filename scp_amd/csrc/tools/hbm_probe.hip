@@ -12,7 +12,8 @@
 //     along i over DI planes (the step's XCD-aware tile numbering), at the
 //     step's two blocks per CU and with its one-plane-ahead loads (the step's
 //     halo re-reads come on top of these streams), in the step's field-major
-//     layout and with the rows of each array's fields side by side.
+//     layout (also with the fields staggered by 4 KiB) and with the rows of
+//     each array's fields side by side.
 // Usage: hbm_probe [reps]   (about 40 GB of device memory, 10-20 s)
 #include <hip/hip_runtime.h>
 
@@ -164,12 +165,14 @@ int main(int argc, char **argv) {
         char what[96];
         snprintf(what, sizeof what, "34 + 34 streams, site-linear, fields %s", stagger ? "staggered by 4 KiB" : "2^27 words apart");
         line(what, bytes, time_ms([&] { k_linear<<<(uint32_t)(n3 / 256), 256>>>(p, n3); }, reps));
-        if (!stagger) {
+        {
             for (int DI : {128, 32}) {
+                if (stagger && DI != 32) continue;
                 const uint32_t nb = (uint32_t)((N / 64) * (N / 4) * ((N + DI - 1) / DI));
                 // LDS reserved as the step's staged planes (71 KB per block): two blocks per CU, as in the step
                 const Layout fm{(size_t)N, (size_t)N * N, (size_t)N, (size_t)N * N};
-                snprintf(what, sizeof what, "34 + 34 streams, march pattern 64 x 4 x %d planes", DI);
+                snprintf(what, sizeof what, "34 + 34 streams, march pattern 64 x 4 x %d planes%s", DI,
+                         stagger ? ", staggered" : "");
                 line(what, bytes, time_ms([&] { k_march_pattern<<<nb, dim3(64, 4), LDS_STEP>>>(p, N, DI, fm); }, reps));
             }
         }
