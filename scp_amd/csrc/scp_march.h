@@ -9,8 +9,9 @@
 //     (KW = 1 in production: two FP32C blocks or one FP64 block per CU);
 //   * the own column of planes i-1, i, i+1 lives in registers (the march queue);
 //   * plane i of the tile plus a one-cell halo ring (rows j0-1, j0+4, columns
-//     k0-1, k0+64, corners included) is staged in LDS, so the j/k neighbours
-//     and the in-plane link diagonals are LDS reads instead of global loads;
+//     k0-1, k0+64; of the corners, the two link diagonals the gauged forces
+//     read) is staged in LDS, so the j/k neighbours and the in-plane link
+//     diagonals are LDS reads instead of global loads;
 //   * three LDS plane buffers rotate: while plane i is read, plane i+1 (whose
 //     tile and halo are needed for the two staple diagonals th_1(i+1, j-1, k)
 //     and th_2(i+1, j, k-1)) sits beside it, and the buffer of plane i-2 is
@@ -139,30 +140,30 @@ __device__ inline gptr<const V> gel(const V *p, size_t idx) { return (gptr<const
 //     row segment per load; with 32-site rows (KW = 0) one load of a wave
 //     covers both rows of field s, lanes 0-31 the top row, 32-63 the bottom;
 //   columns (k0-1 and k0+W): (field, side, row) cells tid, tid + MNT, ...
-//     (CQ per thread). Which cells, by COLS:
-//     HALO_ALL: every field on both sides, rows j0-1 .. j0+H (corners
-//       included): the fused diagnostics read any neighbour;
+//     (CQ per thread), only those the step's reads need — each cell is a
+//     128-byte line that the neighbouring tile streams, and the fewer lines,
+//     the fewer that miss L2 and are read again from memory:
 //     HALO_FACE: every field on both sides, rows j0 .. j0+H-1: the real and
-//       complex forces read the four face neighbours only;
+//       complex forces, and the diagnostics of every mode, read the four face
+//       neighbours only;
 //     HALO_GAUGE: the gauged forces' reads — at k-1 every field, at k+1 every
 //       field but th_2, and two corners, th_2(j0+H, k0-1) (the diagonal
 //       th_2(j+1, k-1) of the last row) and th_1(j0-1, k0+W) (th_1(j-1, k+1) of
-//       the first row): 118 cells of a 64 x 4 tile-plane instead of 180. Each
-//       cell is a 128-byte line that the neighbouring tile streams; the fewer
-//       lines, the fewer that miss L2 and are read again from memory.
+//       the first row): 118 cells of a 64 x 4 tile-plane instead of the 180 of
+//       the whole ring. The gauged diagnostics' face reads (th_2 at k-1, th_0
+//       and th_1 at k+1 among the links) fall within the same set.
 //   Column cells that are not loaded keep stale values in the staged plane and
 //   are never read.
-enum { HALO_ALL = 0, HALO_FACE = 1, HALO_GAUGE = 2 };
-template <int NF, int KW, int TJ = MTJ, int COLS = HALO_ALL>
+enum { HALO_FACE = 1, HALO_GAUGE = 2 };
+template <int NF, int KW, int TJ, int COLS>
 struct Halo {
     using G = MG<KW, TJ>;
-    static constexpr int MR = G::MR;
     static constexpr bool HALF = KW == 0;
     static constexpr int NSEG = HALF ? NF : NF * 2 * KW;   // wave loads of halo rows per plane
     static constexpr int RQ = (NSEG + G::NW - 1) / G::NW;  // of them per wave
     static_assert(COLS != HALO_GAUGE || NF == F_TH + 3, "the gauged cell set names the link fields");
     // halo column cells per plane
-    static constexpr int NCC = COLS == HALO_ALL ? NF * 2 * MR : (COLS == HALO_FACE ? NF * 2 * TJ : (2 * NF - 1) * TJ + 2);
+    static constexpr int NCC = COLS == HALO_FACE ? NF * 2 * TJ : (2 * NF - 1) * TJ + 2;
     static constexpr int CQ = (NCC + G::MNT - 1) / G::MNT; // of them per thread
     int wave, lane, W;
     uint32_t rtop, rbot;      // in-plane element offsets (padded rows) of this lane's top/bottom halo cell (half 0)
@@ -190,20 +191,15 @@ struct Halo {
             if (cell < NCC) {
                 // field, side (0: k0-1, 1: k0+W) and staged row r (0: j0-1 .. H+1: j0+H)
                 int f, side, r, rmax = t.H;
-                if constexpr (COLS == HALO_ALL) {
-                    const int rem = cell % (2 * MR);
-                    f = cell / (2 * MR); side = rem / MR; r = rem % MR; rmax = t.H + 1;
+                constexpr int NL = NF * TJ, NRT = (COLS == HALO_GAUGE ? NF - 1 : NF) * TJ;
+                if (cell < NL) {
+                    side = 0; f = cell / TJ; r = 1 + cell % TJ;
+                } else if (cell < NL + NRT) {
+                    side = 1; f = (cell - NL) / TJ; r = 1 + (cell - NL) % TJ;   // gauged: fields before th_2
+                } else if (cell == NL + NRT) {
+                    side = 0; f = F_TH + 2; r = t.H + 1; rmax = t.H + 1;
                 } else {
-                    constexpr int NL = NF * TJ, NRT = (COLS == HALO_GAUGE ? NF - 1 : NF) * TJ;
-                    if (cell < NL) {
-                        side = 0; f = cell / TJ; r = 1 + cell % TJ;
-                    } else if (cell < NL + NRT) {
-                        side = 1; f = (cell - NL) / TJ; r = 1 + (cell - NL) % TJ;   // gauged: fields before th_2
-                    } else if (cell == NL + NRT) {
-                        side = 0; f = F_TH + 2; r = t.H + 1; rmax = t.H + 1;
-                    } else {
-                        side = 1; f = F_TH + 1; r = 0;
-                    }
+                    side = 1; f = F_TH + 1; r = 0;
                 }
                 if (r <= rmax) {
                     cf[c] = f;
@@ -484,8 +480,8 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
     constexpr int NF = march_nf<MODE>();
     constexpr int NM = MODE == 2 ? 12 : NF;   // fields of the i-1 plane (gauged: matter only)
     constexpr int MPL = MG<KW, TJ>::MPL;
-    // the halo column cells the launch's reads need (the fused diagnostics: all)
-    using HS = Halo<NF, KW, TJ, DIAG ? HALO_ALL : (MODE == 2 ? HALO_GAUGE : HALO_FACE)>;
+    // the halo column cells the launch's reads need
+    using HS = Halo<NF, KW, TJ, MODE == 2 ? HALO_GAUGE : HALO_FACE>;
     constexpr int RQ = HS::RQ, CQ = HS::CQ;
     static_assert(!DIAG || (KW == 1 && TJ == MTJ && BX == 64), "the fused diagnostics run on 64 x 4 blocks");
 
