@@ -7,11 +7,12 @@
 // number of launches it takes part in, so members of different sizes and
 // different step counts share launches. A chain of L steps is captured once
 // as a HIP graph and replayed (only the descriptors change between chains,
-// copied in before the replay). The segment depth DI and the launch form
-// (one block per tile, or a persistent grid of resident blocks walking
-// contiguous tile ranges) are picked by scp_batch_tune from timed steps of the
-// batch itself; results do not depend on either (the per-site arithmetic and
-// the carried link quantities are the same for every segmentation).
+// copied in before the replay). The tile shape, the segment depth DI and the
+// launch form (one block per tile, or a persistent grid of resident blocks
+// walking contiguous tile ranges) are picked by scp_batch_tune from timed
+// steps of the batch itself; results do not depend on any of them (the
+// per-site arithmetic and the carried link quantities are the same for every
+// segmentation).
 #include "scp_grid.h"
 
 #include <hip/hip_runtime.h>
@@ -381,9 +382,27 @@ extern "C" int scp_batch_tune(scp_batch *b, int *steps_taken) {
     static const int DIS[] = {0, 16, 32, 64, 128};
     const int REPS = 3;
     std::vector<int> one(b->members.size(), 1), reps(b->members.size(), REPS);
-    float best = 1e30f;
-    int bdi = 0, bper = 0, bshape = TILE_64x4, taken = 0;
+    struct Cand {
+        float ms;
+        int shape, per, di;
+    };
+    std::vector<Cand> cands;
+    int taken = 0;
+    const Cand dflt{0.f, auto_shape(b), 0, 0};   // what an unconfigured batch runs
     b->shape_auto = false;      // the candidates below set the shape
+    auto timed = [&](const Cand &c, float &ms) {
+        b->shape = c.shape;
+        b->DI = c.di;
+        b->persistent = c.per;
+        b->stale = true;
+        int rc = run_chain(b, one.data(), false);
+        if (rc) return rc;
+        rc = run_chain(b, reps.data(), false, true);
+        if (rc) return rc;
+        HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        taken += 1 + REPS;
+        return 0;
+    };
     bool uniform = true;     // every member's own shape the same: the mixed launch adds nothing
     for (const scp_grid *g : b->members)
         uniform = uniform && grid_tile_shape(g->p.N) == grid_tile_shape(b->members[0]->p.N);
@@ -391,30 +410,38 @@ extern "C" int scp_batch_tune(scp_batch *b, int *steps_taken) {
         if (!shape_ok(b, shape) || (shape == TILE_MIXED && uniform)) continue;
         for (int per = 0; per < 2; per++) {
             for (int di : DIS) {
-                b->shape = shape;
-                b->DI = di;
-                b->persistent = per;
-                b->stale = true;
-                int rc = run_chain(b, one.data(), false);
-                if (rc) return rc;
-                rc = run_chain(b, reps.data(), false, true);
-                if (rc) return rc;
-                float ms = 0;
-                HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-                taken += 1 + REPS;
-                if (ms < best) {
-                    best = ms;
-                    bdi = di;
-                    bper = per;
-                    bshape = shape;
-                }
+                Cand c{0.f, shape, per, di};
+                if (int rc = timed(c, c.ms)) return rc;
+                cands.push_back(c);
             }
         }
     }
-    b->shape = bshape;
+    // One 3-step chain per candidate is noisy (a 40-grid batch picked a shape
+    // 7 % slower than its default): the three fastest and the default are timed
+    // twice more, in alternating order, and the lowest total wins (the default
+    // on a tie).
+    std::stable_sort(cands.begin(), cands.end(), [](const Cand &x, const Cand &y) { return x.ms < y.ms; });
+    std::vector<Cand> fin{dflt};
+    for (const Cand &c : cands) {
+        if (fin.size() == 4) break;
+        if (c.shape != dflt.shape || c.per != dflt.per || c.di != dflt.di) fin.push_back(c);
+    }
+    std::vector<float> tot(fin.size(), 0.f);
+    for (int round = 0; round < 2; round++) {
+        for (size_t k0 = 0; k0 < fin.size(); k0++) {
+            const size_t k = round ? fin.size() - 1 - k0 : k0;
+            float ms = 0.f;
+            if (int rc = timed(fin[k], ms)) return rc;
+            tot[k] += ms;
+        }
+    }
+    size_t bk = 0;
+    for (size_t k = 1; k < fin.size(); k++)
+        if (tot[k] < tot[bk]) bk = k;
+    b->shape = fin[bk].shape;
     b->shape_auto = false;
-    b->DI = bdi;
-    b->persistent = bper;
+    b->DI = fin[bk].di;
+    b->persistent = fin[bk].per;
     b->stale = true;
     b->tuned = true;
     if (steps_taken) *steps_taken = taken;

@@ -471,8 +471,7 @@ __device__ inline void gauss_pending(const KParams &kp, const DiagArgs *da, Diag
 // next tile into the same area. DIAG (with EMIT): the emitted state's per-site
 // diagnostics are accumulated into ds, and the gauged charge density is
 // written to da->rho.
-template <class S, int KW, int MODE, bool EMIT, bool DIAG = false, int TJ = MTJ, int BX = MG<KW, TJ>::TKM,
-          bool UNC = false>
+template <class S, int KW, int MODE, bool EMIT, bool DIAG = false, int TJ = MTJ, int BX = MG<KW, TJ>::TKM>
 __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::B &b, int DI, uint32_t tk,
                                            uint32_t tj, uint32_t seg, typename S::T *lds,
                                            const DiagArgs *da = nullptr, DiagState *ds = nullptr) {
@@ -600,12 +599,11 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
     // own-site words of plane i+1 into sw after plane i's update — so no
     // loop-carried copy of a load result (which makes the compiler wait at the
     // loop end for every memory operation in flight, stores included) is left.
-    // UNC: those loads are issued on every lane and plane (idle ones from
-    // in-bounds places), so no branch around them leaves the compiler unsure
-    // how many memory operations are in flight: it then waits for the staged
-    // plane's loads alone, not for everything. FP32C 512^3 step: 7.35 ms against
-    // 7.55 (INPLACE alone) and 7.75-7.99 (copies); the batched launches keep the
-    // branches (their two inlined marches wait conservatively either way).
+    // Those loads are issued on every lane and plane (idle ones from in-bounds
+    // places), so no branch around them leaves the compiler unsure how many
+    // memory operations are in flight: it then waits for the staged plane's
+    // loads alone, not for everything. FP32C 512^3 step: 7.35 ms against 7.55
+    // (the loads behind lane branches) and 7.75-7.99 (copies).
     constexpr bool INPLACE = !S::REGQ && S::WAHEAD;
     typename S::Own swn[S::WAHEAD && !INPLACE ? NF : 1];
     typename S::Own swi[INPLACE ? NF : 1];
@@ -636,23 +634,11 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
         if constexpr (INPLACE) {
             // plane i+1 into the buffer that held plane i-2, then plane i+2 into its registers
             publish(lds + nbuf * NF * MPL, cp, hr, hc);
-            if constexpr (UNC) {
-                // on every lane (col is 0, an in-bounds site, where the lane is idle)
+            // on every lane (col is 0, an in-bounds site, where the lane is idle)
 #pragma unroll
-                for (int f = 0; f < NF; f++) cp[f] = ld(f, p2e + col);
-                halo_rows_any(p2e, hr);
-                halo_col_any(p2e, hc);
-            } else {
-                if (t.valid) {
-#pragma unroll
-                    for (int f = 0; f < NF; f++) cp[f] = ld(f, p2e + col);
-                } else {
-#pragma unroll
-                    for (int f = 0; f < NF; f++) cp[f] = T(0);
-                }
-                halo_rows(p2e, hr);
-                halo_col(p2e, hc);
-            }
+            for (int f = 0; f < NF; f++) cp[f] = ld(f, p2e + col);
+            halo_rows_any(p2e, hr);
+            halo_col_any(p2e, hc);
         } else {
             if constexpr (S::WAHEAD) {
 #pragma unroll
@@ -761,11 +747,9 @@ __device__ __forceinline__ void march_tile(const KParams &kp, const typename S::
 #pragma unroll
             for (int f = 0; f < NM; f++) cm[f] = c0[f];
             if constexpr (DIAG && MODE == 2) th0m = c0[F_TH];
-            if constexpr (INPLACE && UNC) {
+            if constexpr (INPLACE) {
                 // on every lane and plane (the segment's last reloads its own plane's words, unused)
                 load_own((uint32_t)(i + 1 < t.i1 ? wp(i, N) : i) * NN + col, swi);
-            } else if constexpr (INPLACE) {
-                if (t.valid && i + 1 < t.i1) load_own(e + NN, swi);
             } else {
 #pragma unroll
                 for (int f = 0; f < NF; f++) cp[f] = cpp[f];
@@ -827,8 +811,7 @@ k_march(KParams kp, typename S::B b, int DI) {
     const uint32_t ntk = (kp.N + MG<KW>::TKM - 1) / MG<KW>::TKM, ntj = (kp.N + MTJ - 1) / MTJ;
     uint32_t tk, tj;
     const uint32_t lb = march_tile_of<S::TILE_STRIP>(logical_block(blockIdx.x, gridDim.x), ntk, ntj, tk, tj);
-    // single grid: unconditional prefetch loads (no per-lane branches around them)
-    march_tile<S, KW, MODE, EMIT, false, MTJ, MG<KW>::TKM, true>(kp, b, DI, tk, tj, lb, lds);
+    march_tile<S, KW, MODE, EMIT, false, MTJ, MG<KW>::TKM>(kp, b, DI, tk, tj, lb, lds);
 }
 
 // The emitting launch with the diagnostics fused (single grid, KW = 1): each
@@ -852,7 +835,7 @@ k_march_diag(KParams kp, typename S::B b, int DI, DiagArgs da) {
     for (int q = 0; q < DS_NSUM; q++) ds.S[q] = 0.0;
 #pragma unroll
     for (int q = 0; q < DM_NMAX; q++) ds.M[q] = 0.0;
-    march_tile<S, 1, MODE, true, true, MTJ, MG<1>::TKM, true>(kp, b, DI, tk, tj, lb, lds, &da, &ds);
+    march_tile<S, 1, MODE, true, true, MTJ, MG<1>::TKM>(kp, b, DI, tk, tj, lb, lds, &da, &ds);
     // march_tile ended with a barrier, or the whole block returned before staging: lds is free
     double *red = reinterpret_cast<double *>(lds);
     const int tid = threadIdx.y * MG<1>::TKM + threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -909,10 +892,15 @@ k_march_batch(const BatchGrid<typename S::B> *__restrict__ grids, const BatchTil
         const uint32_t tk = bt.tile % g.ntk, r = bt.tile / g.ntk;
         const uint32_t tj = r % g.ntj, seg = r / g.ntj;
         const int par = (g.cur0 + q) & 1;
+        // the grid's constants and buffers as locals, read once per tile: through the
+        // descriptor the compiler re-reads them with scalar loads in the plane loop,
+        // and every wait for one of those also waits for the LDS reads in flight
+        const KParams kp = g.kp;
+        const typename S::B bb = g.buf[par];
         if (q == g.nq - 1 && g.emit)
-            march_tile<S, KW, MODE, true, false, TJ>(g.kp, g.buf[par], g.DI, tk, tj, seg, lds);
+            march_tile<S, KW, MODE, true, false, TJ>(kp, bb, g.DI, tk, tj, seg, lds);
         else
-            march_tile<S, KW, MODE, false, false, TJ>(g.kp, g.buf[par], g.DI, tk, tj, seg, lds);
+            march_tile<S, KW, MODE, false, false, TJ>(kp, bb, g.DI, tk, tj, seg, lds);
     }
 }
 
@@ -935,12 +923,14 @@ k_march_batch_mix(const BatchGrid<typename S::B> *__restrict__ grids, const Batc
         const uint32_t tj = r % g.ntj, seg = r / g.ntj;
         const int par = (g.cur0 + q) & 1;
         const bool emit = q == g.nq - 1 && g.emit;
+        const KParams kp = g.kp;                  // locals, as in k_march_batch
+        const typename S::B bb = g.buf[par];
         if (g.tk == 32) {
-            if (emit) march_tile<S, 0, MODE, true, false, 8, 64>(g.kp, g.buf[par], g.DI, tk, tj, seg, lds);
-            else march_tile<S, 0, MODE, false, false, 8, 64>(g.kp, g.buf[par], g.DI, tk, tj, seg, lds);
+            if (emit) march_tile<S, 0, MODE, true, false, 8, 64>(kp, bb, g.DI, tk, tj, seg, lds);
+            else march_tile<S, 0, MODE, false, false, 8, 64>(kp, bb, g.DI, tk, tj, seg, lds);
         } else {
-            if (emit) march_tile<S, 1, MODE, true, false, 4, 64>(g.kp, g.buf[par], g.DI, tk, tj, seg, lds);
-            else march_tile<S, 1, MODE, false, false, 4, 64>(g.kp, g.buf[par], g.DI, tk, tj, seg, lds);
+            if (emit) march_tile<S, 1, MODE, true, false, 4, 64>(kp, bb, g.DI, tk, tj, seg, lds);
+            else march_tile<S, 1, MODE, false, false, 4, 64>(kp, bb, g.DI, tk, tj, seg, lds);
         }
     }
 }
